@@ -1,0 +1,313 @@
+"""MI355X (gfx950) BFS engine: Python side of the C ABI.
+
+Mirrors the reference's host interface (YKMorsy/GPU_Graphs, `1D Partition`):
+
+* ``csr(filename)``       -- csr::csr (include/csr.h:9-23, src/csr.cpp:3-67)
+  with ``row_offset``, ``col_idx``, ``num_nodes``, ``num_edges``,
+  ``print_info()``.
+* ``gpuBFS(graph, source)`` -- gpuBFS::gpuBFS (include/gpuBFS.cuh:9-20,
+  src/gpuBFS.cu:13-116): runs the whole BFS in the constructor and exposes
+  ``host_distance``, ``exec_time`` (ms), ``iteration``,
+  ``total_edges_traversed`` and ``print_distance()``.
+
+Differences that the reference leaves undefined or gets wrong are fixed here:
+unreached vertices are -1 (not INT_MAX), ids are bounds-checked, and the
+statistics are real (the reference's are always 0, gpuBFS_kernels.cu:37,
+gpuBFS.cu:63).  Everything runs through libgpugraphs.so; there is no CPU
+path.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import (GB_AUTO_N, GB_DROP_SELF_LOOPS, GB_KEEP_ORDER, GB_MODE_LEVEL_LOOP, GB_MODE_PERSISTENT, GB_ONE_BASED,
+                   GB_SYMMETRIZE, GBError, Stats, check, lib)
+
+__all__ = [
+    "Graph", "DeviceGraph", "DeviceArray", "csr", "gpuBFS", "GBError", "Stats", "device_count",
+    "exclusive_scan", "GB_AUTO_N", "GB_ONE_BASED", "GB_SYMMETRIZE", "GB_KEEP_ORDER", "GB_DROP_SELF_LOOPS",
+    "GB_MODE_PERSISTENT", "GB_MODE_LEVEL_LOOP",
+]
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def device_count():
+    c = C.c_int(0)
+    check(lib().gb_device_count(C.byref(c)))
+    return c.value
+
+
+class Graph:
+    """Host CSR (int64 offsets, int32 columns) owned by the library."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+
+    # -- constructors ----------------------------------------------------
+    @classmethod
+    def from_file(cls, path, flags=0):
+        h = C.c_void_p()
+        check(lib().gb_graph_load_edgelist(str(path).encode(), flags, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_edges(cls, n, src, dst, flags=0):
+        src = np.ascontiguousarray(src, dtype=np.int32)
+        dst = np.ascontiguousarray(dst, dtype=np.int32)
+        if src.shape != dst.shape:
+            raise ValueError("src and dst differ in length")
+        h = C.c_void_p()
+        check(lib().gb_graph_from_edges(int(n), int(src.size), _ptr(src), _ptr(dst), flags, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_csr(cls, offsets, cols, flags=0):
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        h = C.c_void_p()
+        check(lib().gb_graph_from_csr(int(offsets.size - 1), _ptr(offsets), _ptr(cols), flags, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def rmat(cls, scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, permute=True):
+        h = C.c_void_p()
+        check(lib().gb_graph_generate_rmat(scale, edge_factor, a, b, c, seed, int(permute), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def erdos_renyi(cls, n_candidates, n_isolated, m_undirected, seed=1):
+        h = C.c_void_p()
+        check(lib().gb_graph_generate_er(n_candidates, n_isolated, m_undirected, seed, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def grid(cls, width, height, extra_fraction=0.05, seed=1):
+        h = C.c_void_p()
+        check(lib().gb_graph_generate_grid(width, height, extra_fraction, seed, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def load_binary(cls, path):
+        h = C.c_void_p()
+        check(lib().gb_graph_load_binary(str(path).encode(), C.byref(h)))
+        return cls(h.value)
+
+    # -- accessors -------------------------------------------------------
+    @property
+    def n(self):
+        n = C.c_int64()
+        check(lib().gb_graph_info(self._h, C.byref(n), None))
+        return n.value
+
+    @property
+    def m(self):
+        m = C.c_int64()
+        check(lib().gb_graph_info(self._h, None, C.byref(m)))
+        return m.value
+
+    @property
+    def offsets(self):
+        """Borrowed int64[n+1] view (valid while this Graph lives)."""
+        o, c = C.c_void_p(), C.c_void_p()
+        check(lib().gb_graph_csr(self._h, C.byref(o), C.byref(c)))
+        arr = np.ctypeslib.as_array(C.cast(o, C.POINTER(C.c_int64)), shape=(self.n + 1,))
+        arr.flags.writeable = False
+        return arr
+
+    @property
+    def cols(self):
+        """Borrowed int32[m] view (valid while this Graph lives)."""
+        o, c = C.c_void_p(), C.c_void_p()
+        check(lib().gb_graph_csr(self._h, C.byref(o), C.byref(c)))
+        m = self.m
+        if m == 0:
+            return np.zeros(0, dtype=np.int32)
+        arr = np.ctypeslib.as_array(C.cast(c, C.POINTER(C.c_int32)), shape=(m,))
+        arr.flags.writeable = False
+        return arr
+
+    def degrees(self):
+        return np.diff(self.offsets)
+
+    def write_edgelist(self, path):
+        check(lib().gb_graph_write_edgelist(self._h, str(path).encode()))
+
+    def save_binary(self, path):
+        check(lib().gb_graph_save_binary(self._h, str(path).encode()))
+
+    def free(self):
+        if self._h is not None and self._h.value:
+            lib().gb_graph_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """Plain device allocation (gb_dmalloc) of a numpy dtype and length."""
+
+    def __init__(self, device, dtype, count):
+        self.device = device
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        p = C.c_void_p()
+        check(lib().gb_dmalloc(device, self.dtype.itemsize * max(self.count, 1), C.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_numpy(cls, device, arr):
+        arr = np.ascontiguousarray(arr)
+        d = cls(device, arr.dtype, arr.size)
+        d.upload(arr)
+        return d
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        if arr.size != self.count:
+            raise ValueError("size mismatch")
+        check(lib().gb_memcpy_htod(self.device, self.ptr, _ptr(arr), arr.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.count, dtype=self.dtype)
+        check(lib().gb_memcpy_dtoh(self.device, _ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            lib().gb_dfree(self.device, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceGraph:
+    """A graph resident in the HBM of one GPU."""
+
+    def __init__(self, graph, device=0):
+        h = C.c_void_p()
+        check(lib().gb_device_upload(graph._h, device, C.byref(h)))
+        self._h = h
+        self.device = device
+        self.n = graph.n
+        self.m = graph.m
+
+    def config(self, block=256, edges_per_thread=4, blocks_per_cu=0, mode=GB_MODE_PERSISTENT, solo_edges=0):
+        check(lib().gb_bfs_config(self._h, block, edges_per_thread, blocks_per_cu, mode, solo_edges))
+
+    def get_config(self):
+        b, k, bpc, mode = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        solo = C.c_int64()
+        check(lib().gb_bfs_get_config(self._h, C.byref(b), C.byref(k), C.byref(bpc), C.byref(mode), C.byref(solo)))
+        return {"block": b.value, "edges_per_thread": k.value, "blocks_per_cu": bpc.value, "mode": mode.value,
+                "solo_edges": solo.value}
+
+    def memory_bytes(self):
+        b = C.c_int64()
+        check(lib().gb_dgraph_memory(self._h, C.byref(b)))
+        return b.value
+
+    def bfs(self, source, parents=False):
+        """Host-pointer BFS: returns (levels, parents or None, stats dict)."""
+        levels = np.empty(self.n, dtype=np.int32)
+        par = np.empty(self.n, dtype=np.int32) if parents else None
+        st = Stats()
+        check(lib().gb_bfs(self._h, int(source), _ptr(levels), _ptr(par), C.byref(st)))
+        return levels, par, st.as_dict()
+
+    def bfs_device(self, source, d_levels, d_parents=None):
+        """Device-pointer BFS: results stay in HBM (DeviceArray outputs)."""
+        st = Stats()
+        check(lib().gb_bfs_device(self._h, int(source), d_levels.ptr, d_parents.ptr if d_parents else None,
+                                  C.byref(st)))
+        return st.as_dict()
+
+    def free(self):
+        if self._h is not None and self._h.value:
+            lib().gb_device_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def exclusive_scan(values, device=0):
+    """Device-wide exclusive int32 scan (gb_exclusive_scan_i32) of a host array."""
+    values = np.ascontiguousarray(values, dtype=np.int32)
+    d = DeviceArray.from_numpy(device, values)
+    check(lib().gb_exclusive_scan_i32(device, d.ptr, d.ptr, values.size))
+    return d.numpy()
+
+
+class csr:
+    """Mirror of the reference's ``csr`` class (1D Partition/include/csr.h:9-23).
+
+    ``csr(filename)`` parses the reference edge-list format.  Keyword flags:
+    ``one_based`` (MIG/2D loaders), ``symmetrize``, ``keep_order`` (file order
+    and duplicates, exactly the reference's layout), ``auto_n`` (grow n to the
+    largest id + 1).  ``row_offset`` is int64
+    and ``col_idx`` int32.
+    """
+
+    def __init__(self, filename=None, one_based=False, symmetrize=False, keep_order=False, auto_n=False,
+                 graph=None):
+        if graph is None:
+            flags = (GB_ONE_BASED if one_based else 0) | (GB_SYMMETRIZE if symmetrize else 0) | \
+                    (GB_KEEP_ORDER if keep_order else 0) | (GB_AUTO_N if auto_n else 0)
+            graph = Graph.from_file(filename, flags)
+        self.graph = graph
+        self.row_offset = graph.offsets
+        self.col_idx = graph.cols
+        self.num_nodes = graph.n
+        self.num_edges = graph.m
+        self._device_graphs = {}
+
+    def device_graph(self, device=0):
+        if device not in self._device_graphs:
+            self._device_graphs[device] = DeviceGraph(self.graph, device)
+        return self._device_graphs[device]
+
+    def print_info(self):
+        print("\n------CSR INFO------")
+        print(f"Column indices: {self.num_edges}")
+        print(" | ".join(str(int(x)) for x in self.col_idx) + " | ")
+        print(f"Row offset: {self.num_nodes}")
+        print(" | ".join(str(int(x)) for x in self.row_offset) + " | ")
+
+
+class gpuBFS:
+    """Mirror of the reference's ``gpuBFS`` class (1D Partition/include/gpuBFS.cuh:9-20).
+
+    The constructor runs the whole BFS on one GPU (the graph is uploaded on
+    first use and kept resident).  ``host_distance`` holds int32 levels, -1
+    for unreached; ``exec_time`` is device time in ms; ``iteration`` is the
+    number of BFS levels; ``total_edges_traversed`` the CSR entries scanned.
+    """
+
+    def __init__(self, graph, source, device=0, parents=False):
+        dg = graph.device_graph(device)
+        levels, par, st = dg.bfs(source, parents=parents)
+        self.host_distance = levels
+        self.parents = par
+        self.exec_time = st["ms"]
+        self.iteration = st["num_levels"]
+        self.total_edges_traversed = st["m_reached"]
+        self.vertices_reached = st["n_reached"]
+
+    def print_distance(self, graph=None):
+        print("\n------GPU DISTANCE VECTOR------")
+        print(" | ".join(str(int(x)) for x in self.host_distance) + " | ")

@@ -1,0 +1,122 @@
+"""ctypes binding of libgpugraphs.so (the C ABI in include/gpugraphs/gb.h).
+
+The shared library is built in-tree by gpu_graphs_amd/build.py.  There is no
+fallback: if the library is missing, importing the binding raises.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgpugraphs.so")
+
+GB_OK = 0
+GB_EINVAL = -1
+GB_EIO = -2
+GB_ENOMEM = -3
+GB_EDEVICE = -4
+GB_ERANGE = -5
+GB_EBUSY = -6
+GB_EAGAIN = -7
+
+GB_ONE_BASED = 1
+GB_SYMMETRIZE = 2
+GB_KEEP_ORDER = 4
+GB_DROP_SELF_LOOPS = 8
+GB_AUTO_N = 16
+
+GB_MODE_PERSISTENT = 0
+GB_MODE_LEVEL_LOOP = 1
+
+_NAMES = {
+    GB_EINVAL: "GB_EINVAL", GB_EIO: "GB_EIO", GB_ENOMEM: "GB_ENOMEM", GB_EDEVICE: "GB_EDEVICE",
+    GB_ERANGE: "GB_ERANGE", GB_EBUSY: "GB_EBUSY", GB_EAGAIN: "GB_EAGAIN",
+}
+
+
+class GBError(RuntimeError):
+    """A GB_E* status returned by the library, with gb_last_error()'s text."""
+
+    def __init__(self, code, message):
+        super().__init__(f"{_NAMES.get(code, code)}: {message}")
+        self.code = code
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("ms", C.c_double),
+        ("kernel_ms", C.c_double),
+        ("num_levels", C.c_int32),
+        ("reserved", C.c_int32),
+        ("n_reached", C.c_int64),
+        ("m_reached", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+P = C.c_void_p
+PP = C.POINTER(C.c_void_p)
+i32p = C.POINTER(C.c_int32)
+i64p = C.POINTER(C.c_int64)
+
+# name -> (restype, argtypes).  Mirrors include/gpugraphs/gb.h.
+SIGNATURES = {
+    "gb_last_error": (C.c_char_p, []),
+    "gb_version": (C.c_char_p, []),
+    "gb_graph_load_edgelist": (C.c_int, [C.c_char_p, C.c_uint32, PP]),
+    "gb_graph_from_edges": (C.c_int, [C.c_int64, C.c_int64, P, P, C.c_uint32, PP]),
+    "gb_graph_from_csr": (C.c_int, [C.c_int64, P, P, C.c_uint32, PP]),
+    "gb_graph_generate_rmat": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                         C.c_uint64, C.c_int, PP]),
+    "gb_graph_generate_er": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, C.c_uint64, PP]),
+    "gb_graph_generate_grid": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_uint64, PP]),
+    "gb_graph_info": (C.c_int, [P, i64p, i64p]),
+    "gb_graph_csr": (C.c_int, [P, C.POINTER(P), C.POINTER(P)]),
+    "gb_graph_write_edgelist": (C.c_int, [P, C.c_char_p]),
+    "gb_graph_save_binary": (C.c_int, [P, C.c_char_p]),
+    "gb_graph_load_binary": (C.c_int, [C.c_char_p, PP]),
+    "gb_graph_free": (None, [P]),
+    "gb_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "gb_device_upload": (C.c_int, [P, C.c_int, PP]),
+    "gb_dgraph_info": (C.c_int, [P, i64p, i64p, C.POINTER(C.c_int)]),
+    "gb_dgraph_memory": (C.c_int, [P, i64p]),
+    "gb_device_free": (None, [P]),
+    "gb_bfs": (C.c_int, [P, C.c_int32, P, P, C.POINTER(Stats)]),
+    "gb_bfs_device": (C.c_int, [P, C.c_int32, P, P, C.POINTER(Stats)]),
+    "gb_bfs_config": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64]),
+    "gb_bfs_get_config": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int), i64p]),
+    "gb_exclusive_scan_i32": (C.c_int, [C.c_int, P, P, C.c_int64]),
+    "gb_dmalloc": (C.c_int, [C.c_int, C.c_size_t, PP]),
+    "gb_dfree": (C.c_int, [C.c_int, P]),
+    "gb_memcpy_htod": (C.c_int, [C.c_int, P, P, C.c_size_t]),
+    "gb_memcpy_dtoh": (C.c_int, [C.c_int, P, P, C.c_size_t]),
+    "gb_device_synchronize": (C.c_int, [C.c_int]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libgpugraphs.so once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m gpu_graphs_amd.build` "
+                "(there is no CPU fallback)")
+        h = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def check(status):
+    if status != GB_OK:
+        msg = lib().gb_last_error()
+        raise GBError(status, msg.decode() if msg else "")
+    return status

@@ -1,0 +1,339 @@
+// extern "C" entry points declared in include/gpugraphs/gb.h.  Every call
+// catches library exceptions, records the message for gb_last_error() and
+// returns the status; nothing throws across the ABI and nothing exits.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <new>
+
+#include "common.hpp"
+#include "device_graph.hpp"
+#include "host_graph.hpp"
+
+struct gb_graph {
+    gg::HostGraph g;
+};
+struct gb_dgraph {
+    std::unique_ptr<gg::DeviceGraph> d;
+};
+
+namespace gg {
+
+namespace {
+thread_local std::string t_last_error;
+}
+void set_last_error(const std::string& msg) { t_last_error = msg; }
+const char* last_error() { return t_last_error.c_str(); }
+
+bool bfs_config_supported(int block, int k);
+
+template <class F>
+int guarded(F&& fn) {
+    try {
+        fn();
+        return GB_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return GB_ENOMEM;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return GB_EINVAL;
+    } catch (...) {
+        set_last_error("unknown error");
+        return GB_EINVAL;
+    }
+}
+
+void need(bool ok, const char* what) {
+    if (!ok) fail(GB_EINVAL, what);
+}
+
+void check_device_ptr(int device, const void* p, const char* what) {
+    hipPointerAttribute_t attr;
+    hipError_t e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail(GB_EINVAL, std::string(what) + " is not a device pointer");
+    }
+    if (attr.type != hipMemoryTypeDevice && attr.type != hipMemoryTypeManaged)
+        fail(GB_EINVAL, std::string(what) + " is not device memory");
+    if (attr.device != device) fail(GB_EINVAL, std::string(what) + " lives on another device");
+}
+
+}  // namespace gg
+
+using gg::guarded;
+using gg::need;
+
+extern "C" {
+
+const char* gb_last_error(void) { return gg::last_error(); }
+const char* gb_version(void) { return "gpugraphs-mi355x 0.1 (gfx950)"; }
+
+int gb_graph_load_edgelist(const char* path, uint32_t flags, gb_graph** out) {
+    return guarded([&] {
+        need(path && out, "null argument");
+        *out = nullptr;
+        auto h = std::make_unique<gb_graph>();
+        h->g = gg::load_edgelist(path, flags);
+        *out = h.release();
+    });
+}
+
+int gb_graph_from_edges(int64_t n, int64_t m, const int32_t* src, const int32_t* dst, uint32_t flags,
+                        gb_graph** out) {
+    return guarded([&] {
+        need(out && (m == 0 || (src && dst)), "null argument");
+        *out = nullptr;
+        auto h = std::make_unique<gb_graph>();
+        h->g = gg::build_csr(n, src, dst, m, flags);
+        *out = h.release();
+    });
+}
+
+int gb_graph_from_csr(int64_t n, const int64_t* offsets, const int32_t* cols, uint32_t flags, gb_graph** out) {
+    return guarded([&] {
+        need(out && offsets, "null argument");
+        *out = nullptr;
+        need(n >= 0 && n <= INT32_MAX, "vertex count out of range");
+        need(offsets[0] == 0, "offsets[0] must be 0");
+        for (int64_t v = 0; v < n; ++v) need(offsets[v + 1] >= offsets[v], "offsets must be non-decreasing");
+        const int64_t m = offsets[n];
+        need(m == 0 || cols, "null cols");
+        for (int64_t i = 0; i < m; ++i)
+            if (cols[i] < 0 || cols[i] >= n) gg::fail(GB_ERANGE, "column id outside [0, n)");
+        auto h = std::make_unique<gb_graph>();
+        h->g.n = n;
+        h->g.flags = flags;
+        h->g.offsets.assign(offsets, offsets + n + 1);
+        h->g.cols.assign(cols, cols + m);
+        if (flags & GB_SYMMETRIZE) {
+            // Rebuild from the edge list so that reverse edges are added.
+            std::vector<int32_t> s(m), d(h->g.cols);
+            for (int64_t v = 0; v < n; ++v)
+                for (int64_t j = offsets[v]; j < offsets[v + 1]; ++j) s[j] = static_cast<int32_t>(v);
+            h->g = gg::build_csr(n, s.data(), d.data(), m, flags);
+        } else if (!(flags & GB_KEEP_ORDER)) {
+            gg::sort_dedup_rows(h->g, flags & GB_DROP_SELF_LOOPS);
+        }
+        *out = h.release();
+    });
+}
+
+int gb_graph_generate_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed, int permute_ids,
+                           gb_graph** out) {
+    return guarded([&] {
+        need(out, "null argument");
+        *out = nullptr;
+        auto h = std::make_unique<gb_graph>();
+        h->g = gg::generate_rmat(scale, edge_factor, a, b, c, seed, permute_ids != 0);
+        *out = h.release();
+    });
+}
+
+int gb_graph_generate_er(int64_t n_candidates, int64_t n_isolated, int64_t m_undirected, uint64_t seed,
+                         gb_graph** out) {
+    return guarded([&] {
+        need(out, "null argument");
+        *out = nullptr;
+        auto h = std::make_unique<gb_graph>();
+        h->g = gg::generate_er(n_candidates, n_isolated, m_undirected, seed);
+        *out = h.release();
+    });
+}
+
+int gb_graph_generate_grid(int64_t width, int64_t height, double extra_fraction, uint64_t seed, gb_graph** out) {
+    return guarded([&] {
+        need(out, "null argument");
+        *out = nullptr;
+        auto h = std::make_unique<gb_graph>();
+        h->g = gg::generate_grid(width, height, extra_fraction, seed);
+        *out = h.release();
+    });
+}
+
+int gb_graph_info(const gb_graph* g, int64_t* n, int64_t* m) {
+    return guarded([&] {
+        need(g, "null graph");
+        if (n) *n = g->g.n;
+        if (m) *m = g->g.m();
+    });
+}
+
+int gb_graph_csr(const gb_graph* g, const int64_t** offsets, const int32_t** cols) {
+    return guarded([&] {
+        need(g, "null graph");
+        if (offsets) *offsets = g->g.offsets.data();
+        if (cols) *cols = g->g.cols.data();
+    });
+}
+
+int gb_graph_write_edgelist(const gb_graph* g, const char* path) {
+    return guarded([&] {
+        need(g && path, "null argument");
+        gg::write_edgelist(g->g, path);
+    });
+}
+
+int gb_graph_save_binary(const gb_graph* g, const char* path) {
+    return guarded([&] {
+        need(g && path, "null argument");
+        gg::save_binary(g->g, path);
+    });
+}
+
+int gb_graph_load_binary(const char* path, gb_graph** out) {
+    return guarded([&] {
+        need(path && out, "null argument");
+        *out = nullptr;
+        auto h = std::make_unique<gb_graph>();
+        h->g = gg::load_binary(path);
+        *out = h.release();
+    });
+}
+
+void gb_graph_free(gb_graph* g) { delete g; }
+
+int gb_device_count(int* count) {
+    return guarded([&] {
+        need(count, "null argument");
+        int c = 0;
+        hipError_t e = hipGetDeviceCount(&c);
+        if (e == hipErrorNoDevice) {
+            (void)hipGetLastError();
+            c = 0;
+        } else if (e != hipSuccess) {
+            (void)hipGetLastError();
+            gg::fail(GB_EDEVICE, std::string("hipGetDeviceCount failed: ") + hipGetErrorString(e));
+        }
+        *count = c;
+    });
+}
+
+int gb_device_upload(const gb_graph* g, int device, gb_dgraph** out) {
+    return guarded([&] {
+        need(g && out, "null argument");
+        *out = nullptr;
+        auto h = std::make_unique<gb_dgraph>();
+        h->d = std::make_unique<gg::DeviceGraph>(g->g, device);
+        *out = h.release();
+    });
+}
+
+int gb_dgraph_info(const gb_dgraph* dg, int64_t* n, int64_t* m, int* device) {
+    return guarded([&] {
+        need(dg, "null device graph");
+        if (n) *n = dg->d->n;
+        if (m) *m = dg->d->m;
+        if (device) *device = dg->d->device;
+    });
+}
+
+int gb_dgraph_memory(const gb_dgraph* dg, int64_t* bytes) {
+    return guarded([&] {
+        need(dg && bytes, "null argument");
+        *bytes = static_cast<int64_t>(dg->d->memory_bytes());
+    });
+}
+
+void gb_device_free(gb_dgraph* dg) { delete dg; }
+
+int gb_bfs_device(gb_dgraph* dg, int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats* st) {
+    return guarded([&] {
+        need(dg, "null device graph");
+        need(d_levels, "null levels buffer");
+        gg::check_device_ptr(dg->d->device, d_levels, "levels");
+        if (d_parents) gg::check_device_ptr(dg->d->device, d_parents, "parents");
+        dg->d->bfs(source, d_levels, d_parents, st);
+    });
+}
+
+int gb_bfs(gb_dgraph* dg, int32_t source, int32_t* levels, int32_t* parents, gb_stats* st) {
+    return guarded([&] {
+        need(dg, "null device graph");
+        need(levels, "null levels buffer");
+        dg->d->bfs_to_host(source, levels, parents, st);
+    });
+}
+
+int gb_bfs_config(gb_dgraph* dg, int block, int edges_per_thread, int blocks_per_cu, int mode, int64_t solo_edges) {
+    return guarded([&] {
+        need(dg, "null device graph");
+        need(gg::bfs_config_supported(block, edges_per_thread), "unsupported block / edges_per_thread combination");
+        need(blocks_per_cu >= 0, "blocks_per_cu must be >= 0");
+        need(mode == GB_MODE_PERSISTENT || mode == GB_MODE_LEVEL_LOOP, "unknown mode");
+        need(solo_edges >= 0, "solo_edges must be >= 0");
+        gg::BfsConfig& c = dg->d->cfg;
+        c.block = block;
+        c.edges_per_thread = edges_per_thread;
+        c.blocks_per_cu = blocks_per_cu;
+        c.mode = mode;
+        c.solo_edges = solo_edges;
+    });
+}
+
+int gb_bfs_get_config(const gb_dgraph* dg, int* block, int* edges_per_thread, int* blocks_per_cu, int* mode,
+                      int64_t* solo_edges) {
+    return guarded([&] {
+        need(dg, "null device graph");
+        const gg::BfsConfig& c = dg->d->cfg;
+        if (block) *block = c.block;
+        if (edges_per_thread) *edges_per_thread = c.edges_per_thread;
+        if (blocks_per_cu) *blocks_per_cu = c.blocks_per_cu;
+        if (mode) *mode = c.mode;
+        if (solo_edges) *solo_edges = c.solo_edges;
+    });
+}
+
+int gb_exclusive_scan_i32(int device, const int32_t* d_in, int32_t* d_out, int64_t n) {
+    return guarded([&] {
+        if (n > 0) {
+            gg::check_device_ptr(device, d_in, "scan input");
+            gg::check_device_ptr(device, d_out, "scan output");
+        }
+        gg::exclusive_scan_i32(device, d_in, d_out, n);
+    });
+}
+
+int gb_dmalloc(int device, size_t bytes, void** ptr) {
+    return guarded([&] {
+        need(ptr, "null argument");
+        *ptr = nullptr;
+        gg::DeviceGuard guard(device);
+        GG_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+    });
+}
+
+int gb_dfree(int device, void* ptr) {
+    return guarded([&] {
+        gg::DeviceGuard guard(device);
+        if (ptr) GG_HIP(hipFree(ptr));
+    });
+}
+
+int gb_memcpy_htod(int device, void* dst, const void* src, size_t bytes) {
+    return guarded([&] {
+        gg::DeviceGuard guard(device);
+        if (bytes) GG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    });
+}
+
+int gb_memcpy_dtoh(int device, void* dst, const void* src, size_t bytes) {
+    return guarded([&] {
+        gg::DeviceGuard guard(device);
+        if (bytes) GG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+int gb_device_synchronize(int device) {
+    return guarded([&] {
+        gg::DeviceGuard guard(device);
+        GG_HIP(hipDeviceSynchronize());
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+// Device-resident graph and the single-source BFS driver.
+//
+// Replaces gpuBFS (1D Partition/include/gpuBFS.cuh:9-54, gpuBFS.cu:13-208).
+// The reference splits vertices over NVSHMEM PEs and copies a CSR slice to
+// each (init_device, gpuBFS.cu:139-185); here the whole graph lives in the HBM
+// of one GPU and the device buffers are allocated once per handle, not per
+// BFS.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "common.hpp"
+#include "host_graph.hpp"
+
+namespace gg {
+
+#define GG_HIP(call)                                                                                        \
+    do {                                                                                                    \
+        hipError_t _e = (call);                                                                             \
+        if (_e != hipSuccess)                                                                               \
+            ::gg::fail(_e == hipErrorOutOfMemory ? GB_ENOMEM : GB_EDEVICE,                                  \
+                       std::string(#call) + " failed: " + hipGetErrorString(_e) + " (" + __FILE__ + ":" + \
+                           std::to_string(__LINE__) + ")");                                                 \
+    } while (0)
+
+// Makes `device` current for the lifetime of the guard.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int device) {
+        GG_HIP(hipGetDevice(&prev));
+        if (prev != device) GG_HIP(hipSetDevice(device));
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+// Tunables of the single-source top-down kernel.
+struct BfsConfig {
+    int block = 256;              // threads per workgroup (4 waves)
+    int edges_per_thread = 4;     // K: a tile is block * K frontier edges
+    int blocks_per_cu = 0;        // 0 = occupancy maximum
+    int mode = GB_MODE_PERSISTENT;
+    int64_t solo_edges = 0;       // frontiers with <= this many edges run on one workgroup
+};
+
+// Device workspace of one BFS (reused across calls on the same handle).
+struct BfsWorkspace {
+    uint32_t* visited = nullptr;          // ceil(n/32) words
+    int64_t* q_rb[2] = {nullptr, nullptr};    // frontier: first CSR entry of the vertex
+    uint32_t* q_eoff[2] = {nullptr, nullptr}; // frontier: exclusive prefix of degrees in the level
+    int32_t* q_v[2] = {nullptr, nullptr};     // frontier: vertex id (for parents)
+    uint32_t* tile_start[2] = {nullptr, nullptr};  // frontier entry that owns edge t*T
+    unsigned long long* ctl = nullptr;    // counters ring, barrier, stats (see bfs.hip)
+    int32_t* levels = nullptr;            // levels staging for host-pointer calls
+    int32_t* parents = nullptr;
+    int64_t tile_cap = 0;                 // entries per tile_start buffer
+    int tile_edges = 0;                   // T the tile_start buffers were sized for
+    size_t bytes = 0;
+};
+
+struct DeviceGraph {
+    int device = 0;
+    int64_t n = 0, m = 0;
+    int64_t* offsets = nullptr;  // n + 1
+    int32_t* cols = nullptr;     // m
+    size_t graph_bytes = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_kernel = nullptr;
+    int num_cus = 0;
+    BfsConfig cfg;
+    BfsWorkspace ws;
+
+    DeviceGraph(const HostGraph& g, int device);
+    ~DeviceGraph();
+    DeviceGraph(const DeviceGraph&) = delete;
+    DeviceGraph& operator=(const DeviceGraph&) = delete;
+
+    // levels / parents are device pointers (parents may be null).
+    void bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats* st);
+    // Host-pointer variant: runs into the handle's staging buffers, then
+    // copies levels (and parents) to the host.
+    void bfs_to_host(int32_t source, int32_t* h_levels, int32_t* h_parents, gb_stats* st);
+    size_t memory_bytes() const { return graph_bytes + ws.bytes; }
+
+   private:
+    void ensure_workspace();
+    void free_workspace();
+};
+
+// Device-wide exclusive scan of int32 (scan.hip).
+void exclusive_scan_i32(int device, const int32_t* d_in, int32_t* d_out, int64_t n);
+
+}  // namespace gg

@@ -1,0 +1,164 @@
+/*
+ * gb.h -- C ABI of the MI355X (gfx950) BFS engine.
+ *
+ * This is the drop-in boundary for the single-source BFS path of
+ * YKMorsy/GPU_Graphs.  The reference has no FFI layer: its surface is three
+ * C++ classes built by `main` (1D Partition/main.cu:17-35):
+ *
+ *   csr::csr(const char* filename)      1D Partition/include/csr.h:12
+ *       public int* row_offset, int* col_idx, int num_nodes, int num_edges
+ *                                        1D Partition/include/csr.h:17-20
+ *   gpuBFS::gpuBFS(csr&, int source)     1D Partition/include/gpuBFS.cuh:13
+ *       public int* host_distance, float exec_time, int iteration,
+ *       int total_edges_traversed        1D Partition/include/gpuBFS.cuh:17-20
+ *   PrefixSum::sum_scan_blelloch(int* d_out, const int* d_in, size_t n)
+ *                                        1D Partition/include/prefixSum.cuh:7-9
+ *
+ * Each entry point below names the reference interface it replaces.  The C++
+ * facade in gpugraphs.hpp rebuilds the three classes (same member names) on
+ * top of these functions, and the Python package gpu_graphs_amd binds them
+ * with ctypes.
+ *
+ * Conventions
+ *   - Every function returns GB_OK (0) or a negative GB_E* status.  The
+ *     library never calls exit() and no C++ exception crosses the ABI.  The
+ *     message for the last failing call on the calling thread is returned by
+ *     gb_last_error().
+ *   - Vertex ids are 0-based int32.  Levels are int32 with -1 for "not
+ *     reached" (the reference uses INT_MAX, 1D Partition/include/cpuBFS.h:11).
+ *   - Output buffers are caller-owned.  Functions with a _device suffix take
+ *     device pointers (memory on the graph's GPU) and leave results in HBM;
+ *     the others take host pointers and include the device-to-host copy.
+ *   - Handles are library-owned and released with the matching *_free.
+ *     A gb_dgraph is not thread-safe; distinct handles may be used from
+ *     distinct threads.
+ */
+#ifndef GPUGRAPHS_GB_H
+#define GPUGRAPHS_GB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define GB_OK        0
+#define GB_EINVAL   -1   /* bad argument (null handle, k out of range, ...) */
+#define GB_EIO      -2   /* file missing, unreadable or malformed          */
+#define GB_ENOMEM   -3   /* host or device allocation failed               */
+#define GB_EDEVICE  -4   /* HIP runtime error                              */
+#define GB_ERANGE   -5   /* vertex id / source outside [0, n)              */
+#define GB_EBUSY    -6   /* server: memory budget or queue full            */
+#define GB_EAGAIN   -7   /* server: result not ready yet                   */
+
+/* ---- graph construction flags ----------------------------------------- */
+#define GB_ONE_BASED        1u  /* ids in the file are 1-based (MIG loader, csr.cpp:28 `from-1,to-1`) */
+#define GB_SYMMETRIZE       2u  /* add the reverse of every edge                      */
+#define GB_KEEP_ORDER       4u  /* rows keep file order, duplicates and self-loops,
+                                   exactly as csr::csr lays them out (csr.cpp:32-57) */
+#define GB_DROP_SELF_LOOPS  8u  /* remove u->u edges                                  */
+#define GB_AUTO_N          16u  /* n = max(header n, largest id + 1): for files whose ids
+                                   exceed the header (graphs/wiki-Vote.txt, which the
+                                   reference loader overflows)                       */
+
+typedef struct gb_graph  gb_graph;   /* host CSR: int64 offsets, int32 columns */
+typedef struct gb_dgraph gb_dgraph;  /* device-resident copy on one GPU        */
+typedef struct gb_server gb_server;  /* continuous-batching BFS query server   */
+
+/* Per-BFS statistics.  n_reached / m_reached follow the Graph500 counting
+ * used for GTEPS: m_reached is the number of directed CSR entries of reached
+ * vertices (GTEPS uses m_reached/2).  The reference's own counters are broken
+ * (always 0: 1D Partition/src/gpuBFS_kernels.cu:37, gpuBFS.cu:63); these are
+ * exact. */
+typedef struct gb_stats {
+    double  ms;           /* device time, source injection -> last level (hipEvent) */
+    double  kernel_ms;    /* traversal kernel(s) alone, without the level init      */
+    int32_t num_levels;   /* 1 + max level                                           */
+    int32_t reserved;
+    int64_t n_reached;
+    int64_t m_reached;
+} gb_stats;
+
+/* ---- errors ------------------------------------------------------------ */
+const char* gb_last_error(void);
+const char* gb_version(void);
+
+/* ---- host graph (replaces csr::csr, 1D Partition/src/csr.cpp:3-67) ----- */
+
+/* Reads the reference edge-list format: line 1 `n`, line 2 `m` (a hint; the
+ * reference trusts it, csr.cpp:16-22), then whitespace-separated `from to`
+ * pairs.  Ids outside [0,n) after the GB_ONE_BASED shift are GB_ERANGE (the
+ * reference overflows, csr.cpp:28). */
+int  gb_graph_load_edgelist(const char* path, uint32_t flags, gb_graph** out);
+/* Builds a CSR from an edge list held in memory (src[i] -> dst[i]). */
+int  gb_graph_from_edges(int64_t n, int64_t m, const int32_t* src, const int32_t* dst,
+                         uint32_t flags, gb_graph** out);
+/* Copies an existing CSR (offsets[n+1], cols[offsets[n]]); rows are sorted
+ * and deduplicated unless GB_KEEP_ORDER. */
+int  gb_graph_from_csr(int64_t n, const int64_t* offsets, const int32_t* cols,
+                       uint32_t flags, gb_graph** out);
+/* Synthetic inputs.  All are symmetric, without self-loops or duplicates,
+ * and deterministic for a given seed. */
+int  gb_graph_generate_rmat(int scale, int edge_factor, double a, double b, double c,
+                            uint64_t seed, int permute_ids, gb_graph** out);
+int  gb_graph_generate_er(int64_t n_candidates, int64_t n_isolated, int64_t m_undirected,
+                          uint64_t seed, gb_graph** out);
+int  gb_graph_generate_grid(int64_t width, int64_t height, double extra_fraction,
+                            uint64_t seed, gb_graph** out);
+/* n = num_nodes, m = num_edges (directed CSR entries). */
+int  gb_graph_info(const gb_graph* g, int64_t* n, int64_t* m);
+/* Borrowed views of row_offset / col_idx (valid until gb_graph_free). */
+int  gb_graph_csr(const gb_graph* g, const int64_t** offsets, const int32_t** cols);
+/* Writes the graph in the reference text format (one `u v` line per CSR entry). */
+int  gb_graph_write_edgelist(const gb_graph* g, const char* path);
+/* Binary CSR cache (little-endian header + offsets + cols). */
+int  gb_graph_save_binary(const gb_graph* g, const char* path);
+int  gb_graph_load_binary(const char* path, gb_graph** out);
+void gb_graph_free(gb_graph* g);
+
+/* ---- device graph (replaces gpuBFS::init_device, gpuBFS.cu:139-185) ---- */
+int  gb_device_count(int* count);
+int  gb_device_upload(const gb_graph* g, int device, gb_dgraph** out);
+int  gb_dgraph_info(const gb_dgraph* dg, int64_t* n, int64_t* m, int* device);
+/* Bytes of HBM the handle holds (graph + single-source workspace). */
+int  gb_dgraph_memory(const gb_dgraph* dg, int64_t* bytes);
+void gb_device_free(gb_dgraph* dg);
+
+/* ---- single-source BFS (replaces gpuBFS::gpuBFS, gpuBFS.cu:13-116) ----- */
+
+/* Execution modes for gb_bfs_config. */
+#define GB_MODE_PERSISTENT  0   /* one cooperative launch, grid barrier per level */
+#define GB_MODE_LEVEL_LOOP  1   /* one launch per level, host reads the frontier size */
+
+/* levels: n int32 (host), -1 = unreached.  parents: nullable, n int32,
+ * parent[source] = source, -1 = unreached (any valid BFS tree). */
+int  gb_bfs(gb_dgraph* dg, int32_t source, int32_t* levels, int32_t* parents, gb_stats* st);
+/* Same, with levels / parents in device memory of the graph's GPU. */
+int  gb_bfs_device(gb_dgraph* dg, int32_t source, int32_t* d_levels, int32_t* d_parents,
+                   gb_stats* st);
+/* Tunables: block size (64..1024, multiple of 64), edges per thread per tile,
+ * blocks per CU (0 = occupancy maximum), mode, and the frontier edge count
+ * under which one workgroup runs a level alone. */
+int  gb_bfs_config(gb_dgraph* dg, int block, int edges_per_thread, int blocks_per_cu,
+                   int mode, int64_t solo_edges);
+int  gb_bfs_get_config(const gb_dgraph* dg, int* block, int* edges_per_thread,
+                       int* blocks_per_cu, int* mode, int64_t* solo_edges);
+
+/* ---- device-wide exclusive scan (replaces PrefixSum::sum_scan_blelloch,
+ *      1D Partition/src/prefixSum.cu:139-224).  d_out[i] = sum(d_in[0..i)).
+ *      In-place (d_out == d_in) is allowed.  No allocation per call. ----- */
+int  gb_exclusive_scan_i32(int device, const int32_t* d_in, int32_t* d_out, int64_t n);
+
+/* ---- device memory helpers (so hosts need no other GPU runtime) -------- */
+int  gb_dmalloc(int device, size_t bytes, void** ptr);
+int  gb_dfree(int device, void* ptr);
+int  gb_memcpy_htod(int device, void* dst, const void* src, size_t bytes);
+int  gb_memcpy_dtoh(int device, void* dst, const void* src, size_t bytes);
+int  gb_device_synchronize(int device);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPUGRAPHS_GB_H */

@@ -1,0 +1,179 @@
+"""GPU parity of single-source BFS (gb_bfs / gb_bfs_device) against the oracle.
+
+Levels must be bit-exact: against the scipy goldens for the reference's own
+fixture graphs (every source), against the CPU oracle for generated graphs,
+and, at full size, against the oracle for sampled sources plus the
+size-independent level invariants.  Parents must pass the Graph500 rules.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import gpu_graphs_amd as gg
+from conftest import FIXTURES, GOLDEN, load_fixture
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CONFIGS = [(256, 1), (256, 2), (256, 4), (256, 8), (512, 2), (512, 4), (1024, 1), (1024, 2), (128, 8), (64, 16)]
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("name", list(FIXTURES))
+def test_fixture_levels_every_source(gpu, name, golden_levels):
+    g = load_fixture(name)
+    sources = golden_levels[f"{name}__sources"]
+    want = golden_levels[f"{name}__levels"]
+    dg = g.device_graph(gpu)
+    log(f"fixture {name} n={g.num_nodes} m={g.num_edges} sources={len(sources)}")
+    for i, s in enumerate(sources):
+        lv, par, st = dg.bfs(int(s), parents=True)
+        np.testing.assert_array_equal(lv, want[i], err_msg=f"{name} source {s}")
+        assert O.check_parents(g.row_offset, g.col_idx, int(s), lv, par) == 0, (name, s)
+        assert st["n_reached"] == int((want[i] >= 0).sum())
+        assert st["num_levels"] == int(want[i].max()) + 1
+
+
+def test_reference_api_mirror(gpu):
+    g = gg.csr(os.path.join(GOLDEN, "graphs", "tree.txt"))
+    b = gg.gpuBFS(g, 1)
+    assert b.host_distance.tolist() == [-1, 0, 1, 1, 2, 2, 2, 2]
+    assert b.iteration == 3 and b.total_edges_traversed == 6 and b.exec_time > 0
+
+
+def test_er_workload_golden(gpu):
+    g = gg.csr(os.path.join(GOLDEN, "er_10200.txt"))
+    want = np.load(os.path.join(GOLDEN, "er_10200_levels.npy"))
+    lv, _, st = g.device_graph(gpu).bfs(0)
+    np.testing.assert_array_equal(lv, want)
+    assert st["n_reached"] == 10000
+
+
+@pytest.mark.parametrize("mode", [gg.GB_MODE_PERSISTENT, gg.GB_MODE_LEVEL_LOOP])
+@pytest.mark.parametrize("block,k", CONFIGS)
+def test_rmat_all_kernel_shapes(gpu, mode, block, k):
+    G = gg.Graph.rmat(14, seed=41)
+    dg = gg.DeviceGraph(G, gpu)
+    dg.config(block=block, edges_per_thread=k, mode=mode)
+    off, cols = G.offsets, G.cols
+    deg = np.diff(off)
+    rng = np.random.default_rng(block * 10 + k + mode)
+    log(f"rmat14 block={block} k={k} mode={mode}")
+    for s in rng.choice(np.nonzero(deg)[0], 4, replace=False):
+        lv, par, st = dg.bfs(int(s), parents=True)
+        want, _, ost = O.bfs(off, cols, int(s))
+        np.testing.assert_array_equal(lv, want)
+        assert O.check_parents(off, cols, int(s), lv, par) == 0
+        assert st["n_reached"] == ost["n_reached"] and st["m_reached"] == ost["m_reached"]
+        assert st["num_levels"] == ost["num_levels"]
+
+
+@pytest.mark.parametrize("scale,seed", [(16, 1), (18, 2)])
+def test_rmat_levels_match_oracle(gpu, scale, seed):
+    G = gg.Graph.rmat(scale, seed=seed)
+    dg = gg.DeviceGraph(G, gpu)
+    off, cols = G.offsets, G.cols
+    deg = np.diff(off)
+    rng = np.random.default_rng(seed)
+    srcs = list(rng.choice(np.nonzero(deg)[0], 8, replace=False)) + [int(np.argmax(deg))]
+    log(f"rmat{scale} n={G.n} m={G.m}")
+    for s in srcs:
+        lv, _, st = dg.bfs(int(s))
+        want, _, ost = O.bfs(off, cols, int(s))
+        np.testing.assert_array_equal(lv, want, err_msg=f"source {s}")
+        assert st["m_reached"] == ost["m_reached"]
+
+
+def test_isolated_and_degree_zero_sources(gpu):
+    G = gg.Graph.rmat(12, seed=3)
+    deg = np.diff(G.offsets)
+    dg = gg.DeviceGraph(G, gpu)
+    s = int(np.nonzero(deg == 0)[0][0])
+    lv, par, st = dg.bfs(s, parents=True)
+    assert lv[s] == 0 and (np.delete(lv, s) == -1).all()
+    assert par[s] == s and st["n_reached"] == 1 and st["num_levels"] == 1 and st["m_reached"] == 0
+
+
+def test_grid_many_levels(gpu):
+    G = gg.Graph.grid(300, 200, 0.05, seed=4)
+    dg = gg.DeviceGraph(G, gpu)
+    for mode in (gg.GB_MODE_PERSISTENT, gg.GB_MODE_LEVEL_LOOP):
+        dg.config(mode=mode)
+        for s in (0, 123 * 300 + 77):
+            lv, _, st = dg.bfs(s)
+            want, _, ost = O.bfs(G.offsets, G.cols, s)
+            np.testing.assert_array_equal(lv, want)
+            assert st["num_levels"] == ost["num_levels"] and st["num_levels"] > 300
+
+
+def test_hub_spanning_many_tiles(gpu):
+    # star with 100,000 leaves plus a tail: the hub's row covers ~100 tiles
+    n = 100_002
+    src = np.concatenate([np.zeros(100_000, np.int32), [100_000]]).astype(np.int32)
+    dst = np.concatenate([np.arange(1, 100_001, dtype=np.int32), [100_001]]).astype(np.int32)
+    G = gg.Graph.from_edges(n, np.concatenate([src, dst]), np.concatenate([dst, src]))
+    dg = gg.DeviceGraph(G, gpu)
+    for block, k in [(256, 4), (64, 16), (1024, 2)]:
+        dg.config(block=block, edges_per_thread=k)
+        lv, par, _ = dg.bfs(5, parents=True)
+        want, _, _ = O.bfs(G.offsets, G.cols, 5)
+        np.testing.assert_array_equal(lv, want)
+        assert lv.max() == 4
+        assert O.check_parents(G.offsets, G.cols, 5, lv, par) == 0
+
+
+def test_tiny_graphs_and_errors(gpu):
+    G = gg.Graph.from_edges(1, [], [])
+    dg = gg.DeviceGraph(G, gpu)
+    lv, _, st = dg.bfs(0)
+    assert lv.tolist() == [0] and st["num_levels"] == 1
+    with pytest.raises(gg.GBError) as e:
+        dg.bfs(1)
+    assert e.value.code == -5
+    with pytest.raises(gg.GBError) as e:
+        dg.bfs(-1)
+    assert e.value.code == -5
+    with pytest.raises(gg.GBError):
+        dg.config(block=96, edges_per_thread=3)
+    # self-loops and duplicates in a kept-order (reference layout) graph
+    G = gg.Graph.from_edges(4, [0, 0, 0, 1, 1, 2], [0, 1, 1, 1, 2, 3], flags=gg.GB_KEEP_ORDER)
+    lv, _, _ = gg.DeviceGraph(G, gpu).bfs(0)
+    assert lv.tolist() == [0, 1, 2, 3]
+
+
+def test_device_pointer_path_and_determinism(gpu):
+    G = gg.Graph.rmat(15, seed=8)
+    dg = gg.DeviceGraph(G, gpu)
+    deg = np.diff(G.offsets)
+    s = int(np.argmax(deg))
+    d_lv = gg.DeviceArray(gpu, np.int32, G.n)
+    d_par = gg.DeviceArray(gpu, np.int32, G.n)
+    st = dg.bfs_device(s, d_lv, d_par)
+    lv = d_lv.numpy()
+    want, _, ost = O.bfs(G.offsets, G.cols, s)
+    np.testing.assert_array_equal(lv, want)
+    assert O.check_parents(G.offsets, G.cols, s, lv, d_par.numpy()) == 0
+    assert st["n_reached"] == ost["n_reached"]
+    for _ in range(5):
+        np.testing.assert_array_equal(dg.bfs(s)[0], want)
+
+
+@pytest.mark.parametrize("n", [0, 1, 15, 16, 4095, 4096, 4097, 1_000_003])
+def test_exclusive_scan(gpu, n):
+    rng = np.random.default_rng(n)
+    x = rng.integers(-2**31, 2**31 - 1, size=n, dtype=np.int64).astype(np.int32)
+    if n == 0:
+        d = gg.DeviceArray(gpu, np.int32, 1)
+        gg._ffi.check(gg.lib().gb_exclusive_scan_i32(gpu, d.ptr, d.ptr, 0))
+        return
+    np.testing.assert_array_equal(gg.exclusive_scan(x, gpu), O.exclusive_scan(x))
+
+
+def test_scan_known_answer(gpu):
+    # 1D Partition/misc/test_prefix.cu:91-93
+    assert gg.exclusive_scan(np.arange(16), gpu).tolist() == [0, 0, 1, 3, 6, 10, 15, 21, 28, 36, 45, 55, 66, 78,
+                                                               91, 105]
