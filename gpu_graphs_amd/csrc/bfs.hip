@@ -39,17 +39,24 @@ namespace gg {
 
 namespace {
 
-// ctl[] layout (64-bit words).
+// ctl[] layout (64-bit words).  Every word that workgroups use to agree with
+// each other (ring counters, barrier, abort) is read and written only with
+// atomic read-modify-write operations, which are coherent across the XCDs'
+// separate L2 caches; plain loads of these words could return a stale line.
 enum : int {
-    CTL_RING0 = 0,     // 3 packed frontier counters: level L uses ring[L % 3]
-    CTL_BAR_COUNT = 3, // grid barrier arrivals
-    CTL_BAR_GEN = 4,   // grid barrier generation
-    CTL_NREACH = 5,    // reached vertices
-    CTL_MREACH = 6,    // sum of degrees of reached vertices
-    CTL_MAXLEV = 7,    // largest level assigned
-    CTL_ERR = 8,       // internal consistency error (non-zero = bad)
+    CTL_RING0 = 0,      // 3 packed frontier counters: level L uses ring[L % 3]
+    CTL_BAR_COUNT = 3,  // grid barrier arrivals, monotonic (never reset in a launch)
+    CTL_ABORT = 4,      // set when a barrier wait times out; every workgroup then exits
+    CTL_NREACH = 5,     // reached vertices
+    CTL_MREACH = 6,     // sum of degrees of reached vertices
+    CTL_MAXLEV = 7,     // largest level assigned
+    CTL_ERR = 8,        // internal consistency error (non-zero = bad)
+    CTL_BAR_SEEN = 9,   // on timeout: arrivals observed by the workgroup that gave up
+    CTL_BAR_WANT = 10,  // on timeout: arrivals it waited for
     CTL_WORDS = 16
 };
+
+enum : unsigned long long { ERR_TILE = 1, ERR_LEVELS = 2 };
 
 struct BfsArgs {
     const int64_t* __restrict__ offsets;
@@ -65,6 +72,7 @@ struct BfsArgs {
     int64_t n;
     uint32_t nwords;
     int32_t source;
+    uint64_t spin_limit;  // wall-clock ticks a barrier may wait before aborting
 };
 
 template <int BLOCK, int K>
@@ -82,31 +90,51 @@ struct TileLds {
     unsigned long long base;
 };
 
-__device__ __forceinline__ unsigned long long ld_acquire(const unsigned long long* p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_relaxed(unsigned long long* p, unsigned long long v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Coherent read of a control word (atomic add of zero).
+__device__ __forceinline__ unsigned long long ctl_read(unsigned long long* p) {
+    return __hip_atomic_fetch_add(p, 0ull, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Sense-free generation barrier over all workgroups of a cooperative launch.
-// Release/acquire at agent scope makes every workgroup's stores (queue,
-// tile_start, counters) visible to every other XCD after the barrier.
-__device__ void grid_barrier(unsigned long long* ctl, uint32_t nblocks) {
+// Grid barrier over all workgroups of a cooperative launch.  The arrival
+// counter only grows: barrier number `epoch` (0, 1, ...) is complete when it
+// reaches (epoch + 1) * nblocks, so no workgroup reads anything before it
+// arrives and a stale read can only delay, never release, a waiter.  The
+// wait is bounded by `spin_limit` wall-clock ticks: on timeout the waiter
+// raises CTL_ABORT (recording what it saw) and every workgroup leaves the
+// BFS loop, so a launch that is not fully co-resident ends with an error
+// instead of hanging.  Returns false when the BFS must stop.
+__device__ bool grid_barrier(const BfsArgs& a, uint32_t nblocks, unsigned long long epoch) {
+    __shared__ int s_ok;
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long gen = ld_acquire(&ctl[CTL_BAR_GEN]);
-        __threadfence();
-        unsigned long long arrived = atomicAdd(&ctl[CTL_BAR_COUNT], 1ull);
-        if (arrived == nblocks - 1) {
-            st_relaxed(&ctl[CTL_BAR_COUNT], 0ull);
-            __hip_atomic_fetch_add(&ctl[CTL_BAR_GEN], 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            while (ld_acquire(&ctl[CTL_BAR_GEN]) == gen) __builtin_amdgcn_s_sleep(1);
+        __threadfence();  // release this workgroup's queue / level / tile_start stores
+        const unsigned long long target = (epoch + 1) * nblocks;
+        unsigned long long seen = __hip_atomic_fetch_add(&a.ctl[CTL_BAR_COUNT], 1ull, __ATOMIC_ACQ_REL,
+                                                         __HIP_MEMORY_SCOPE_AGENT) + 1;
+        int ok = 1;
+        const uint64_t t0 = wall_clock64();
+        while (seen < target) {
+            __builtin_amdgcn_s_sleep(8);
+            seen = ctl_read(&a.ctl[CTL_BAR_COUNT]);
+            if (seen >= target) break;
+            if (ctl_read(&a.ctl[CTL_ABORT])) {
+                ok = 0;
+                break;
+            }
+            if (static_cast<uint64_t>(wall_clock64()) - t0 > a.spin_limit) {
+                if (atomicOr(&a.ctl[CTL_ABORT], 1ull) == 0) {
+                    atomicExch(&a.ctl[CTL_BAR_SEEN], seen);
+                    atomicExch(&a.ctl[CTL_BAR_WANT], target);
+                }
+                ok = 0;
+                break;
+            }
         }
-        __threadfence();
+        __threadfence();  // acquire the other workgroups' stores
+        s_ok = ok;
     }
     __syncthreads();
+    return s_ok != 0;
 }
 
 // Expands tile t of level L: frontier edges [t*T, min((t+1)*T, E)).
@@ -123,7 +151,7 @@ __device__ void expand_tile(const BfsArgs& a, TileLds<BLOCK, K>& s, int L, uint3
     const uint32_t i1 = (t + 1 < ntiles) ? a.tile_start[cur][t + 1] : F - 1;
     uint32_t c = i1 - i0 + 1;
     if (i1 < i0 || c > T + 1 || i1 >= F) {  // cannot happen; guard the LDS arrays
-        if (tid == 0) atomicOr(&a.ctl[CTL_ERR], 1ull);
+        if (tid == 0) atomicOr(&a.ctl[CTL_ERR], ERR_TILE);
         c = 0;
     }
     for (uint32_t j = tid; j <= c; j += BLOCK) {
@@ -243,13 +271,29 @@ __device__ void expand_tile(const BfsArgs& a, TileLds<BLOCK, K>& s, int L, uint3
     __syncthreads();
 }
 
+// Reads the packed frontier counter of level L once per workgroup and
+// broadcasts it through LDS, so all threads of a workgroup agree.  After an
+// abort it reads as empty, so a late-starting workgroup does nothing.
+__device__ __forceinline__ unsigned long long level_counter(const BfsArgs& a, int L) {
+    __shared__ unsigned long long s_cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_cnt = ctl_read(&a.ctl[CTL_RING0 + L % 3]);
+        if (ctl_read(&a.ctl[CTL_ABORT])) s_cnt = 0;
+    }
+    __syncthreads();
+    return s_cnt;
+}
+
 template <int BLOCK, int K>
-__device__ void expand_level(const BfsArgs& a, TileLds<BLOCK, K>& s, int L, uint32_t b, uint32_t G) {
+__device__ void expand_level(const BfsArgs& a, TileLds<BLOCK, K>& s, int L, unsigned long long cnt, uint32_t b,
+                             uint32_t G) {
     constexpr uint32_t T = BLOCK * K;
-    const unsigned long long cnt = ld_acquire(&a.ctl[CTL_RING0 + L % 3]);
     const uint32_t F = static_cast<uint32_t>(cnt >> 32), E = static_cast<uint32_t>(cnt);
     if (b == 0 && threadIdx.x == 0) {
-        st_relaxed(&a.ctl[CTL_RING0 + (L + 2) % 3], 0ull);
+        // ring[(L+2)%3] held level L-1's count, read by every workgroup
+        // before the barrier that ended level L-1; it next counts level L+2.
+        atomicExch(&a.ctl[CTL_RING0 + (L + 2) % 3], 0ull);
         atomicAdd(&a.ctl[CTL_MREACH], static_cast<unsigned long long>(E));
     }
     if (F == 0) return;
@@ -260,17 +304,23 @@ __device__ void expand_level(const BfsArgs& a, TileLds<BLOCK, K>& s, int L, uint
 template <int BLOCK, int K>
 __global__ __launch_bounds__(BLOCK) void bfs_level_kernel(BfsArgs a, int L) {
     __shared__ TileLds<BLOCK, K> s;
-    expand_level<BLOCK, K>(a, s, L, blockIdx.x, gridDim.x);
+    expand_level<BLOCK, K>(a, s, L, level_counter(a, L), blockIdx.x, gridDim.x);
 }
 
+// All levels in one cooperative launch.  A BFS has at most n levels; the
+// loop is also bounded by that, so a corrupted counter cannot spin forever.
 template <int BLOCK, int K>
 __global__ __launch_bounds__(BLOCK) void bfs_persistent_kernel(BfsArgs a) {
     __shared__ TileLds<BLOCK, K> s;
     for (int L = 0;; ++L) {
-        const unsigned long long cnt = ld_acquire(&a.ctl[CTL_RING0 + L % 3]);
-        if ((cnt >> 32) == 0) break;  // every workgroup reads the same value
-        expand_level<BLOCK, K>(a, s, L, blockIdx.x, gridDim.x);
-        grid_barrier(a.ctl, gridDim.x);
+        const unsigned long long cnt = level_counter(a, L);
+        if ((cnt >> 32) == 0) break;  // every workgroup reads the same final value
+        if (L > a.n) {
+            if (threadIdx.x == 0) atomicOr(&a.ctl[CTL_ERR], ERR_LEVELS);
+            break;
+        }
+        expand_level<BLOCK, K>(a, s, L, cnt, blockIdx.x, gridDim.x);
+        if (!grid_barrier(a, gridDim.x, static_cast<unsigned long long>(L))) break;
     }
 }
 
@@ -339,6 +389,10 @@ DeviceGraph::DeviceGraph(const HostGraph& g, int dev) : device(dev), n(g.n), m(g
     hipDeviceProp_t prop;
     GG_HIP(hipGetDeviceProperties(&prop, device));
     num_cus = prop.multiProcessorCount;
+    int clock_khz = 0;
+    GG_HIP(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, device));
+    if (clock_khz <= 0) clock_khz = 100000;
+    spin_limit_ticks = static_cast<uint64_t>(clock_khz) * 1000ull * kBarrierTimeoutSeconds;
     try {
         GG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         GG_HIP(hipEventCreate(&ev_start));
@@ -447,6 +501,7 @@ void DeviceGraph::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_
     a.n = n;
     a.nwords = static_cast<uint32_t>((n + 31) / 32);
     a.source = source;
+    a.spin_limit = spin_limit_ticks;
 
     int occ = 0;
     if (cfg.mode == GB_MODE_PERSISTENT)
@@ -456,7 +511,11 @@ void DeviceGraph::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_
         GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(ks->level),
                                                            cfg.block, 0));
     if (occ < 1) fail(GB_EDEVICE, "BFS kernel cannot be resident on a CU");
-    const int per_cu = cfg.blocks_per_cu > 0 ? std::min(cfg.blocks_per_cu, occ) : occ;
+    // Default: at most 16 waves per CU (half the wave slots), leaving
+    // headroom so every workgroup of the cooperative launch is co-resident.
+    const int waves_per_block = (cfg.block + 63) / 64;
+    const int default_per_cu = std::max(1, std::min(occ, 16 / waves_per_block));
+    const int per_cu = cfg.blocks_per_cu > 0 ? std::min(cfg.blocks_per_cu, occ) : default_per_cu;
     const uint32_t grid = static_cast<uint32_t>(per_cu * num_cus);
     const uint32_t clear_grid = static_cast<uint32_t>(std::min<int64_t>((n + 255) / 256, int64_t(num_cus) * 8));
 
@@ -474,11 +533,12 @@ void DeviceGraph::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_
         GG_HIP(hipHostMalloc(&h_cnt, sizeof(unsigned long long) * CTL_WORDS, hipHostMallocDefault));
         try {
             unsigned long long cnt = 1ull << 32;
-            for (int L = 0; (cnt >> 32) != 0; ++L) {
+            for (int64_t L = 0; (cnt >> 32) != 0; ++L) {
+                if (L > n) fail(GB_EDEVICE, "internal error: more BFS levels than vertices");
                 const uint32_t E = L == 0 ? 0u : static_cast<uint32_t>(cnt);
                 uint32_t g = grid;
                 if (L > 0) g = static_cast<uint32_t>(std::min<uint64_t>(grid, (uint64_t(E) + T - 1) / T));
-                hipLaunchKernelGGL(ks->level, dim3(std::max(1u, g)), dim3(cfg.block), 0, stream, a, L);
+                hipLaunchKernelGGL(ks->level, dim3(std::max(1u, g)), dim3(cfg.block), 0, stream, a, static_cast<int>(L));
                 GG_HIP(hipGetLastError());
                 GG_HIP(hipMemcpyAsync(h_cnt, ws.ctl, sizeof(unsigned long long) * CTL_WORDS, hipMemcpyDeviceToHost,
                                       stream));
@@ -495,7 +555,13 @@ void DeviceGraph::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_
     unsigned long long h_ctl[CTL_WORDS];
     GG_HIP(hipMemcpyAsync(h_ctl, ws.ctl, sizeof(h_ctl), hipMemcpyDeviceToHost, stream));
     GG_HIP(hipStreamSynchronize(stream));
-    if (h_ctl[CTL_ERR]) fail(GB_EDEVICE, "internal error: inconsistent frontier tile index");
+    if (h_ctl[CTL_ABORT])
+        fail(GB_EDEVICE, "persistent BFS aborted: a grid barrier waited longer than the limit (" +
+                             std::to_string(h_ctl[CTL_BAR_SEEN]) + " of " + std::to_string(h_ctl[CTL_BAR_WANT]) +
+                             " arrivals seen; " + std::to_string(grid) + " workgroups of " +
+                             std::to_string(cfg.block) + " threads were not all co-resident)");
+    if (h_ctl[CTL_ERR] & ERR_TILE) fail(GB_EDEVICE, "internal error: inconsistent frontier tile index");
+    if (h_ctl[CTL_ERR] & ERR_LEVELS) fail(GB_EDEVICE, "internal error: more BFS levels than vertices");
     if (st) {
         float ms = 0.f, kms = 0.f;
         GG_HIP(hipEventElapsedTime(&ms, ev_start, ev_stop));

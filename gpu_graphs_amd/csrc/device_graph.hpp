@@ -71,6 +71,10 @@ struct DeviceGraph {
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_kernel = nullptr;
     int num_cus = 0;
+    // Longest a persistent kernel's grid barrier waits before aborting the
+    // BFS with an error (a legitimate level takes milliseconds).
+    static constexpr uint64_t kBarrierTimeoutSeconds = 5;
+    uint64_t spin_limit_ticks = 0;
     BfsConfig cfg;
     BfsWorkspace ws;
 

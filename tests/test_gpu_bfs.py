@@ -122,7 +122,7 @@ def test_hub_spanning_many_tiles(gpu):
         lv, par, _ = dg.bfs(5, parents=True)
         want, _, _ = O.bfs(G.offsets, G.cols, 5)
         np.testing.assert_array_equal(lv, want)
-        assert lv.max() == 4
+        assert lv.max() == 3  # leaf 5 -> hub 0 -> leaf 100000 -> tail 100001
         assert O.check_parents(G.offsets, G.cols, 5, lv, par) == 0
 
 

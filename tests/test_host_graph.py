@@ -1,0 +1,225 @@
+"""Host side of the engine (no GPU): the edge-list loader that replaces
+csr::csr (1D Partition/src/csr.cpp:3-67), the CSR builder and the
+generators.  Compared against the oracle loader, scipy, the reference's own
+loader (oracle/_ref, when built) and structural invariants."""
+import os
+
+import numpy as np
+import pytest
+from scipy.sparse import csr_matrix
+
+import gpu_graphs_amd as gg
+from conftest import FIXTURES, GOLDEN, fixture_path
+from oracle import oracle as O
+
+
+def assert_csr_invariants(g, symmetric=False, no_loops=False):
+    off, cols = g.offsets, g.cols
+    n = g.n
+    assert off[0] == 0 and off[-1] == g.m == cols.size
+    assert (np.diff(off) >= 0).all()
+    if cols.size:
+        assert cols.min() >= 0 and cols.max() < n
+    rows = np.repeat(np.arange(n), np.diff(off))
+    key = rows.astype(np.int64) * n + cols
+    assert (np.diff(key) > 0).all(), "rows must be sorted and duplicate-free"
+    if no_loops:
+        assert (rows != cols).all()
+    if symmetric:
+        a = csr_matrix((np.ones(cols.size), cols, off), shape=(n, n))
+        assert (a != a.T).nnz == 0
+
+
+@pytest.mark.parametrize("name", list(FIXTURES))
+def test_loader_vs_oracle_loader(name):
+    one_based, auto_n = FIXTURES[name]
+    g = gg.csr(fixture_path(name), one_based=one_based, keep_order=True, auto_n=auto_n)
+    if not auto_n:
+        off, cols = O.load_edgelist(fixture_path(name), one_based=one_based)
+        np.testing.assert_array_equal(g.row_offset, off)
+        np.testing.assert_array_equal(g.col_idx, cols)
+    s = gg.csr(fixture_path(name), one_based=one_based, auto_n=auto_n)
+    assert_csr_invariants(s.graph)
+    # same edge set as scipy's deduplicated CSR of the kept-order graph
+    n = g.num_nodes
+    a = csr_matrix((np.ones(g.num_edges), g.col_idx.copy(), g.row_offset.copy()), shape=(n, n))
+    a.sum_duplicates()
+    a.sort_indices()
+    np.testing.assert_array_equal(s.row_offset, a.indptr)
+    np.testing.assert_array_equal(s.col_idx, a.indices)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref/libref.so not built")
+@pytest.mark.parametrize("name", ["tree", "test4"])
+def test_keep_order_matches_reference_csr(name):
+    ro, ci = O.ref_load_csr(fixture_path(name))
+    g = gg.csr(fixture_path(name), keep_order=True)
+    np.testing.assert_array_equal(g.row_offset, ro)
+    np.testing.assert_array_equal(g.col_idx, ci)
+
+
+def test_tree_layout_and_header_hint():
+    g = gg.csr(fixture_path("tree"))
+    assert g.num_nodes == 8 and g.num_edges == 6
+    assert g.row_offset.tolist() == [0, 0, 2, 4, 6, 6, 6, 6, 6]
+    assert g.col_idx.tolist() == [2, 3, 4, 5, 6, 7]
+    g = gg.csr(fixture_path("test4"))  # header says 7 edges, file has 5 pairs
+    assert g.num_edges == 5
+
+
+def test_loader_errors(tmp_path):
+    def load(text, **kw):
+        p = tmp_path / "g.txt"
+        p.write_text(text)
+        return gg.csr(str(p), **kw)
+
+    with pytest.raises(gg.GBError) as e:
+        load("4\n2\n0 1\n1 4\n")
+    assert e.value.code == -5 and "outside" in str(e.value)
+    with pytest.raises(gg.GBError) as e:
+        load("4\n2\n0 1\n1 x\n")
+    assert e.value.code == -2 and "malformed" in str(e.value)
+    with pytest.raises(gg.GBError) as e:
+        load("4\n2\n0 1\n1\n")
+    assert e.value.code == -2 and "dangling" in str(e.value)
+    with pytest.raises(gg.GBError) as e:
+        load("")
+    assert e.value.code == -2
+    with pytest.raises(gg.GBError) as e:
+        load("3\n1\n0 3\n", one_based=False)
+    assert e.value.code == -5
+    g = load("3\n1\n0 3\n", auto_n=True)
+    assert g.num_nodes == 4
+    with pytest.raises(gg.GBError):
+        load("3\n1\n1 0\n", one_based=True)  # id 0 is -1 after the one-based shift
+    g = load("# comment\n3\n2\n0 1\n% another\n1 2\n")
+    assert g.num_edges == 2
+    g = load("3\n0\n")
+    assert g.num_nodes == 3 and g.num_edges == 0
+
+
+def test_symmetrize_and_flags(tmp_path):
+    p = tmp_path / "g.txt"
+    p.write_text("4\n5\n0 1\n1 2\n2 2\n0 1\n3 0\n")
+    g = gg.csr(str(p), symmetrize=True)
+    assert_csr_invariants(g.graph, symmetric=True)
+    assert g.num_edges == 7  # {0-1, 1-2, 0-3} both ways + the 2-2 loop once
+    g = gg.Graph.from_file(str(p), gg.GB_SYMMETRIZE | gg.GB_DROP_SELF_LOOPS)
+    assert_csr_invariants(g, symmetric=True, no_loops=True)
+    assert g.m == 6
+    g = gg.csr(str(p), keep_order=True)
+    assert g.col_idx.tolist() == [1, 1, 2, 2, 0]
+
+
+def test_from_csr_and_binary_roundtrip(tmp_path):
+    g = gg.Graph.rmat(10, seed=3)
+    h = gg.Graph.from_csr(g.offsets, g.cols)
+    np.testing.assert_array_equal(h.offsets, g.offsets)
+    np.testing.assert_array_equal(h.cols, g.cols)
+    path = str(tmp_path / "g.bin")
+    g.save_binary(path)
+    b = gg.Graph.load_binary(path)
+    np.testing.assert_array_equal(b.offsets, g.offsets)
+    np.testing.assert_array_equal(b.cols, g.cols)
+    with open(path, "r+b") as f:
+        f.truncate(100)
+    with pytest.raises(gg.GBError):
+        gg.Graph.load_binary(path)
+    txt = str(tmp_path / "g.txt")
+    g.write_edgelist(txt)
+    t = gg.Graph.from_file(txt)
+    np.testing.assert_array_equal(t.offsets, g.offsets)
+    np.testing.assert_array_equal(t.cols, g.cols)
+
+
+def test_rmat_properties_and_determinism(monkeypatch):
+    g = gg.Graph.rmat(14, 16, seed=99)
+    assert g.n == 2**14
+    assert_csr_invariants(g, symmetric=True, no_loops=True)
+    assert 0.5 * 2 * 16 * 2**14 < g.m <= 2 * 16 * 2**14
+    deg = g.degrees()
+    assert deg.max() > 20 * deg.mean()  # power-law hubs
+    monkeypatch.setenv("GB_THREADS", "1")
+    g1 = gg.Graph.rmat(14, 16, seed=99)
+    np.testing.assert_array_equal(g1.offsets, g.offsets)
+    np.testing.assert_array_equal(g1.cols, g.cols)
+    g2 = gg.Graph.rmat(14, 16, seed=100)
+    assert g2.m != g.m or not np.array_equal(g2.cols, g.cols)
+
+
+def test_rmat_quadrant_probabilities():
+    # Without the id permutation, the top bit of (u, v) follows the first
+    # quadrant draw: P(u < n/2) = a + b = 0.76, P(v < n/2) = a + c = 0.76.
+    g = gg.Graph.rmat(12, 16, seed=5, permute=False)
+    off, cols = g.offsets, g.cols
+    rows = np.repeat(np.arange(g.n), np.diff(off))
+    half = g.n // 2
+    both_low = np.mean((rows < half) & (cols < half))
+    both_high = np.mean((rows >= half) & (cols >= half))
+    # symmetrised: P(both low) ~ a = 0.57 and P(both high) ~ d = 0.05, less dedup
+    assert 0.45 < both_low < 0.65 and 0.02 < both_high < 0.08
+
+
+def test_er_workload_matches_golden_file():
+    g = gg.Graph.erdos_renyi(10000, 200, 50000, seed=20260002)
+    f = gg.Graph.from_file(os.path.join(GOLDEN, "er_10200.txt"))
+    np.testing.assert_array_equal(g.offsets, f.offsets)
+    np.testing.assert_array_equal(g.cols, f.cols)
+    assert g.n == 10200 and g.m == 100000
+    assert_csr_invariants(g, symmetric=True, no_loops=True)
+    assert (g.degrees()[10000:] == 0).all()
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (1, 7), (9, 1), (40, 30), (200, 300)])
+def test_grid_spanning_tree_plus_extra(w, h):
+    g = gg.Graph.grid(w, h, 0.05, seed=12)
+    n = w * h
+    assert_csr_invariants(g, symmetric=True, no_loops=True)
+    grid_edges = h * (w - 1) + (h - 1) * w
+    extra = int(round(0.05 * (grid_edges - (n - 1))))
+    assert g.m == 2 * (n - 1 + extra)
+    lv, _, st = O.bfs(g.offsets, g.cols, 0)
+    assert st["n_reached"] == n  # spanning: connected
+    off, cols = g.offsets, g.cols
+    rows = np.repeat(np.arange(n), np.diff(off))
+    d = np.abs(rows - cols)
+    assert np.isin(d, [1, w]).all()  # only grid edges
+    if n > 1:
+        assert (lv.max() >= (w - 1) + (h - 1) - 0)  # at least the Manhattan diameter from a corner
+
+
+def _mix64(x):
+    x = x + np.uint64(0x9E3779B97F4A7C15)
+    x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return x ^ (x >> np.uint64(31))
+
+
+def _rng_at(seed, stream, i):
+    with np.errstate(over="ignore"):
+        base = _mix64(np.uint64(seed) ^ (np.uint64(stream) * np.uint64(0xD1B54A32D192ED03)))
+        return _mix64(base + i.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15))
+
+
+def test_grid_tree_is_the_minimum_spanning_tree_of_its_weights():
+    # The tree must be the unique MST of the grid under the generator's edge
+    # keys (random 32-bit weight, ties by edge id).  The keys are recomputed
+    # here from the published SplitMix64 finaliser and the MST taken with
+    # scipy (Kruskal/Prim), on exact integer ranks of the keys.
+    from scipy.sparse.csgraph import minimum_spanning_tree
+    w, h, seed = 37, 23, 77
+    n = w * h
+    eh = h * (w - 1)
+    e = np.arange(eh + (h - 1) * w, dtype=np.int64)
+    u = np.where(e < eh, (e // (w - 1)) * w + e % (w - 1), e - eh)
+    v = np.where(e < eh, u + 1, u + w)
+    with np.errstate(over="ignore"):
+        key = (_rng_at(seed, 2, e) & np.uint64(0xFFFFFFFF00000000)) | e.astype(np.uint64)
+    rank = np.empty(e.size, dtype=np.float64)
+    rank[np.argsort(key)] = np.arange(1, e.size + 1)
+    mst = minimum_spanning_tree(csr_matrix((rank, (u, v)), shape=(n, n))).tocoo()
+    want = set(zip(np.minimum(mst.row, mst.col).tolist(), np.maximum(mst.row, mst.col).tolist()))
+    g = gg.Graph.grid(w, h, 0.0, seed=seed)
+    rows = np.repeat(np.arange(n), np.diff(g.offsets))
+    got = {(a, b) for a, b in zip(rows.tolist(), g.cols.tolist()) if a < b}
+    assert got == want
