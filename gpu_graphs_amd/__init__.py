@@ -21,13 +21,13 @@ import ctypes as C
 import numpy as np
 
 from . import _ffi
-from ._ffi import (GB_AUTO_N, GB_DROP_SELF_LOOPS, GB_KEEP_ORDER, GB_MODE_LEVEL_LOOP, GB_MODE_PERSISTENT, GB_ONE_BASED,
+from ._ffi import (GB_AUTO_N, GB_DROP_SELF_LOOPS, GB_KEEP_ORDER, GB_MODE_ASYNC, GB_MODE_LEVEL_LOOP, GB_ONE_BASED,
                    GB_SYMMETRIZE, GBError, Stats, check, lib)
 
 __all__ = [
     "Graph", "DeviceGraph", "DeviceArray", "csr", "gpuBFS", "GBError", "Stats", "device_count",
     "exclusive_scan", "GB_AUTO_N", "GB_ONE_BASED", "GB_SYMMETRIZE", "GB_KEEP_ORDER", "GB_DROP_SELF_LOOPS",
-    "GB_MODE_PERSISTENT", "GB_MODE_LEVEL_LOOP",
+    "GB_MODE_ASYNC", "GB_MODE_LEVEL_LOOP",
 ]
 
 
@@ -203,15 +203,16 @@ class DeviceGraph:
         self.n = graph.n
         self.m = graph.m
 
-    def config(self, block=256, edges_per_thread=4, blocks_per_cu=0, mode=GB_MODE_PERSISTENT, solo_edges=0):
-        check(lib().gb_bfs_config(self._h, block, edges_per_thread, blocks_per_cu, mode, solo_edges))
+    def config(self, block=256, edges_per_thread=4, blocks_per_cu=0, mode=GB_MODE_ASYNC, dense_edges=0):
+        """Kernel tunables; dense_edges=0 selects the default threshold."""
+        check(lib().gb_bfs_config(self._h, block, edges_per_thread, blocks_per_cu, mode, dense_edges))
 
     def get_config(self):
         b, k, bpc, mode = C.c_int(), C.c_int(), C.c_int(), C.c_int()
-        solo = C.c_int64()
-        check(lib().gb_bfs_get_config(self._h, C.byref(b), C.byref(k), C.byref(bpc), C.byref(mode), C.byref(solo)))
+        dense = C.c_int64()
+        check(lib().gb_bfs_get_config(self._h, C.byref(b), C.byref(k), C.byref(bpc), C.byref(mode), C.byref(dense)))
         return {"block": b.value, "edges_per_thread": k.value, "blocks_per_cu": bpc.value, "mode": mode.value,
-                "solo_edges": solo.value}
+                "dense_edges": dense.value}
 
     def memory_bytes(self):
         b = C.c_int64()

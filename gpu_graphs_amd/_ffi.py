@@ -24,7 +24,7 @@ GB_KEEP_ORDER = 4
 GB_DROP_SELF_LOOPS = 8
 GB_AUTO_N = 16
 
-GB_MODE_PERSISTENT = 0
+GB_MODE_ASYNC = 0
 GB_MODE_LEVEL_LOOP = 1
 
 _NAMES = {

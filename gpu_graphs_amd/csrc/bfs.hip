@@ -1,32 +1,41 @@
 // Single-source top-down BFS for gfx950.
 //
 // Replaces the reference's level loop (1D Partition/src/gpuBFS.cu:66-100) and
-// its kernels `linear_bfs` (gpuBFS_kernels.cu:16-80, thread per queue entry,
+// its kernels `linear_bfs` (gpuBFS_kernels.cu:16-80: thread per queue entry,
 // serial neighbour loop, one global atomicAdd per edge, no visited filter) and
 // the unlaunched degree-tiered `expand_contract_bfs` (gpuBFS_kernels.cu:
 // 249-478).  Level semantics follow the reference CPU oracle cpuBFS
 // (1D Partition/src/cpuBFS.cpp:34-58): a vertex first seen while expanding
 // level L gets level L+1.
 //
-// Design (one GPU, whole graph resident in HBM):
-//  * Frontier entries are (first CSR entry, exclusive degree prefix, vertex).
-//    Entries get their queue slot and their edge offset from ONE 64-bit
-//    atomicAdd per workgroup tile on a packed (count << 32 | edges) counter,
-//    so the degree prefix needed for edge-parallel work distribution comes
-//    out of the enqueue itself -- no separate scan pass per level.
-//  * A level is cut into tiles of T = BLOCK*K consecutive frontier edges.
-//    The enqueuer records, for every tile boundary inside a new entry's edge
-//    range, which entry owns it (tile_start), so a workgroup finds its
-//    entries with one load.  Every thread then owns K edges of the tile
-//    (coalesced column reads), whatever the degrees: hubs with 10^5 edges are
-//    spread over many workgroups instead of one thread.
-//  * Visited claims: plain load of the bitmap word (may be stale, never
-//    wrongly "visited"), then atomicOr; the winner writes level L+1 (and the
-//    parent).  Levels are therefore deterministic.
-//  * Winners are appended to an LDS list with wave-aggregated LDS atomics
-//    (ballot + mbcnt), scanned once per tile (block scan) and written out.
-//  * Persistent mode runs all levels in one cooperative launch with a grid
-//    barrier between levels; level-loop mode launches once per level.
+// Design (one GPU, whole graph resident in HBM).  Measured on MI355X: one
+// same-address 64-bit atomic costs ~12 ns (they serialise), a random
+// atomicOr into an L2-resident bitmap ~0.04 ns of chip time, back-to-back
+// kernel launches ~1.5 us, a software grid barrier 17-57 us.  Hence no grid
+// barrier and no per-tile atomics on a shared counter in large levels:
+//
+//  * A level L runs as two launches, `expand(L)` then `compact(L)`, issued
+//    back to back with no host synchronisation; the host only checks a done
+//    flag once per batch of levels.
+//  * The frontier of a level is a queue of entries (vertex, first CSR entry,
+//    exclusive prefix of degrees).  expand(L) cuts the level's edges into
+//    tiles of T = BLOCK*K edges; tile_start[t] names the entry owning edge
+//    t*T, so each workgroup loads its entries with one lookup and every
+//    thread owns K edges of the tile (coalesced column reads) whatever the
+//    degree distribution: hubs are spread over many workgroups.
+//  * Claims: plain load of the visited-bitmap word (may be stale, never
+//    wrongly "visited"), then atomicOr; the winner writes level L+1 and the
+//    parent.  Levels are deterministic.
+//  * Sparse levels (at most `dense_edges` frontier edges) build the next
+//    queue directly: winners go to an LDS list (wave-aggregated LDS atomics,
+//    ballot + mbcnt), get a block scan, and ONE 64-bit atomicAdd per tile on
+//    a packed (count << 32 | edges) counter gives both their queue slots and
+//    their edge offsets.
+//  * Dense levels skip the enqueue.  compact(L) rebuilds the queue from the
+//    bitmaps: new = visited & ~prev, with a single-pass exclusive scan of
+//    (count, edges) over 4096-vertex chunks -- per-thread serial sums, a
+//    wave64 scan, an LDS-staged block scan and a decoupled look-back across
+//    chunks (Merrill & Garland 2016) -- so no counter is shared.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,29 +48,30 @@ namespace gg {
 
 namespace {
 
-// ctl[] layout (64-bit words).  Every word that workgroups use to agree with
-// each other (ring counters, barrier, abort) is read and written only with
-// atomic read-modify-write operations, which are coherent across the XCDs'
-// separate L2 caches; plain loads of these words could return a stale line.
+// ctl[] layout (64-bit words).
 enum : int {
-    CTL_RING0 = 0,      // 3 packed frontier counters: level L uses ring[L % 3]
-    CTL_BAR_COUNT = 3,  // grid barrier arrivals, monotonic (never reset in a launch)
-    CTL_ABORT = 4,      // set when a barrier wait times out; every workgroup then exits
-    CTL_NREACH = 5,     // reached vertices
-    CTL_MREACH = 6,     // sum of degrees of reached vertices
-    CTL_MAXLEV = 7,     // largest level assigned
-    CTL_ERR = 8,        // internal consistency error (non-zero = bad)
-    CTL_BAR_SEEN = 9,   // on timeout: arrivals observed by the workgroup that gave up
-    CTL_BAR_WANT = 10,  // on timeout: arrivals it waited for
+    CTL_RING0 = 0,   // 3 packed (count << 32 | edges) frontier counters: level L uses ring[L % 3]
+    CTL_NREACH = 3,  // reached vertices
+    CTL_MREACH = 4,  // sum of degrees of reached vertices
+    CTL_MAXLEV = 5,  // largest level assigned
+    CTL_DONE = 6,    // set by the first expand() that finds an empty frontier
+    CTL_ERR = 7,     // internal consistency error (bit set)
     CTL_WORDS = 16
 };
+enum : unsigned long long { ERR_TILE = 1, ERR_LOOKBACK = 2 };
 
-enum : unsigned long long { ERR_TILE = 1, ERR_LEVELS = 2 };
+constexpr int kCompactBlock = 256;
+constexpr int kCompactRounds = 16;                          // vertices per thread
+constexpr int kChunk = kCompactBlock * kCompactRounds;      // vertices per compaction chunk
+constexpr unsigned long long kFlagAgg = 1ull << 62;         // look-back: chunk aggregate published
+constexpr unsigned long long kFlagPrefix = 2ull << 62;      // look-back: inclusive prefix published
+constexpr unsigned long long kValueMask = (1ull << 62) - 1;
 
 struct BfsArgs {
     const int64_t* __restrict__ offsets;
     const int32_t* __restrict__ cols;
-    uint32_t* visited;
+    uint32_t* vis;    // claimed vertices
+    uint32_t* prev;   // vertices already placed in (or handled by) a frontier queue
     int32_t* levels;
     int32_t* parents;
     int64_t* q_rb[2];
@@ -69,10 +79,14 @@ struct BfsArgs {
     int32_t* q_v[2];
     uint32_t* tile_start[2];
     unsigned long long* ctl;
+    unsigned long long* lb[2];  // look-back status per compaction chunk, by level parity
     int64_t n;
     uint32_t nwords;
+    uint32_t nchunks;
+    uint32_t dense_edges;
+    uint32_t tile_edges;        // T of the expand kernel (tile_start granularity)
     int32_t source;
-    uint64_t spin_limit;  // wall-clock ticks a barrier may wait before aborting
+    uint64_t spin_limit;        // wall-clock ticks a look-back may wait
 };
 
 template <int BLOCK, int K>
@@ -80,67 +94,30 @@ struct TileLds {
     static constexpr int T = BLOCK * K;
     uint32_t eoff[T + 2];     // edge prefix of the tile's entries (+ end sentinel)
     int64_t rb[T + 1];        // first CSR entry of each tile entry
-    int32_t v[T + 1];         // vertex id of each tile entry (parents only)
-    int64_t win_rb[T];        // winners with degree > 0: first CSR entry
+    int32_t v[T + 1];         // vertex id of each tile entry
+    int64_t win_rb[T];        // sparse levels: winners with degree > 0
     uint32_t win_deg[T + 1];  // their degree, then (in place) exclusive prefix
-    int32_t win_v[T];         // their vertex id
+    int32_t win_v[T];
     uint32_t wave_tmp[BLOCK / 64];
     uint32_t nwin;
-    uint32_t nreach;
+    uint32_t nzero;
     unsigned long long base;
 };
 
-// Coherent read of a control word (atomic add of zero).
-__device__ __forceinline__ unsigned long long ctl_read(unsigned long long* p) {
+__device__ __forceinline__ unsigned long long rmw_read(unsigned long long* p) {
     return __hip_atomic_fetch_add(p, 0ull, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Grid barrier over all workgroups of a cooperative launch.  The arrival
-// counter only grows: barrier number `epoch` (0, 1, ...) is complete when it
-// reaches (epoch + 1) * nblocks, so no workgroup reads anything before it
-// arrives and a stale read can only delay, never release, a waiter.  The
-// wait is bounded by `spin_limit` wall-clock ticks: on timeout the waiter
-// raises CTL_ABORT (recording what it saw) and every workgroup leaves the
-// BFS loop, so a launch that is not fully co-resident ends with an error
-// instead of hanging.  Returns false when the BFS must stop.
-__device__ bool grid_barrier(const BfsArgs& a, uint32_t nblocks, unsigned long long epoch) {
-    __shared__ int s_ok;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();  // release this workgroup's queue / level / tile_start stores
-        const unsigned long long target = (epoch + 1) * nblocks;
-        unsigned long long seen = __hip_atomic_fetch_add(&a.ctl[CTL_BAR_COUNT], 1ull, __ATOMIC_ACQ_REL,
-                                                         __HIP_MEMORY_SCOPE_AGENT) + 1;
-        int ok = 1;
-        const uint64_t t0 = wall_clock64();
-        while (seen < target) {
-            __builtin_amdgcn_s_sleep(8);
-            seen = ctl_read(&a.ctl[CTL_BAR_COUNT]);
-            if (seen >= target) break;
-            if (ctl_read(&a.ctl[CTL_ABORT])) {
-                ok = 0;
-                break;
-            }
-            if (static_cast<uint64_t>(wall_clock64()) - t0 > a.spin_limit) {
-                if (atomicOr(&a.ctl[CTL_ABORT], 1ull) == 0) {
-                    atomicExch(&a.ctl[CTL_BAR_SEEN], seen);
-                    atomicExch(&a.ctl[CTL_BAR_WANT], target);
-                }
-                ok = 0;
-                break;
-            }
-        }
-        __threadfence();  // acquire the other workgroups' stores
-        s_ok = ok;
-    }
-    __syncthreads();
-    return s_ok != 0;
+// Writes tile_start[q][t] = slot for every tile boundary t*T in [eo, eo + d).
+__device__ __forceinline__ void mark_tiles(uint32_t* ts, uint32_t T, uint32_t eo, uint32_t d, uint32_t slot) {
+    const uint64_t stop = uint64_t(eo) + d;
+    for (uint64_t tt = (uint64_t(eo) + T - 1) / T; tt * T < stop; ++tt) ts[tt] = slot;
 }
 
 // Expands tile t of level L: frontier edges [t*T, min((t+1)*T, E)).
 template <int BLOCK, int K>
 __device__ void expand_tile(const BfsArgs& a, TileLds<BLOCK, K>& s, int L, uint32_t t, uint32_t ntiles,
-                            uint32_t F, uint32_t E) {
+                            uint32_t F, uint32_t E, bool dense) {
     constexpr uint32_t T = BLOCK * K;
     const int cur = L & 1, nxt = cur ^ 1;
     const uint32_t tid = threadIdx.x;
@@ -150,7 +127,7 @@ __device__ void expand_tile(const BfsArgs& a, TileLds<BLOCK, K>& s, int L, uint3
     const uint32_t i0 = a.tile_start[cur][t];
     const uint32_t i1 = (t + 1 < ntiles) ? a.tile_start[cur][t + 1] : F - 1;
     uint32_t c = i1 - i0 + 1;
-    if (i1 < i0 || c > T + 1 || i1 >= F) {  // cannot happen; guard the LDS arrays
+    if (i1 < i0 || c > T + 1 || i1 >= F) {  // cannot happen; guards the LDS arrays
         if (tid == 0) atomicOr(&a.ctl[CTL_ERR], ERR_TILE);
         c = 0;
     }
@@ -158,14 +135,14 @@ __device__ void expand_tile(const BfsArgs& a, TileLds<BLOCK, K>& s, int L, uint3
         if (j < c) {
             s.eoff[j] = a.q_eoff[cur][i0 + j];
             s.rb[j] = a.q_rb[cur][i0 + j];
-            if (a.parents) s.v[j] = a.q_v[cur][i0 + j];
+            s.v[j] = a.q_v[cur][i0 + j];
         } else {
             s.eoff[j] = (i0 + c < F) ? a.q_eoff[cur][i0 + c] : E;
         }
     }
     if (tid == 0) {
         s.nwin = 0;
-        s.nreach = 0;
+        s.nzero = 0;
     }
     __syncthreads();
     if (c == 0) return;
@@ -183,7 +160,7 @@ __device__ void expand_tile(const BfsArgs& a, TileLds<BLOCK, K>& s, int L, uint3
         if (act[k]) {
             uint32_t lo = 0, hi = c - 1;  // last entry with eoff <= e
             while (lo < hi) {
-                uint32_t mid = (lo + hi + 1) >> 1;
+                const uint32_t mid = (lo + hi + 1) >> 1;
                 if (s.eoff[mid] <= e) lo = mid;
                 else hi = mid - 1;
             }
@@ -193,47 +170,54 @@ __device__ void expand_tile(const BfsArgs& a, TileLds<BLOCK, K>& s, int L, uint3
     }
     uint32_t word[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) word[k] = act[k] ? a.visited[static_cast<uint32_t>(u[k]) >> 5] : ~0u;
+    for (int k = 0; k < K; ++k) word[k] = act[k] ? a.vis[static_cast<uint32_t>(u[k]) >> 5] : ~0u;
 
     const uint32_t lane = tid & 63u;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const uint32_t bit = 1u << (static_cast<uint32_t>(u[k]) & 31u);
+        const uint32_t uu = static_cast<uint32_t>(u[k]);
+        const uint32_t bit = 1u << (uu & 31u);
         bool win = false;
-        int64_t rb = 0;
-        uint32_t deg = 0;
         if (act[k] && !(word[k] & bit)) {
-            uint32_t old = atomicOr(&a.visited[static_cast<uint32_t>(u[k]) >> 5], bit);
+            const uint32_t old = atomicOr(&a.vis[uu >> 5], bit);
             if (!(old & bit)) {
                 win = true;
-                a.levels[u[k]] = L + 1;
-                if (a.parents) a.parents[u[k]] = s.v[owner[k]];
-                rb = a.offsets[u[k]];
-                deg = static_cast<uint32_t>(a.offsets[u[k] + 1] - rb);
+                a.levels[uu] = L + 1;
+                if (a.parents) a.parents[uu] = s.v[owner[k]];
             }
         }
-        const uint64_t any = wave_ballot(win);
-        if (any) {
-            const uint64_t keep = wave_ballot(win && deg > 0);
-            if (lane == 0) atomicAdd(&s.nreach, static_cast<uint32_t>(__popcll(any)));
-            if (keep) {
-                uint32_t pos = 0;
-                if (lane == 0) pos = atomicAdd(&s.nwin, static_cast<uint32_t>(__popcll(keep)));
-                pos = __shfl(pos, 0, 64);
-                if (win && deg > 0) {
-                    const uint32_t slot = pos + rank_below(keep);
-                    s.win_rb[slot] = rb;
-                    s.win_deg[slot] = deg;
-                    s.win_v[slot] = u[k];
-                }
+        if (dense) continue;  // compact() builds the next queue from the bitmaps
+        int64_t rb = 0;
+        uint32_t deg = 0;
+        if (win) {
+            atomicOr(&a.prev[uu >> 5], bit);  // queued (or degree 0): compact() must skip it
+            rb = a.offsets[uu];
+            deg = static_cast<uint32_t>(a.offsets[uu + 1] - rb);
+        }
+        const uint64_t keep = wave_ballot(win && deg > 0);
+        const uint64_t zero = wave_ballot(win && deg == 0);
+        if (zero && lane == 0) atomicAdd(&s.nzero, static_cast<uint32_t>(__popcll(zero)));
+        if (keep) {
+            uint32_t pos = 0;
+            if (lane == 0) pos = atomicAdd(&s.nwin, static_cast<uint32_t>(__popcll(keep)));
+            pos = __shfl(pos, 0, 64);
+            if (win && deg > 0) {
+                const uint32_t slot = pos + rank_below(keep);
+                s.win_rb[slot] = rb;
+                s.win_deg[slot] = deg;
+                s.win_v[slot] = u[k];
             }
         }
+    }
+    if (dense) {
+        __syncthreads();
+        return;
     }
     __syncthreads();
 
     const uint32_t nwin = s.nwin;
-    if (tid == 0 && s.nreach) {
-        atomicAdd(&a.ctl[CTL_NREACH], static_cast<unsigned long long>(s.nreach));
+    if (tid == 0 && s.nzero) {  // degree-0 winners are reached but never queued
+        atomicAdd(&a.ctl[CTL_NREACH], static_cast<unsigned long long>(s.nzero));
         atomicMax(&a.ctl[CTL_MAXLEV], static_cast<unsigned long long>(L + 1));
     }
     if (nwin == 0) {
@@ -252,8 +236,7 @@ __device__ void expand_tile(const BfsArgs& a, TileLds<BLOCK, K>& s, int L, uint3
     }
     if (tid == 0) {
         s.win_deg[nwin] = running;
-        s.base = atomicAdd(&a.ctl[CTL_RING0 + (L + 1) % 3],
-                           (static_cast<unsigned long long>(nwin) << 32) | running);
+        s.base = atomicAdd(&a.ctl[CTL_RING0 + (L + 1) % 3], (static_cast<unsigned long long>(nwin) << 32) | running);
     }
     __syncthreads();
     const uint32_t slot0 = static_cast<uint32_t>(s.base >> 32);
@@ -264,83 +247,190 @@ __device__ void expand_tile(const BfsArgs& a, TileLds<BLOCK, K>& s, int L, uint3
         const uint32_t d = s.win_deg[i + 1] - s.win_deg[i];
         a.q_rb[nxt][slot] = s.win_rb[i];
         a.q_eoff[nxt][slot] = eo;
-        if (a.parents) a.q_v[nxt][slot] = s.win_v[i];
-        const uint64_t stop = uint64_t(eo) + d;
-        for (uint64_t tt = (uint64_t(eo) + T - 1) / T; tt * T < stop; ++tt) a.tile_start[nxt][tt] = slot;
+        a.q_v[nxt][slot] = s.win_v[i];
+        mark_tiles(a.tile_start[nxt], T, eo, d, slot);
     }
     __syncthreads();
 }
 
-// Reads the packed frontier counter of level L once per workgroup and
-// broadcasts it through LDS, so all threads of a workgroup agree.  After an
-// abort it reads as empty, so a late-starting workgroup does nothing.
-__device__ __forceinline__ unsigned long long level_counter(const BfsArgs& a, int L) {
-    __shared__ unsigned long long s_cnt;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        s_cnt = ctl_read(&a.ctl[CTL_RING0 + L % 3]);
-        if (ctl_read(&a.ctl[CTL_ABORT])) s_cnt = 0;
-    }
-    __syncthreads();
-    return s_cnt;
-}
-
+// Level L, step 1: expand the frontier queue of level L.
 template <int BLOCK, int K>
-__device__ void expand_level(const BfsArgs& a, TileLds<BLOCK, K>& s, int L, unsigned long long cnt, uint32_t b,
-                             uint32_t G) {
+__global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(BfsArgs a, int L) {
+    __shared__ TileLds<BLOCK, K> s;
     constexpr uint32_t T = BLOCK * K;
+    // Levels queued past the end of the search are no-ops: once done, the
+    // ring slots are no longer reset and would hold stale counts.
+    if (a.ctl[CTL_DONE]) return;
+    const unsigned long long cnt = a.ctl[CTL_RING0 + L % 3];  // final: written by earlier launches
     const uint32_t F = static_cast<uint32_t>(cnt >> 32), E = static_cast<uint32_t>(cnt);
-    if (b == 0 && threadIdx.x == 0) {
-        // ring[(L+2)%3] held level L-1's count, read by every workgroup
-        // before the barrier that ended level L-1; it next counts level L+2.
-        atomicExch(&a.ctl[CTL_RING0 + (L + 2) % 3], 0ull);
-        atomicAdd(&a.ctl[CTL_MREACH], static_cast<unsigned long long>(E));
+    if (F == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl[CTL_DONE] = 1;
+        return;
     }
-    if (F == 0) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // ring[(L+2)%3] held level L-1's counter; it next counts level L+2.
+        a.ctl[CTL_RING0 + (L + 2) % 3] = 0;
+        atomicAdd(&a.ctl[CTL_MREACH], static_cast<unsigned long long>(E));
+        atomicAdd(&a.ctl[CTL_NREACH], static_cast<unsigned long long>(F));
+        atomicMax(&a.ctl[CTL_MAXLEV], static_cast<unsigned long long>(L));
+    }
+    const bool dense = E > a.dense_edges;
     const uint32_t ntiles = static_cast<uint32_t>((uint64_t(E) + T - 1) / T);
-    for (uint32_t t = b; t < ntiles; t += G) expand_tile<BLOCK, K>(a, s, L, t, ntiles, F, E);
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) expand_tile<BLOCK, K>(a, s, L, t, ntiles, F, E, dense);
 }
 
-template <int BLOCK, int K>
-__global__ __launch_bounds__(BLOCK) void bfs_level_kernel(BfsArgs a, int L) {
-    __shared__ TileLds<BLOCK, K> s;
-    expand_level<BLOCK, K>(a, s, L, level_counter(a, L), blockIdx.x, gridDim.x);
-}
-
-// All levels in one cooperative launch.  A BFS has at most n levels; the
-// loop is also bounded by that, so a corrupted counter cannot spin forever.
-template <int BLOCK, int K>
-__global__ __launch_bounds__(BLOCK) void bfs_persistent_kernel(BfsArgs a) {
-    __shared__ TileLds<BLOCK, K> s;
-    for (int L = 0;; ++L) {
-        const unsigned long long cnt = level_counter(a, L);
-        if ((cnt >> 32) == 0) break;  // every workgroup reads the same final value
-        if (L > a.n) {
-            if (threadIdx.x == 0) atomicOr(&a.ctl[CTL_ERR], ERR_LEVELS);
+// Decoupled look-back for chunk c (thread 0 only): publishes the chunk's
+// aggregate, accumulates predecessors until one with an inclusive prefix,
+// publishes its own inclusive prefix and returns the exclusive one.  Waits are
+// bounded; on timeout the BFS is flagged as failed.
+__device__ unsigned long long lookback(const BfsArgs& a, unsigned long long* st, uint32_t c,
+                                       unsigned long long agg) {
+    if (c == 0) {
+        atomicExch(&st[0], kFlagPrefix | agg);
+        return 0;
+    }
+    atomicExch(&st[c], kFlagAgg | agg);
+    unsigned long long excl = 0;
+    int64_t j = static_cast<int64_t>(c) - 1;
+    const uint64_t t0 = wall_clock64();
+    while (j >= 0) {
+        const unsigned long long sj = rmw_read(&st[j]);
+        const unsigned long long flag = sj & ~kValueMask;
+        if (flag == kFlagPrefix) {
+            excl += sj & kValueMask;
             break;
         }
-        expand_level<BLOCK, K>(a, s, L, cnt, blockIdx.x, gridDim.x);
-        if (!grid_barrier(a, gridDim.x, static_cast<unsigned long long>(L))) break;
+        if (flag == kFlagAgg) {
+            excl += sj & kValueMask;
+            --j;
+            continue;
+        }
+        if (static_cast<uint64_t>(wall_clock64()) - t0 > a.spin_limit) {
+            atomicOr(&a.ctl[CTL_ERR], ERR_LOOKBACK);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    atomicExch(&st[c], kFlagPrefix | (excl + agg));
+    return excl;
+}
+
+// Level L, step 2 (dense levels only): queue of level L+1 from the bitmaps.
+// Chunk c covers vertices [c*kChunk, (c+1)*kChunk); thread t takes vertices
+// c*kChunk + r*256 + t for r < 16, so the offsets loads of a wave are
+// contiguous.  The thread's new vertices occupy consecutive queue slots.
+__device__ void compact_chunk(const BfsArgs& a, int L, uint32_t c) {
+    __shared__ unsigned long long s_wave[kCompactBlock / 64];
+    __shared__ unsigned long long s_excl;
+    __shared__ uint32_t s_zero[kCompactBlock / 64];
+    const uint32_t tid = threadIdx.x;
+    const int64_t vbase = int64_t(c) * kChunk;
+    uint32_t newmask = 0;  // bit r: vertex vbase + r*256 + tid is new
+    uint32_t kept = 0, zero = 0, edges = 0;
+#pragma unroll
+    for (int r = 0; r < kCompactRounds; ++r) {
+        const int64_t v = vbase + r * kCompactBlock + tid;
+        if (v < a.n) {
+            const uint32_t w = static_cast<uint32_t>(v >> 5);
+            const uint32_t nb = a.vis[w] & ~a.prev[w];
+            if ((nb >> (v & 31)) & 1u) {
+                const uint32_t d = static_cast<uint32_t>(a.offsets[v + 1] - a.offsets[v]);
+                if (d) {
+                    newmask |= 1u << r;
+                    ++kept;
+                    edges += d;
+                } else {
+                    ++zero;
+                }
+            }
+        }
+    }
+    // Block scan of (kept, edges) packed in 64 bits; zero-degree count summed.
+    const unsigned long long mine = (static_cast<unsigned long long>(kept) << 32) | edges;
+    unsigned long long total;
+    const unsigned long long excl_local = block_exclusive_scan<kCompactBlock>(mine, s_wave, total);
+    const uint32_t zsum = wave_sum(zero);
+    if ((tid & 63) == 0) s_zero[tid >> 6] = zsum;
+    // prev |= vis for the chunk's words: every read of prev above happened
+    // before the block scan's barriers.
+    for (uint32_t i = tid; i < kChunk / 32; i += kCompactBlock) {
+        const int64_t w = vbase / 32 + i;
+        if (w < a.nwords) a.prev[w] = a.vis[w];
+    }
+    if (tid == 0) {
+        const uint32_t p = L & 1;
+        s_excl = lookback(a, a.lb[p], c, total);
+        uint32_t z = 0;
+        for (int i = 0; i < kCompactBlock / 64; ++i) z += s_zero[i];
+        if (z) {
+            atomicAdd(&a.ctl[CTL_NREACH], static_cast<unsigned long long>(z));
+            atomicMax(&a.ctl[CTL_MAXLEV], static_cast<unsigned long long>(L + 1));
+        }
+        if (c == a.nchunks - 1) a.ctl[CTL_RING0 + (L + 1) % 3] = s_excl + total;  // level L+1 totals
+    }
+    __syncthreads();
+    if (!newmask) return;
+    const unsigned long long pre = s_excl + excl_local;
+    uint32_t slot = static_cast<uint32_t>(pre >> 32);
+    uint32_t eo = static_cast<uint32_t>(pre);
+    const int q = (L + 1) & 1;
+#pragma unroll
+    for (int r = 0; r < kCompactRounds; ++r) {
+        if ((newmask >> r) & 1u) {
+            const int64_t v = vbase + r * kCompactBlock + tid;
+            const int64_t rb = a.offsets[v];
+            const uint32_t d = static_cast<uint32_t>(a.offsets[v + 1] - rb);
+            a.q_rb[q][slot] = rb;
+            a.q_eoff[q][slot] = eo;
+            a.q_v[q][slot] = static_cast<int32_t>(v);
+            mark_tiles(a.tile_start[q], a.tile_edges, eo, d, slot);
+            ++slot;
+            eo += d;
+        }
     }
 }
 
-// levels = -1, parents = -1, visited = 0.
+__global__ __launch_bounds__(kCompactBlock) void bfs_compact_kernel(BfsArgs a, int L) {
+    if (a.ctl[CTL_DONE]) return;
+    const unsigned long long cnt = a.ctl[CTL_RING0 + L % 3];
+    const uint32_t F = static_cast<uint32_t>(cnt >> 32), E = static_cast<uint32_t>(cnt);
+    // The look-back status of level L+1 (other parity) starts from zero.
+    unsigned long long* other = a.lb[(L + 1) & 1];
+    for (uint32_t i = blockIdx.x * kCompactBlock + threadIdx.x; i < a.nchunks; i += gridDim.x * kCompactBlock)
+        other[i] = 0;
+    if (F == 0 || E <= a.dense_edges) return;  // sparse level: expand() queued level L+1
+    for (uint32_t c = blockIdx.x; c < a.nchunks; c += gridDim.x) {
+        compact_chunk(a, L, c);
+        __syncthreads();
+    }
+}
+
+// levels = -1, parents = -1, bitmaps and look-back status = 0.
 __global__ void bfs_clear_kernel(BfsArgs a) {
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n; i += stride) {
         a.levels[i] = -1;
         if (a.parents) a.parents[i] = -1;
-        if (i < a.nwords) a.visited[i] = 0u;
+        if (i < a.nwords) {
+            a.vis[i] = 0u;
+            a.prev[i] = 0u;
+        }
+        if (i < a.nchunks) {
+            a.lb[0][i] = 0;
+            a.lb[1][i] = 0;
+        }
     }
 }
 
 // Level 0: the source alone (claimed, level 0, parent = itself).
-__global__ void bfs_seed_kernel(BfsArgs a, uint32_t T) {
+__global__ void bfs_seed_kernel(BfsArgs a) {
     const int32_t s = a.source;
     const int64_t rb = a.offsets[s];
     const uint32_t d = static_cast<uint32_t>(a.offsets[s + 1] - rb);
     if (threadIdx.x == 0) {
-        a.visited[static_cast<uint32_t>(s) >> 5] = 1u << (static_cast<uint32_t>(s) & 31u);
+        const uint32_t bit = 1u << (static_cast<uint32_t>(s) & 31u);
+        a.vis[static_cast<uint32_t>(s) >> 5] = bit;
+        a.prev[static_cast<uint32_t>(s) >> 5] = bit;
         a.levels[s] = 0;
         if (a.parents) a.parents[s] = s;
         a.q_rb[0][0] = rb;
@@ -348,21 +438,19 @@ __global__ void bfs_seed_kernel(BfsArgs a, uint32_t T) {
         a.q_v[0][0] = s;
         for (int i = 0; i < CTL_WORDS; ++i) a.ctl[i] = 0;
         a.ctl[CTL_RING0] = d ? ((1ull << 32) | d) : 0ull;
-        a.ctl[CTL_NREACH] = 1;
+        a.ctl[CTL_NREACH] = d ? 0 : 1;  // expand(0) counts a source with edges
     }
-    for (uint64_t tt = threadIdx.x; tt * T < d; tt += blockDim.x) a.tile_start[0][tt] = 0;
+    for (uint64_t tt = threadIdx.x; tt * a.tile_edges < d; tt += blockDim.x) a.tile_start[0][tt] = 0;
 }
 
-using PersistentFn = void (*)(BfsArgs);
-using LevelFn = void (*)(BfsArgs, int);
+using ExpandFn = void (*)(BfsArgs, int);
 
 struct KernelSet {
     int block, k;
-    PersistentFn persistent;
-    LevelFn level;
+    ExpandFn expand;
 };
 
-#define GG_KSET(B, K) {B, K, bfs_persistent_kernel<B, K>, bfs_level_kernel<B, K>}
+#define GG_KSET(B, K) {B, K, bfs_expand_kernel<B, K>}
 const KernelSet kKernels[] = {GG_KSET(256, 1), GG_KSET(256, 2), GG_KSET(256, 4), GG_KSET(256, 8),
                               GG_KSET(512, 1), GG_KSET(512, 2), GG_KSET(512, 4), GG_KSET(1024, 1),
                               GG_KSET(1024, 2), GG_KSET(128, 4), GG_KSET(128, 8), GG_KSET(64, 8),
@@ -382,6 +470,7 @@ bool bfs_config_supported(int block, int k) { return find_kernels(block, k) != n
 DeviceGraph::DeviceGraph(const HostGraph& g, int dev) : device(dev), n(g.n), m(g.m()) {
     if (m >= (int64_t(1) << 32) - 1) fail(GB_EINVAL, "graphs with 2^32 or more CSR entries are not supported");
     if (n < 1) fail(GB_EINVAL, "graph has no vertices");
+    if (n >= (int64_t(1) << 30)) fail(GB_EINVAL, "graphs with 2^30 or more vertices are not supported");
     int count = 0;
     GG_HIP(hipGetDeviceCount(&count));
     if (dev < 0 || dev >= count) fail(GB_EINVAL, "device " + std::to_string(dev) + " does not exist");
@@ -392,12 +481,17 @@ DeviceGraph::DeviceGraph(const HostGraph& g, int dev) : device(dev), n(g.n), m(g
     int clock_khz = 0;
     GG_HIP(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, device));
     if (clock_khz <= 0) clock_khz = 100000;
-    spin_limit_ticks = static_cast<uint64_t>(clock_khz) * 1000ull * kBarrierTimeoutSeconds;
+    spin_limit_ticks = static_cast<uint64_t>(clock_khz) * 1000ull * kSpinTimeoutSeconds;
+    cfg.dense_edges = default_dense_edges();
     try {
         GG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         GG_HIP(hipEventCreate(&ev_start));
         GG_HIP(hipEventCreate(&ev_stop));
         GG_HIP(hipEventCreate(&ev_kernel));
+        GG_HIP(hipEventCreateWithFlags(&ev_batch[0], hipEventDisableTiming));
+        GG_HIP(hipEventCreateWithFlags(&ev_batch[1], hipEventDisableTiming));
+        GG_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_ctl), sizeof(unsigned long long) * 2 * kCtlWords,
+                             hipHostMallocDefault));
         GG_HIP(hipMalloc(&offsets, sizeof(int64_t) * (n + 1)));
         GG_HIP(hipMalloc(&cols, sizeof(int32_t) * std::max<int64_t>(m, 1)));
         graph_bytes = sizeof(int64_t) * (n + 1) + sizeof(int32_t) * std::max<int64_t>(m, 1);
@@ -417,17 +511,20 @@ DeviceGraph::~DeviceGraph() {
     if (cols) (void)hipFree(cols);
     offsets = nullptr;
     cols = nullptr;
-    if (ev_start) (void)hipEventDestroy(ev_start);
-    if (ev_stop) (void)hipEventDestroy(ev_stop);
-    if (ev_kernel) (void)hipEventDestroy(ev_kernel);
+    if (h_ctl) (void)hipHostFree(h_ctl);
+    h_ctl = nullptr;
+    for (hipEvent_t* e : {&ev_start, &ev_stop, &ev_kernel, &ev_batch[0], &ev_batch[1]}) {
+        if (*e) (void)hipEventDestroy(*e);
+        *e = nullptr;
+    }
     if (stream) (void)hipStreamDestroy(stream);
-    ev_start = ev_stop = ev_kernel = nullptr;
     stream = nullptr;
 }
 
 void DeviceGraph::free_workspace() {
-    void* ptrs[] = {ws.visited, ws.q_rb[0], ws.q_rb[1], ws.q_eoff[0], ws.q_eoff[1], ws.q_v[0], ws.q_v[1],
-                    ws.tile_start[0], ws.tile_start[1], ws.ctl, ws.levels, ws.parents};
+    void* ptrs[] = {ws.vis,           ws.prev,          ws.q_rb[0], ws.q_rb[1], ws.q_eoff[0], ws.q_eoff[1],
+                    ws.q_v[0],        ws.q_v[1],        ws.ctl,     ws.lb[0],   ws.lb[1],     ws.levels,
+                    ws.parents,       ws.tile_start[0], ws.tile_start[1]};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     ws = BfsWorkspace{};
@@ -436,10 +533,10 @@ void DeviceGraph::free_workspace() {
 void DeviceGraph::ensure_workspace() {
     const int T = cfg.block * cfg.edges_per_thread;
     const int64_t tile_cap = (m + T - 1) / T + 2;
-    if (ws.ctl && ws.tile_edges == T && ws.tile_cap >= tile_cap) return;
-    if (ws.ctl && ws.tile_edges != T) {  // only the tile index depends on T
-        if (ws.tile_start[0]) (void)hipFree(ws.tile_start[0]);
-        if (ws.tile_start[1]) (void)hipFree(ws.tile_start[1]);
+    if (ws.ctl && ws.tile_edges == T) return;
+    if (ws.ctl) {  // only the tile index depends on T
+        (void)hipFree(ws.tile_start[0]);
+        (void)hipFree(ws.tile_start[1]);
         ws.bytes -= 2 * sizeof(uint32_t) * ws.tile_cap;
         ws.tile_start[0] = ws.tile_start[1] = nullptr;
         GG_HIP(hipMalloc(&ws.tile_start[0], sizeof(uint32_t) * tile_cap));
@@ -450,20 +547,23 @@ void DeviceGraph::ensure_workspace() {
         return;
     }
     const int64_t nwords = (n + 31) / 32;
+    const int64_t nchunks = (n + kChunk - 1) / kChunk;
     size_t bytes = 0;
     auto alloc = [&](auto** p, size_t sz) {
         GG_HIP(hipMalloc(reinterpret_cast<void**>(p), sz));
         bytes += sz;
     };
     try {
-        alloc(&ws.visited, sizeof(uint32_t) * nwords);
+        alloc(&ws.vis, sizeof(uint32_t) * nwords);
+        alloc(&ws.prev, sizeof(uint32_t) * nwords);
         for (int i = 0; i < 2; ++i) {
             alloc(&ws.q_rb[i], sizeof(int64_t) * n);
             alloc(&ws.q_eoff[i], sizeof(uint32_t) * n);
             alloc(&ws.q_v[i], sizeof(int32_t) * n);
             alloc(&ws.tile_start[i], sizeof(uint32_t) * tile_cap);
+            alloc(&ws.lb[i], sizeof(unsigned long long) * nchunks);
         }
-        alloc(&ws.ctl, sizeof(unsigned long long) * CTL_WORDS);
+        alloc(&ws.ctl, sizeof(unsigned long long) * kCtlWords);
         alloc(&ws.levels, sizeof(int32_t) * n);
         alloc(&ws.parents, sizeof(int32_t) * n);
     } catch (...) {
@@ -472,10 +572,12 @@ void DeviceGraph::ensure_workspace() {
     }
     ws.tile_cap = tile_cap;
     ws.tile_edges = T;
+    ws.nchunks = nchunks;
     ws.bytes = bytes;
 }
 
 void DeviceGraph::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats* st) {
+    static_assert(kCtlWords == CTL_WORDS, "control block size");
     if (source < 0 || source >= n)
         fail(GB_ERANGE, "source " + std::to_string(source) + " outside [0, " + std::to_string(n) + ")");
     if (!d_levels) fail(GB_EINVAL, "levels buffer is null");
@@ -483,12 +585,12 @@ void DeviceGraph::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_
     if (!ks) fail(GB_EINVAL, "unsupported block / edges_per_thread combination");
     DeviceGuard guard(device);
     ensure_workspace();
-    const uint32_t T = static_cast<uint32_t>(cfg.block * cfg.edges_per_thread);
 
     BfsArgs a;
     a.offsets = offsets;
     a.cols = cols;
-    a.visited = ws.visited;
+    a.vis = ws.vis;
+    a.prev = ws.prev;
     a.levels = d_levels;
     a.parents = d_parents;
     for (int i = 0; i < 2; ++i) {
@@ -496,82 +598,102 @@ void DeviceGraph::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_
         a.q_eoff[i] = ws.q_eoff[i];
         a.q_v[i] = ws.q_v[i];
         a.tile_start[i] = ws.tile_start[i];
+        a.lb[i] = ws.lb[i];
     }
     a.ctl = ws.ctl;
     a.n = n;
     a.nwords = static_cast<uint32_t>((n + 31) / 32);
+    a.nchunks = static_cast<uint32_t>(ws.nchunks);
+    a.dense_edges = static_cast<uint32_t>(std::min<int64_t>(cfg.dense_edges, 0xffffffffll));
+    a.tile_edges = static_cast<uint32_t>(cfg.block * cfg.edges_per_thread);
     a.source = source;
     a.spin_limit = spin_limit_ticks;
 
     int occ = 0;
-    if (cfg.mode == GB_MODE_PERSISTENT)
-        GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(ks->persistent),
-                                                           cfg.block, 0));
-    else
-        GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(ks->level),
-                                                           cfg.block, 0));
+    GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(ks->expand), cfg.block, 0));
     if (occ < 1) fail(GB_EDEVICE, "BFS kernel cannot be resident on a CU");
-    // Default: at most 16 waves per CU (half the wave slots), leaving
-    // headroom so every workgroup of the cooperative launch is co-resident.
     const int waves_per_block = (cfg.block + 63) / 64;
     const int default_per_cu = std::max(1, std::min(occ, 16 / waves_per_block));
     const int per_cu = cfg.blocks_per_cu > 0 ? std::min(cfg.blocks_per_cu, occ) : default_per_cu;
     const uint32_t grid = static_cast<uint32_t>(per_cu * num_cus);
+    // compact(): chunks are assigned statically and wait on lower-numbered
+    // chunks, so the whole grid must be resident: 4 blocks of 256 per CU.
+    const uint32_t cgrid = static_cast<uint32_t>(std::min<int64_t>(ws.nchunks, int64_t(num_cus) * 4));
     const uint32_t clear_grid = static_cast<uint32_t>(std::min<int64_t>((n + 255) / 256, int64_t(num_cus) * 8));
+
+    auto launch_level = [&](int L) {
+        hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(cfg.block), 0, stream, a, L);
+        hipLaunchKernelGGL(bfs_compact_kernel, dim3(cgrid), dim3(kCompactBlock), 0, stream, a, L);
+    };
 
     GG_HIP(hipEventRecord(ev_start, stream));
     hipLaunchKernelGGL(bfs_clear_kernel, dim3(clear_grid), dim3(256), 0, stream, a);
-    hipLaunchKernelGGL(bfs_seed_kernel, dim3(1), dim3(256), 0, stream, a, T);
+    hipLaunchKernelGGL(bfs_seed_kernel, dim3(1), dim3(256), 0, stream, a);
     GG_HIP(hipGetLastError());
     GG_HIP(hipEventRecord(ev_kernel, stream));
-    if (cfg.mode == GB_MODE_PERSISTENT) {
-        void* args[] = {&a};
-        GG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(ks->persistent), dim3(grid), dim3(cfg.block),
-                                          args, 0, stream));
-    } else {
-        unsigned long long* h_cnt = nullptr;
-        GG_HIP(hipHostMalloc(&h_cnt, sizeof(unsigned long long) * CTL_WORDS, hipHostMallocDefault));
-        try {
-            unsigned long long cnt = 1ull << 32;
-            for (int64_t L = 0; (cnt >> 32) != 0; ++L) {
-                if (L > n) fail(GB_EDEVICE, "internal error: more BFS levels than vertices");
-                const uint32_t E = L == 0 ? 0u : static_cast<uint32_t>(cnt);
-                uint32_t g = grid;
-                if (L > 0) g = static_cast<uint32_t>(std::min<uint64_t>(grid, (uint64_t(E) + T - 1) / T));
-                hipLaunchKernelGGL(ks->level, dim3(std::max(1u, g)), dim3(cfg.block), 0, stream, a, static_cast<int>(L));
-                GG_HIP(hipGetLastError());
-                GG_HIP(hipMemcpyAsync(h_cnt, ws.ctl, sizeof(unsigned long long) * CTL_WORDS, hipMemcpyDeviceToHost,
-                                      stream));
-                GG_HIP(hipStreamSynchronize(stream));
-                cnt = h_cnt[CTL_RING0 + (L + 1) % 3];
+    int L = 0;
+    if (cfg.mode == GB_MODE_LEVEL_LOOP) {
+        // One level at a time; the host reads the next frontier size.
+        for (;; ++L) {
+            if (L > n) fail(GB_EDEVICE, "internal error: more BFS levels than vertices");
+            launch_level(L);
+            GG_HIP(hipGetLastError());
+            GG_HIP(hipMemcpyAsync(h_ctl, ws.ctl, sizeof(unsigned long long) * kCtlWords, hipMemcpyDeviceToHost,
+                                  stream));
+            GG_HIP(hipStreamSynchronize(stream));
+            if ((h_ctl[CTL_RING0 + (L + 1) % 3] >> 32) == 0) {
+                ++L;
+                launch_level(L);  // sets the done flag and the final counters
+                break;
             }
-        } catch (...) {
-            (void)hipHostFree(h_cnt);
-            throw;
         }
-        GG_HIP(hipHostFree(h_cnt));
+    } else {
+        // Levels are queued ahead with no host synchronisation.  The first
+        // batch covers the previous BFS's depth + 1 (one empty level ends the
+        // search); if the search is deeper, batches of doubling size (at most
+        // 64 levels) are pipelined: the done flag of batch b-1 is checked
+        // while batch b runs.  Extra levels are no-ops on an empty frontier.
+        auto enqueue = [&](int count, int slot) {
+            for (int i = 0; i < count; ++i) launch_level(L++);
+            GG_HIP(hipGetLastError());
+            GG_HIP(hipMemcpyAsync(h_ctl + slot * kCtlWords, ws.ctl, sizeof(unsigned long long) * kCtlWords,
+                                  hipMemcpyDeviceToHost, stream));
+            GG_HIP(hipEventRecord(ev_batch[slot], stream));
+        };
+        auto done = [&](int slot) {
+            GG_HIP(hipEventSynchronize(ev_batch[slot]));
+            return h_ctl[slot * kCtlWords + CTL_DONE] != 0;
+        };
+        enqueue(last_num_levels > 0 ? last_num_levels + 1 : 8, 0);
+        if (!done(0)) {
+            int batch = 8;
+            enqueue(batch, 1);
+            for (int b = 2;; ++b) {
+                if (L > n + 64) fail(GB_EDEVICE, "internal error: more BFS levels than vertices");
+                batch = std::min(batch * 2, 64);
+                enqueue(batch, b & 1);
+                if (done((b - 1) & 1)) break;
+            }
+        }
     }
     GG_HIP(hipEventRecord(ev_stop, stream));
-    unsigned long long h_ctl[CTL_WORDS];
-    GG_HIP(hipMemcpyAsync(h_ctl, ws.ctl, sizeof(h_ctl), hipMemcpyDeviceToHost, stream));
+    unsigned long long fin[kCtlWords];
+    GG_HIP(hipMemcpyAsync(fin, ws.ctl, sizeof(fin), hipMemcpyDeviceToHost, stream));
     GG_HIP(hipStreamSynchronize(stream));
-    if (h_ctl[CTL_ABORT])
-        fail(GB_EDEVICE, "persistent BFS aborted: a grid barrier waited longer than the limit (" +
-                             std::to_string(h_ctl[CTL_BAR_SEEN]) + " of " + std::to_string(h_ctl[CTL_BAR_WANT]) +
-                             " arrivals seen; " + std::to_string(grid) + " workgroups of " +
-                             std::to_string(cfg.block) + " threads were not all co-resident)");
-    if (h_ctl[CTL_ERR] & ERR_TILE) fail(GB_EDEVICE, "internal error: inconsistent frontier tile index");
-    if (h_ctl[CTL_ERR] & ERR_LEVELS) fail(GB_EDEVICE, "internal error: more BFS levels than vertices");
+    if (fin[CTL_ERR] & ERR_TILE) fail(GB_EDEVICE, "internal error: inconsistent frontier tile index");
+    if (fin[CTL_ERR] & ERR_LOOKBACK) fail(GB_EDEVICE, "frontier compaction look-back timed out");
+    if (!fin[CTL_DONE]) fail(GB_EDEVICE, "internal error: BFS did not reach an empty frontier");
+    last_num_levels = static_cast<int>(fin[CTL_MAXLEV]) + 1;
     if (st) {
         float ms = 0.f, kms = 0.f;
         GG_HIP(hipEventElapsedTime(&ms, ev_start, ev_stop));
         GG_HIP(hipEventElapsedTime(&kms, ev_kernel, ev_stop));
         st->ms = ms;
         st->kernel_ms = kms;
-        st->num_levels = static_cast<int32_t>(h_ctl[CTL_MAXLEV]) + 1;
+        st->num_levels = last_num_levels;
         st->reserved = 0;
-        st->n_reached = static_cast<int64_t>(h_ctl[CTL_NREACH]);
-        st->m_reached = static_cast<int64_t>(h_ctl[CTL_MREACH]);
+        st->n_reached = static_cast<int64_t>(fin[CTL_NREACH]);
+        st->m_reached = static_cast<int64_t>(fin[CTL_MREACH]);
     }
 }
 

@@ -260,24 +260,24 @@ int gb_bfs(gb_dgraph* dg, int32_t source, int32_t* levels, int32_t* parents, gb_
     });
 }
 
-int gb_bfs_config(gb_dgraph* dg, int block, int edges_per_thread, int blocks_per_cu, int mode, int64_t solo_edges) {
+int gb_bfs_config(gb_dgraph* dg, int block, int edges_per_thread, int blocks_per_cu, int mode, int64_t dense_edges) {
     return guarded([&] {
         need(dg, "null device graph");
         need(gg::bfs_config_supported(block, edges_per_thread), "unsupported block / edges_per_thread combination");
         need(blocks_per_cu >= 0, "blocks_per_cu must be >= 0");
-        need(mode == GB_MODE_PERSISTENT || mode == GB_MODE_LEVEL_LOOP, "unknown mode");
-        need(solo_edges >= 0, "solo_edges must be >= 0");
+        need(mode == GB_MODE_ASYNC || mode == GB_MODE_LEVEL_LOOP, "unknown mode");
+        need(dense_edges >= 0, "dense_edges must be >= 0");
         gg::BfsConfig& c = dg->d->cfg;
         c.block = block;
         c.edges_per_thread = edges_per_thread;
         c.blocks_per_cu = blocks_per_cu;
         c.mode = mode;
-        c.solo_edges = solo_edges;
+        c.dense_edges = dense_edges > 0 ? dense_edges : dg->d->default_dense_edges();
     });
 }
 
 int gb_bfs_get_config(const gb_dgraph* dg, int* block, int* edges_per_thread, int* blocks_per_cu, int* mode,
-                      int64_t* solo_edges) {
+                      int64_t* dense_edges) {
     return guarded([&] {
         need(dg, "null device graph");
         const gg::BfsConfig& c = dg->d->cfg;
@@ -285,7 +285,7 @@ int gb_bfs_get_config(const gb_dgraph* dg, int* block, int* edges_per_thread, in
         if (edges_per_thread) *edges_per_thread = c.edges_per_thread;
         if (blocks_per_cu) *blocks_per_cu = c.blocks_per_cu;
         if (mode) *mode = c.mode;
-        if (solo_edges) *solo_edges = c.solo_edges;
+        if (dense_edges) *dense_edges = c.dense_edges;
     });
 }
 

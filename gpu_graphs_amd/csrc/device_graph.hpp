@@ -38,27 +38,30 @@ struct DeviceGuard {
     }
 };
 
-// Tunables of the single-source top-down kernel.
+// Tunables of the single-source top-down kernels.
 struct BfsConfig {
-    int block = 256;              // threads per workgroup (4 waves)
+    int block = 256;              // threads per workgroup of expand() (4 waves)
     int edges_per_thread = 4;     // K: a tile is block * K frontier edges
-    int blocks_per_cu = 0;        // 0 = occupancy maximum
-    int mode = GB_MODE_PERSISTENT;
-    int64_t solo_edges = 0;       // frontiers with <= this many edges run on one workgroup
+    int blocks_per_cu = 0;        // 0 = at most 16 waves per CU
+    int mode = GB_MODE_ASYNC;
+    int64_t dense_edges = 0;      // levels with more frontier edges use bitmap compaction
 };
 
 // Device workspace of one BFS (reused across calls on the same handle).
 struct BfsWorkspace {
-    uint32_t* visited = nullptr;          // ceil(n/32) words
-    int64_t* q_rb[2] = {nullptr, nullptr};    // frontier: first CSR entry of the vertex
-    uint32_t* q_eoff[2] = {nullptr, nullptr}; // frontier: exclusive prefix of degrees in the level
-    int32_t* q_v[2] = {nullptr, nullptr};     // frontier: vertex id (for parents)
+    uint32_t* vis = nullptr;                    // ceil(n/32) words: claimed vertices
+    uint32_t* prev = nullptr;                   // ceil(n/32) words: vertices already queued
+    int64_t* q_rb[2] = {nullptr, nullptr};      // frontier: first CSR entry of the vertex
+    uint32_t* q_eoff[2] = {nullptr, nullptr};   // frontier: exclusive prefix of degrees in the level
+    int32_t* q_v[2] = {nullptr, nullptr};       // frontier: vertex id
     uint32_t* tile_start[2] = {nullptr, nullptr};  // frontier entry that owns edge t*T
-    unsigned long long* ctl = nullptr;    // counters ring, barrier, stats (see bfs.hip)
-    int32_t* levels = nullptr;            // levels staging for host-pointer calls
+    unsigned long long* lb[2] = {nullptr, nullptr};  // compaction look-back status per chunk
+    unsigned long long* ctl = nullptr;          // counters ring, stats, flags (see bfs.hip)
+    int32_t* levels = nullptr;                  // levels staging for host-pointer calls
     int32_t* parents = nullptr;
-    int64_t tile_cap = 0;                 // entries per tile_start buffer
-    int tile_edges = 0;                   // T the tile_start buffers were sized for
+    int64_t tile_cap = 0;                       // entries per tile_start buffer
+    int64_t nchunks = 0;                        // compaction chunks of 4096 vertices
+    int tile_edges = 0;                         // T the tile_start buffers were sized for
     size_t bytes = 0;
 };
 
@@ -70,11 +73,15 @@ struct DeviceGraph {
     size_t graph_bytes = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_kernel = nullptr;
+    hipEvent_t ev_batch[2] = {nullptr, nullptr};
+    static constexpr int kCtlWords = 16;
+    unsigned long long* h_ctl = nullptr;  // pinned: 2 snapshots of the control block
     int num_cus = 0;
-    // Longest a persistent kernel's grid barrier waits before aborting the
-    // BFS with an error (a legitimate level takes milliseconds).
-    static constexpr uint64_t kBarrierTimeoutSeconds = 5;
+    // Longest a device-side wait (compaction look-back) spins before the BFS
+    // fails with an error; a legitimate wait takes microseconds.
+    static constexpr uint64_t kSpinTimeoutSeconds = 5;
     uint64_t spin_limit_ticks = 0;
+    int last_num_levels = 0;              // depth of the previous BFS: first batch size
     BfsConfig cfg;
     BfsWorkspace ws;
 
@@ -89,6 +96,7 @@ struct DeviceGraph {
     // copies levels (and parents) to the host.
     void bfs_to_host(int32_t source, int32_t* h_levels, int32_t* h_parents, gb_stats* st);
     size_t memory_bytes() const { return graph_bytes + ws.bytes; }
+    int64_t default_dense_edges() const { return std::max<int64_t>(int64_t(1) << 19, m / 256); }
 
    private:
     void ensure_workspace();

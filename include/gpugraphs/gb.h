@@ -129,8 +129,8 @@ void gb_device_free(gb_dgraph* dg);
 /* ---- single-source BFS (replaces gpuBFS::gpuBFS, gpuBFS.cu:13-116) ----- */
 
 /* Execution modes for gb_bfs_config. */
-#define GB_MODE_PERSISTENT  0   /* one cooperative launch, grid barrier per level */
-#define GB_MODE_LEVEL_LOOP  1   /* one launch per level, host reads the frontier size */
+#define GB_MODE_ASYNC       0   /* levels queued ahead in batches, no per-level host sync */
+#define GB_MODE_LEVEL_LOOP  1   /* host reads the frontier size after every level        */
 
 /* levels: n int32 (host), -1 = unreached.  parents: nullable, n int32,
  * parent[source] = source, -1 = unreached (any valid BFS tree). */
@@ -138,13 +138,16 @@ int  gb_bfs(gb_dgraph* dg, int32_t source, int32_t* levels, int32_t* parents, gb
 /* Same, with levels / parents in device memory of the graph's GPU. */
 int  gb_bfs_device(gb_dgraph* dg, int32_t source, int32_t* d_levels, int32_t* d_parents,
                    gb_stats* st);
-/* Tunables: block size (64..1024, multiple of 64), edges per thread per tile,
- * blocks per CU (0 = occupancy maximum), mode, and the frontier edge count
- * under which one workgroup runs a level alone. */
+/* Tunables: expand() workgroup size and edges per thread per tile (supported
+ * pairs: 256x{1,2,4,8}, 512x{1,2,4}, 1024x{1,2}, 128x{4,8}, 64x{8,16}),
+ * workgroups per CU (0 = at most 16 waves per CU), mode, and the frontier
+ * edge count above which a level's next frontier is rebuilt from the
+ * visited bitmap by a scan instead of queued by atomics (0 = default,
+ * max(2^19, m/256)). */
 int  gb_bfs_config(gb_dgraph* dg, int block, int edges_per_thread, int blocks_per_cu,
-                   int mode, int64_t solo_edges);
+                   int mode, int64_t dense_edges);
 int  gb_bfs_get_config(const gb_dgraph* dg, int* block, int* edges_per_thread,
-                       int* blocks_per_cu, int* mode, int64_t* solo_edges);
+                       int* blocks_per_cu, int* mode, int64_t* dense_edges);
 
 /* ---- device-wide exclusive scan (replaces PrefixSum::sum_scan_blelloch,
  *      1D Partition/src/prefixSum.cu:139-224).  d_out[i] = sum(d_in[0..i)).

@@ -53,16 +53,22 @@ def test_er_workload_golden(gpu):
     assert st["n_reached"] == 10000
 
 
-@pytest.mark.parametrize("mode", [gg.GB_MODE_PERSISTENT, gg.GB_MODE_LEVEL_LOOP])
+# dense_edges: 1 = every level rebuilds its queue by bitmap compaction,
+# 2**32-1 = every level queues by atomics, 0 = the default threshold.
+DENSE = [1, 2**32 - 1, 0]
+
+
+@pytest.mark.parametrize("dense", DENSE)
+@pytest.mark.parametrize("mode", [gg.GB_MODE_ASYNC, gg.GB_MODE_LEVEL_LOOP])
 @pytest.mark.parametrize("block,k", CONFIGS)
-def test_rmat_all_kernel_shapes(gpu, mode, block, k):
+def test_rmat_all_kernel_shapes(gpu, mode, block, k, dense):
     G = gg.Graph.rmat(14, seed=41)
     dg = gg.DeviceGraph(G, gpu)
-    dg.config(block=block, edges_per_thread=k, mode=mode)
+    dg.config(block=block, edges_per_thread=k, mode=mode, dense_edges=dense)
     off, cols = G.offsets, G.cols
     deg = np.diff(off)
     rng = np.random.default_rng(block * 10 + k + mode)
-    log(f"rmat14 block={block} k={k} mode={mode}")
+    log(f"rmat14 block={block} k={k} mode={mode} dense={dense}")
     for s in rng.choice(np.nonzero(deg)[0], 4, replace=False):
         lv, par, st = dg.bfs(int(s), parents=True)
         want, _, ost = O.bfs(off, cols, int(s))
@@ -70,6 +76,25 @@ def test_rmat_all_kernel_shapes(gpu, mode, block, k):
         assert O.check_parents(off, cols, int(s), lv, par) == 0
         assert st["n_reached"] == ost["n_reached"] and st["m_reached"] == ost["m_reached"]
         assert st["num_levels"] == ost["num_levels"]
+
+
+@pytest.mark.parametrize("dense", [1, 5000])
+@pytest.mark.parametrize("name", ["wiki-Vote", "test2", "test_isolated_node", "test4"])
+def test_fixtures_through_compaction(gpu, name, dense, golden_levels):
+    # Directed fixtures with sinks: degree-0 vertices are reached but never
+    # queued, which the compaction path counts separately.
+    g = load_fixture(name)
+    dg = g.device_graph(gpu)
+    dg.config(dense_edges=dense)
+    sources = golden_levels[f"{name}__sources"]
+    want = golden_levels[f"{name}__levels"]
+    for i, s in enumerate(sources[:16]):
+        lv, par, st = dg.bfs(int(s), parents=True)
+        np.testing.assert_array_equal(lv, want[i], err_msg=f"{name} source {s}")
+        assert O.check_parents(g.row_offset, g.col_idx, int(s), lv, par) == 0
+        assert st["n_reached"] == int((want[i] >= 0).sum())
+        assert st["num_levels"] == int(want[i].max()) + 1
+    dg.config()
 
 
 @pytest.mark.parametrize("scale,seed", [(16, 1), (18, 2)])
@@ -101,8 +126,8 @@ def test_isolated_and_degree_zero_sources(gpu):
 def test_grid_many_levels(gpu):
     G = gg.Graph.grid(300, 200, 0.05, seed=4)
     dg = gg.DeviceGraph(G, gpu)
-    for mode in (gg.GB_MODE_PERSISTENT, gg.GB_MODE_LEVEL_LOOP):
-        dg.config(mode=mode)
+    for mode, dense in ((gg.GB_MODE_ASYNC, 0), (gg.GB_MODE_LEVEL_LOOP, 0), (gg.GB_MODE_ASYNC, 1)):
+        dg.config(mode=mode, dense_edges=dense)
         for s in (0, 123 * 300 + 77):
             lv, _, st = dg.bfs(s)
             want, _, ost = O.bfs(G.offsets, G.cols, s)
@@ -117,8 +142,8 @@ def test_hub_spanning_many_tiles(gpu):
     dst = np.concatenate([np.arange(1, 100_001, dtype=np.int32), [100_001]]).astype(np.int32)
     G = gg.Graph.from_edges(n, np.concatenate([src, dst]), np.concatenate([dst, src]))
     dg = gg.DeviceGraph(G, gpu)
-    for block, k in [(256, 4), (64, 16), (1024, 2)]:
-        dg.config(block=block, edges_per_thread=k)
+    for block, k, dense in [(256, 4, 0), (64, 16, 0), (1024, 2, 0), (256, 4, 1), (64, 16, 1)]:
+        dg.config(block=block, edges_per_thread=k, dense_edges=dense)
         lv, par, _ = dg.bfs(5, parents=True)
         want, _, _ = O.bfs(G.offsets, G.cols, 5)
         np.testing.assert_array_equal(lv, want)
