@@ -104,8 +104,12 @@ struct TileLds {
     unsigned long long base;
 };
 
+// Coherent read of a look-back status word.  Relaxed on purpose: the status
+// words carry all the data exchanged inside a launch (the queue they index
+// is read by the next launch), and an agent-scope acquire would invalidate
+// the whole L2 of the XCD on every poll (buffer_inv sc1 on gfx950).
 __device__ __forceinline__ unsigned long long rmw_read(unsigned long long* p) {
-    return __hip_atomic_fetch_add(p, 0ull, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_fetch_add(p, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Writes tile_start[q][t] = slot for every tile boundary t*T in [eo, eo + d).
@@ -172,28 +176,47 @@ __device__ void expand_tile(const BfsArgs& a, TileLds<BLOCK, K>& s, int L, uint3
 #pragma unroll
     for (int k = 0; k < K; ++k) word[k] = act[k] ? a.vis[static_cast<uint32_t>(u[k]) >> 5] : ~0u;
 
-    const uint32_t lane = tid & 63u;
+    // Claims: all K atomics are issued before any result is used, so their
+    // round trips overlap instead of running back to back.
+    uint32_t old[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const uint32_t uu = static_cast<uint32_t>(u[k]);
         const uint32_t bit = 1u << (uu & 31u);
-        bool win = false;
-        if (act[k] && !(word[k] & bit)) {
-            const uint32_t old = atomicOr(&a.vis[uu >> 5], bit);
-            if (!(old & bit)) {
-                win = true;
-                a.levels[uu] = L + 1;
-                if (a.parents) a.parents[uu] = s.v[owner[k]];
-            }
+        old[k] = (act[k] && !(word[k] & bit)) ? atomicOr(&a.vis[uu >> 5], bit) : ~0u;
+    }
+    bool won[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint32_t uu = static_cast<uint32_t>(u[k]);
+        won[k] = !(old[k] & (1u << (uu & 31u)));
+        if (won[k]) {
+            a.levels[uu] = L + 1;
+            if (a.parents) a.parents[uu] = s.v[owner[k]];
         }
-        if (dense) continue;  // compact() builds the next queue from the bitmaps
-        int64_t rb = 0;
-        uint32_t deg = 0;
-        if (win) {
-            atomicOr(&a.prev[uu >> 5], bit);  // queued (or degree 0): compact() must skip it
-            rb = a.offsets[uu];
-            deg = static_cast<uint32_t>(a.offsets[uu + 1] - rb);
+    }
+    if (dense) {  // compact() builds the next queue from the bitmaps
+        __syncthreads();
+        return;
+    }
+    int64_t wrb[K];
+    int64_t wre[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint32_t uu = static_cast<uint32_t>(u[k]);
+        wrb[k] = wre[k] = 0;
+        if (won[k]) {
+            atomicOr(&a.prev[uu >> 5], 1u << (uu & 31u));  // queued (or degree 0): compact() must skip it
+            wrb[k] = a.offsets[uu];
+            wre[k] = a.offsets[uu + 1];
         }
+    }
+    const uint32_t lane = tid & 63u;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int64_t rb = wrb[k];
+        const uint32_t deg = static_cast<uint32_t>(wre[k] - wrb[k]);
+        const bool win = won[k];
         const uint64_t keep = wave_ballot(win && deg > 0);
         const uint64_t zero = wave_ballot(win && deg == 0);
         if (zero && lane == 0) atomicAdd(&s.nzero, static_cast<uint32_t>(__popcll(zero)));
@@ -208,10 +231,6 @@ __device__ void expand_tile(const BfsArgs& a, TileLds<BLOCK, K>& s, int L, uint3
                 s.win_v[slot] = u[k];
             }
         }
-    }
-    if (dense) {
-        __syncthreads();
-        return;
     }
     __syncthreads();
 
