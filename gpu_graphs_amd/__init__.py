@@ -22,7 +22,7 @@ import numpy as np
 
 from . import _ffi
 from ._ffi import (GB_AUTO_N, GB_DROP_SELF_LOOPS, GB_KEEP_ORDER, GB_MODE_ASYNC, GB_MODE_LEVEL_LOOP, GB_ONE_BASED,
-                   GB_SYMMETRIZE, GBError, Stats, check, lib)
+                   GB_SYMMETRIZE, BatchStats, GBError, Stats, check, lib)
 
 __all__ = [
     "Graph", "DeviceGraph", "DeviceArray", "csr", "gpuBFS", "GBError", "Stats", "device_count",
@@ -233,6 +233,30 @@ class DeviceGraph:
         check(lib().gb_bfs_device(self._h, int(source), d_levels.ptr, d_parents.ptr if d_parents else None,
                                   C.byref(st)))
         return st.as_dict()
+
+    def batch_config(self, block=256, edges_per_thread=2, use_graph=True):
+        check(lib().gb_batch_config(self._h, block, edges_per_thread, int(use_graph)))
+
+    def get_batch_config(self):
+        b, k, g = C.c_int(), C.c_int(), C.c_int()
+        check(lib().gb_batch_get_config(self._h, C.byref(b), C.byref(k), C.byref(g)))
+        return {"block": b.value, "edges_per_thread": k.value, "use_graph": bool(g.value)}
+
+    def bfs_batch(self, sources, stats=True):
+        """Multi-source BFS of up to 64 sources: returns (levels[k, n], stats or None)."""
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        levels = np.empty((src.size, self.n), dtype=np.int32)
+        st = BatchStats() if stats else None
+        check(lib().gb_bfs_batch(self._h, _ptr(src), src.size, _ptr(levels), C.byref(st) if stats else None))
+        return levels, (st.as_dict() if stats else None)
+
+    def bfs_batch_device(self, sources, d_levels, stats=False):
+        """Multi-source BFS into a device buffer of k*n int32 (results stay in HBM)."""
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        st = BatchStats()
+        check(lib().gb_bfs_batch_device(self._h, _ptr(src), src.size, d_levels.ptr,
+                                        C.byref(st) if stats else None))
+        return st.as_dict() if stats else None
 
     def free(self):
         if self._h is not None and self._h.value:

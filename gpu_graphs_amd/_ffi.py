@@ -55,6 +55,21 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("ms", C.c_double),
+        ("num_levels", C.c_int32),
+        ("k", C.c_int32),
+        ("n_reached", C.c_int64 * 64),
+        ("m_reached", C.c_int64 * 64),
+    ]
+
+    def as_dict(self):
+        k = self.k
+        return {"ms": self.ms, "num_levels": self.num_levels, "k": k,
+                "n_reached": list(self.n_reached)[:k], "m_reached": list(self.m_reached)[:k]}
+
+
 P = C.c_void_p
 PP = C.POINTER(C.c_void_p)
 i32p = C.POINTER(C.c_int32)
@@ -87,6 +102,10 @@ SIGNATURES = {
     "gb_bfs_config": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64]),
     "gb_bfs_get_config": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                     C.POINTER(C.c_int), i64p]),
+    "gb_bfs_batch": (C.c_int, [P, P, C.c_int, P, C.POINTER(BatchStats)]),
+    "gb_bfs_batch_device": (C.c_int, [P, P, C.c_int, P, C.POINTER(BatchStats)]),
+    "gb_batch_config": (C.c_int, [P, C.c_int, C.c_int, C.c_int]),
+    "gb_batch_get_config": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "gb_exclusive_scan_i32": (C.c_int, [C.c_int, P, P, C.c_int64]),
     "gb_dmalloc": (C.c_int, [C.c_int, C.c_size_t, PP]),
     "gb_dfree": (C.c_int, [C.c_int, P]),

@@ -42,6 +42,7 @@
 #include <cstring>
 
 #include "device_graph.hpp"
+#include "frontier.hpp"
 #include "wave.hpp"
 
 namespace gg {
@@ -63,9 +64,6 @@ enum : unsigned long long { ERR_TILE = 1, ERR_LOOKBACK = 2 };
 constexpr int kCompactBlock = 256;
 constexpr int kCompactRounds = 16;                          // vertices per thread
 constexpr int kChunk = kCompactBlock * kCompactRounds;      // vertices per compaction chunk
-constexpr unsigned long long kFlagAgg = 1ull << 62;         // look-back: chunk aggregate published
-constexpr unsigned long long kFlagPrefix = 2ull << 62;      // look-back: inclusive prefix published
-constexpr unsigned long long kValueMask = (1ull << 62) - 1;
 
 struct BfsArgs {
     const int64_t* __restrict__ offsets;
@@ -103,20 +101,6 @@ struct TileLds {
     uint32_t nzero;
     unsigned long long base;
 };
-
-// Coherent read of a look-back status word.  Relaxed on purpose: the status
-// words carry all the data exchanged inside a launch (the queue they index
-// is read by the next launch), and an agent-scope acquire would invalidate
-// the whole L2 of the XCD on every poll (buffer_inv sc1 on gfx950).
-__device__ __forceinline__ unsigned long long rmw_read(unsigned long long* p) {
-    return __hip_atomic_fetch_add(p, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Writes tile_start[q][t] = slot for every tile boundary t*T in [eo, eo + d).
-__device__ __forceinline__ void mark_tiles(uint32_t* ts, uint32_t T, uint32_t eo, uint32_t d, uint32_t slot) {
-    const uint64_t stop = uint64_t(eo) + d;
-    for (uint64_t tt = (uint64_t(eo) + T - 1) / T; tt * T < stop; ++tt) ts[tt] = slot;
-}
 
 // Expands tile t of level L: frontier edges [t*T, min((t+1)*T, E)).
 template <int BLOCK, int K>
@@ -162,12 +146,7 @@ __device__ void expand_tile(const BfsArgs& a, TileLds<BLOCK, K>& s, int L, uint3
         u[k] = 0;
         owner[k] = 0;
         if (act[k]) {
-            uint32_t lo = 0, hi = c - 1;  // last entry with eoff <= e
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi + 1) >> 1;
-                if (s.eoff[mid] <= e) lo = mid;
-                else hi = mid - 1;
-            }
+            const uint32_t lo = owner_of(s.eoff, c, e);
             owner[k] = lo;
             u[k] = a.cols[s.rb[lo] + (e - s.eoff[lo])];
         }
@@ -298,42 +277,6 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(BfsArgs a, int L) {
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) expand_tile<BLOCK, K>(a, s, L, t, ntiles, F, E, dense);
 }
 
-// Decoupled look-back for chunk c (thread 0 only): publishes the chunk's
-// aggregate, accumulates predecessors until one with an inclusive prefix,
-// publishes its own inclusive prefix and returns the exclusive one.  Waits are
-// bounded; on timeout the BFS is flagged as failed.
-__device__ unsigned long long lookback(const BfsArgs& a, unsigned long long* st, uint32_t c,
-                                       unsigned long long agg) {
-    if (c == 0) {
-        atomicExch(&st[0], kFlagPrefix | agg);
-        return 0;
-    }
-    atomicExch(&st[c], kFlagAgg | agg);
-    unsigned long long excl = 0;
-    int64_t j = static_cast<int64_t>(c) - 1;
-    const uint64_t t0 = wall_clock64();
-    while (j >= 0) {
-        const unsigned long long sj = rmw_read(&st[j]);
-        const unsigned long long flag = sj & ~kValueMask;
-        if (flag == kFlagPrefix) {
-            excl += sj & kValueMask;
-            break;
-        }
-        if (flag == kFlagAgg) {
-            excl += sj & kValueMask;
-            --j;
-            continue;
-        }
-        if (static_cast<uint64_t>(wall_clock64()) - t0 > a.spin_limit) {
-            atomicOr(&a.ctl[CTL_ERR], ERR_LOOKBACK);
-            break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-    }
-    atomicExch(&st[c], kFlagPrefix | (excl + agg));
-    return excl;
-}
-
 // Level L, step 2 (dense levels only): queue of level L+1 from the bitmaps.
 // Chunk c covers vertices [c*kChunk, (c+1)*kChunk); thread t takes vertices
 // c*kChunk + r*256 + t for r < 16, so the offsets loads of a wave are
@@ -378,7 +321,7 @@ __device__ void compact_chunk(const BfsArgs& a, int L, uint32_t c) {
     }
     if (tid == 0) {
         const uint32_t p = L & 1;
-        s_excl = lookback(a, a.lb[p], c, total);
+        s_excl = lookback(a.lb[p], c, total, a.spin_limit, &a.ctl[CTL_ERR], ERR_LOOKBACK);
         uint32_t z = 0;
         for (int i = 0; i < kCompactBlock / 64; ++i) z += s_zero[i];
         if (z) {
@@ -526,6 +469,7 @@ DeviceGraph::DeviceGraph(const HostGraph& g, int dev) : device(dev), n(g.n), m(g
 DeviceGraph::~DeviceGraph() {
     (void)hipSetDevice(device);
     free_workspace();
+    free_ms_workspace();
     if (offsets) (void)hipFree(offsets);
     if (cols) (void)hipFree(cols);
     offsets = nullptr;

@@ -26,8 +26,6 @@ thread_local std::string t_last_error;
 void set_last_error(const std::string& msg) { t_last_error = msg; }
 const char* last_error() { return t_last_error.c_str(); }
 
-bool bfs_config_supported(int block, int k);
-
 template <class F>
 int guarded(F&& fn) {
     try {
@@ -286,6 +284,40 @@ int gb_bfs_get_config(const gb_dgraph* dg, int* block, int* edges_per_thread, in
         if (blocks_per_cu) *blocks_per_cu = c.blocks_per_cu;
         if (mode) *mode = c.mode;
         if (dense_edges) *dense_edges = c.dense_edges;
+    });
+}
+
+int gb_bfs_batch(gb_dgraph* dg, const int32_t* sources, int k, int32_t* levels, gb_batch_stats* st) {
+    return guarded([&] {
+        need(dg && sources && levels, "null argument");
+        dg->d->bfs_batch_to_host(sources, k, levels, st);
+    });
+}
+
+int gb_bfs_batch_device(gb_dgraph* dg, const int32_t* sources, int k, int32_t* d_levels, gb_batch_stats* st) {
+    return guarded([&] {
+        need(dg && sources && d_levels, "null argument");
+        gg::check_device_ptr(dg->d->device, d_levels, "levels");
+        dg->d->bfs_batch(sources, k, d_levels, st);
+    });
+}
+
+int gb_batch_config(gb_dgraph* dg, int block, int edges_per_thread, int use_graph) {
+    return guarded([&] {
+        need(dg, "null device graph");
+        need(gg::ms_config_supported(block, edges_per_thread), "unsupported block / edges_per_thread combination");
+        dg->d->ms_cfg.block = block;
+        dg->d->ms_cfg.edges_per_thread = edges_per_thread;
+        dg->d->ms_cfg.use_graph = use_graph != 0;
+    });
+}
+
+int gb_batch_get_config(const gb_dgraph* dg, int* block, int* edges_per_thread, int* use_graph) {
+    return guarded([&] {
+        need(dg, "null device graph");
+        if (block) *block = dg->d->ms_cfg.block;
+        if (edges_per_thread) *edges_per_thread = dg->d->ms_cfg.edges_per_thread;
+        if (use_graph) *use_graph = dg->d->ms_cfg.use_graph ? 1 : 0;
     });
 }
 

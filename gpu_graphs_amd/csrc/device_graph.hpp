@@ -65,6 +65,40 @@ struct BfsWorkspace {
     size_t bytes = 0;
 };
 
+// Tunables of the batched multi-source kernels.
+struct MsConfig {
+    int block = 256;
+    int edges_per_thread = 2;
+    bool use_graph = true;  // replay one captured hipGraph per batch
+};
+
+// Device workspace of the batched multi-source BFS.
+struct MsWorkspace {
+    unsigned long long* seen = nullptr;   // n: sources that reached each vertex
+    unsigned long long* next = nullptr;   // n: sources reaching it in the current level
+    int64_t* q_rb = nullptr;              // queue of active vertices
+    uint32_t* q_eoff = nullptr;
+    int32_t* q_v = nullptr;
+    unsigned long long* q_fm = nullptr;   // their frontier masks
+    uint32_t* tile_start = nullptr;
+    unsigned long long* ctl = nullptr;
+    unsigned long long* lb[2] = {nullptr, nullptr};
+    int32_t* sources = nullptr;           // 64, device
+    int32_t* h_sources = nullptr;         // 64, pinned (read by the captured copy)
+    unsigned long long* h_ctl = nullptr;  // pinned snapshot of ctl
+    unsigned long long* partial = nullptr;  // per-workgroup statistics
+    int32_t* levels = nullptr;            // staging for host-pointer calls
+    size_t levels_bytes = 0;
+    int64_t tile_cap = 0;
+    int64_t nchunks = 0;
+    int tile_edges = 0;
+    int last_depth = 0;
+    size_t bytes = 0;
+    hipGraphExec_t graph_exec = nullptr;  // captured batch sequence
+    int graph_k = 0, graph_depth = 0, graph_block = 0, graph_kpt = 0;
+    int32_t* graph_levels = nullptr;
+};
+
 struct DeviceGraph {
     int device = 0;
     int64_t n = 0, m = 0;
@@ -84,6 +118,8 @@ struct DeviceGraph {
     int last_num_levels = 0;              // depth of the previous BFS: first batch size
     BfsConfig cfg;
     BfsWorkspace ws;
+    MsConfig ms_cfg;
+    MsWorkspace ms;
 
     DeviceGraph(const HostGraph& g, int device);
     ~DeviceGraph();
@@ -95,13 +131,25 @@ struct DeviceGraph {
     // Host-pointer variant: runs into the handle's staging buffers, then
     // copies levels (and parents) to the host.
     void bfs_to_host(int32_t source, int32_t* h_levels, int32_t* h_parents, gb_stats* st);
-    size_t memory_bytes() const { return graph_bytes + ws.bytes; }
+    // Batched multi-source BFS of k <= 64 sources (host array); levels is a
+    // device buffer of k rows of n int32.
+    void bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_batch_stats* st);
+    void bfs_batch_to_host(const int32_t* h_sources, int k, int32_t* h_levels, gb_batch_stats* st);
+    // HBM a batch needs beyond the graph (for admission control).
+    static size_t batch_bytes_estimate(int64_t n, int64_t m, int k, bool host_levels);
+
+    size_t memory_bytes() const { return graph_bytes + ws.bytes + ms.bytes + ms.levels_bytes; }
     int64_t default_dense_edges() const { return std::max<int64_t>(int64_t(1) << 19, m / 256); }
 
    private:
     void ensure_workspace();
     void free_workspace();
+    void ensure_ms_workspace();
+    void free_ms_workspace();
 };
+
+bool bfs_config_supported(int block, int k);
+bool ms_config_supported(int block, int k);
 
 // Device-wide exclusive scan of int32 (scan.hip).
 void exclusive_scan_i32(int device, const int32_t* d_in, int32_t* d_out, int64_t n);

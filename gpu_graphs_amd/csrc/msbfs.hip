@@ -1,0 +1,537 @@
+// Batched multi-source BFS: up to 64 sources per launch sequence, one bit
+// per source in 64-bit masks (MS-BFS, Then et al., VLDB 2015).  There is no
+// counterpart in the reference, whose engine runs one source per process
+// (1D Partition/src/gpuBFS.cu:13-116); each source's levels must equal what
+// the single-source path (and cpuBFS, 1D Partition/src/cpuBFS.cpp:5-62)
+// gives for that source.
+//
+// Per vertex: seen (sources that reached it), next (sources reaching it in
+// the level being expanded).  A level L is two launches:
+//   update_compact(L): for every vertex with next != 0: new = next & ~seen,
+//       seen |= new, level[s][v] = L for every bit s of new, and (degree > 0)
+//       an entry (vertex, first CSR entry, edge prefix, mask = new) in the
+//       queue of active vertices; the edge prefix comes from the same
+//       single-pass look-back scan as the single-source compaction.
+//   expand(L): edge-parallel tiles over the queue; for edge (u, v) the bits
+//       mask(u) & ~seen(v) are OR-ed into next(v) with a 64-bit atomicOr whose
+//       result is not needed, so the atomics do not stall the wave.
+// Level writes: lanes of a wave hold 64 consecutive vertices; the wave loops
+// over the union of their new bits and stores level[s][v] with the lanes that
+// have bit s, so each store instruction covers one 256-byte run of row s.
+// The whole sequence (source upload, clears, levels) can be captured once
+// into a hipGraph and replayed per batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "device_graph.hpp"
+#include "frontier.hpp"
+#include "wave.hpp"
+
+namespace gg {
+
+namespace {
+
+enum : int { MS_CNT = 0, MS_DONE = 1, MS_ERR = 2, MS_MAXLEV = 3, MS_WORDS = 16 };
+enum : unsigned long long { MS_ERR_TILE = 1, MS_ERR_LOOKBACK = 2 };
+
+constexpr int kUcBlock = 256;
+constexpr int kUcRounds = 16;
+constexpr int kUcChunk = kUcBlock * kUcRounds;
+
+struct MsArgs {
+    const int64_t* __restrict__ offsets;
+    const int32_t* __restrict__ cols;
+    unsigned long long* seen;
+    unsigned long long* next;
+    int32_t* levels;             // [k][n]
+    const int32_t* sources;      // k, device
+    int64_t* q_rb;
+    uint32_t* q_eoff;
+    int32_t* q_v;
+    unsigned long long* q_fm;    // frontier mask of each queue entry
+    uint32_t* tile_start;
+    unsigned long long* ctl;
+    unsigned long long* lb[2];
+    int64_t n;
+    int k;
+    uint32_t nchunks;
+    uint32_t tile_edges;
+    uint64_t spin_limit;
+};
+
+template <int BLOCK, int K>
+struct MsTileLds {
+    static constexpr int T = BLOCK * K;
+    uint32_t eoff[T + 2];
+    int64_t rb[T + 1];
+    unsigned long long fm[T + 1];
+};
+
+// seen = next = 0; levels = -1 (k rows of n); look-back status = 0.
+__global__ void ms_clear_kernel(MsArgs a) {
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    const int64_t tid0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (int64_t i = tid0; i < a.n; i += stride) {
+        a.seen[i] = 0;
+        a.next[i] = 0;
+        if (i < a.nchunks) {
+            a.lb[0][i] = 0;
+            a.lb[1][i] = 0;
+        }
+    }
+    const int64_t total = a.n * a.k;
+    int4* lv4 = reinterpret_cast<int4*>(a.levels);
+    // 16-byte stores only on a 16-byte aligned buffer (caller-owned memory).
+    const int64_t n4 = (reinterpret_cast<uintptr_t>(a.levels) & 15) ? 0 : total / 4;
+    const int4 minus1 = make_int4(-1, -1, -1, -1);
+    for (int64_t i = tid0; i < n4; i += stride) lv4[i] = minus1;
+    for (int64_t i = n4 * 4 + tid0; i < total; i += stride) a.levels[i] = -1;
+}
+
+__global__ void ms_seed_kernel(MsArgs a) {
+    if (threadIdx.x == 0)
+        for (int i = 0; i < MS_WORDS; ++i) a.ctl[i] = 0;
+    if (threadIdx.x < static_cast<uint32_t>(a.k))
+        atomicOr(&a.next[a.sources[threadIdx.x]], 1ull << threadIdx.x);
+}
+
+// Stores level L for the new bits of this lane's vertex: the wave walks the
+// union of its lanes' bits, one predicated store per source present.
+__device__ __forceinline__ void write_levels(const MsArgs& a, int64_t v, unsigned long long nv, int L) {
+    unsigned long long any = nv;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) any |= __shfl_xor(any, o, 64);
+    while (any) {
+        const int s = __ffsll(static_cast<long long>(any)) - 1;
+        any &= any - 1;
+        if ((nv >> s) & 1ull) a.levels[int64_t(s) * a.n + v] = L;
+    }
+}
+
+__device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c) {
+    __shared__ unsigned long long s_wave[kUcBlock / 64];
+    __shared__ unsigned long long s_excl;
+    const uint32_t tid = threadIdx.x;
+    const int64_t vbase = int64_t(c) * kUcChunk;
+    uint32_t active = 0;  // bit r: vertex vbase + r*256 + tid goes to the queue
+    uint32_t kept = 0, edges = 0;
+    unsigned long long newbits[kUcRounds];
+#pragma unroll
+    for (int r = 0; r < kUcRounds; ++r) {
+        const int64_t v = vbase + r * kUcBlock + tid;
+        unsigned long long nv = 0;
+        if (v < a.n) {
+            const unsigned long long nx = a.next[v];
+            if (nx) {
+                const unsigned long long sv = a.seen[v];
+                nv = nx & ~sv;
+                a.seen[v] = sv | nv;
+                a.next[v] = 0;
+            }
+        }
+        newbits[r] = nv;
+        // every lane of the wave takes part in the shuffles of write_levels
+        if (__ballot(nv != 0)) write_levels(a, v, nv, L);
+        if (nv) {
+            const uint32_t d = static_cast<uint32_t>(a.offsets[v + 1] - a.offsets[v]);
+            if (d) {
+                active |= 1u << r;
+                ++kept;
+                edges += d;
+            }
+        }
+    }
+    bool anynew = false;
+#pragma unroll
+    for (int r = 0; r < kUcRounds; ++r) anynew |= newbits[r] != 0;
+    anynew = __syncthreads_or(anynew);
+    const unsigned long long mine = (static_cast<unsigned long long>(kept) << 32) | edges;
+    unsigned long long total;
+    const unsigned long long excl_local = block_exclusive_scan<kUcBlock>(mine, s_wave, total);
+    if (tid == 0) {
+        s_excl = lookback(a.lb[L & 1], c, total, a.spin_limit, &a.ctl[MS_ERR], MS_ERR_LOOKBACK);
+        if (c == a.nchunks - 1) a.ctl[MS_CNT] = s_excl + total;
+        // Every writer of this level stores the same L and levels run in
+        // order, so the final value is the deepest level reached.
+        if (anynew) a.ctl[MS_MAXLEV] = static_cast<unsigned long long>(L);
+    }
+    __syncthreads();
+    if (!active) return;
+    const unsigned long long pre = s_excl + excl_local;
+    uint32_t slot = static_cast<uint32_t>(pre >> 32);
+    uint32_t eo = static_cast<uint32_t>(pre);
+#pragma unroll
+    for (int r = 0; r < kUcRounds; ++r) {
+        if ((active >> r) & 1u) {
+            const int64_t v = vbase + r * kUcBlock + tid;
+            const int64_t rb = a.offsets[v];
+            const uint32_t d = static_cast<uint32_t>(a.offsets[v + 1] - rb);
+            a.q_rb[slot] = rb;
+            a.q_eoff[slot] = eo;
+            a.q_v[slot] = static_cast<int32_t>(v);
+            a.q_fm[slot] = newbits[r];
+            mark_tiles(a.tile_start, a.tile_edges, eo, d, slot);
+            ++slot;
+            eo += d;
+        }
+    }
+}
+
+// Level L, step 1: assign level L to the bits found by expand(L-1) and queue
+// the vertices that carry them.
+__global__ __launch_bounds__(kUcBlock) void ms_update_compact_kernel(MsArgs a, int L) {
+    if (a.ctl[MS_DONE]) return;
+    unsigned long long* other = a.lb[(L + 1) & 1];
+    for (uint32_t i = blockIdx.x * kUcBlock + threadIdx.x; i < a.nchunks; i += gridDim.x * kUcBlock) other[i] = 0;
+    for (uint32_t c = blockIdx.x; c < a.nchunks; c += gridDim.x) {
+        ms_uc_chunk(a, L, c);
+        __syncthreads();
+    }
+}
+
+// Level L, step 2: push the queue's masks along every edge.
+template <int BLOCK, int K>
+__global__ __launch_bounds__(BLOCK) void ms_expand_kernel(MsArgs a, int L) {
+    __shared__ MsTileLds<BLOCK, K> s;
+    constexpr uint32_t T = BLOCK * K;
+    if (a.ctl[MS_DONE]) return;
+    const unsigned long long cnt = a.ctl[MS_CNT];
+    const uint32_t F = static_cast<uint32_t>(cnt >> 32), E = static_cast<uint32_t>(cnt);
+    if (F == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl[MS_DONE] = 1;
+        return;
+    }
+    const uint32_t ntiles = static_cast<uint32_t>((uint64_t(E) + T - 1) / T);
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t tid = threadIdx.x;
+        const uint32_t base = t * T;
+        const uint64_t tile_end = uint64_t(base) + T;
+        const uint32_t end = tile_end < E ? static_cast<uint32_t>(tile_end) : E;
+        const uint32_t i0 = a.tile_start[t];
+        const uint32_t i1 = (t + 1 < ntiles) ? a.tile_start[t + 1] : F - 1;
+        uint32_t c = i1 - i0 + 1;
+        if (i1 < i0 || c > T + 1 || i1 >= F) {
+            if (tid == 0) atomicOr(&a.ctl[MS_ERR], MS_ERR_TILE);
+            c = 0;
+        }
+        for (uint32_t j = tid; j <= c; j += BLOCK) {
+            if (j < c) {
+                s.eoff[j] = a.q_eoff[i0 + j];
+                s.rb[j] = a.q_rb[i0 + j];
+                s.fm[j] = a.q_fm[i0 + j];
+            } else {
+                s.eoff[j] = (i0 + c < F) ? a.q_eoff[i0 + c] : E;
+            }
+        }
+        __syncthreads();
+        if (c) {
+            int32_t v[K];
+            unsigned long long fm[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const uint32_t e = base + k * BLOCK + tid;
+                v[k] = -1;
+                fm[k] = 0;
+                if (e < end) {
+                    const uint32_t o = owner_of(s.eoff, c, e);
+                    fm[k] = s.fm[o];
+                    v[k] = a.cols[s.rb[o] + (e - s.eoff[o])];
+                }
+            }
+            unsigned long long sv[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) sv[k] = v[k] >= 0 ? a.seen[v[k]] : ~0ull;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const unsigned long long m = fm[k] & ~sv[k];
+                if (m) __hip_atomic_fetch_or(&a.next[v[k]], m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+using MsExpandFn = void (*)(MsArgs, int);
+struct MsKernelSet {
+    int block, k;
+    MsExpandFn expand;
+};
+#define GG_MSKSET(B, K) {B, K, ms_expand_kernel<B, K>}
+const MsKernelSet kMsKernels[] = {GG_MSKSET(256, 1), GG_MSKSET(256, 2), GG_MSKSET(256, 4), GG_MSKSET(256, 8),
+                                  GG_MSKSET(512, 1), GG_MSKSET(512, 2), GG_MSKSET(512, 4), GG_MSKSET(1024, 1),
+                                  GG_MSKSET(1024, 2), GG_MSKSET(128, 4), GG_MSKSET(128, 8), GG_MSKSET(64, 8),
+                                  GG_MSKSET(64, 16)};
+#undef GG_MSKSET
+
+const MsKernelSet* find_ms_kernels(int block, int k) {
+    for (const auto& ks : kMsKernels)
+        if (ks.block == block && ks.k == k) return &ks;
+    return nullptr;
+}
+
+// Per-source statistics from the final seen masks: reached vertices and the
+// sum of their degrees, reduced per wave then per workgroup into
+// partial[blockIdx][k][2]; the host adds the rows.
+__global__ __launch_bounds__(256) void ms_stats_kernel(MsArgs a, unsigned long long* partial) {
+    __shared__ unsigned long long s_acc[64][2];
+    for (int i = threadIdx.x; i < 64; i += blockDim.x) s_acc[i][0] = s_acc[i][1] = 0;
+    __syncthreads();
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t base = int64_t(blockIdx.x) * blockDim.x; base < a.n; base += stride) {
+        const int64_t v = base + threadIdx.x;
+        unsigned long long sv = 0;
+        unsigned long long d = 0;
+        if (v < a.n) {
+            sv = a.seen[v];
+            d = static_cast<unsigned long long>(a.offsets[v + 1] - a.offsets[v]);
+        }
+        unsigned long long any = sv;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) any |= __shfl_xor(any, o, 64);
+        while (any) {
+            const int s = __ffsll(static_cast<long long>(any)) - 1;
+            any &= any - 1;
+            const bool has = (sv >> s) & 1ull;
+            const unsigned long long cntw = __popcll(__ballot(has));
+            const unsigned long long degw = wave_sum(has ? d : 0ull);
+            if ((threadIdx.x & 63) == 0) {
+                atomicAdd(&s_acc[s][0], cntw);
+                atomicAdd(&s_acc[s][1], degw);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < a.k; i += blockDim.x) {
+        partial[(int64_t(blockIdx.x) * 64 + i) * 2 + 0] = s_acc[i][0];
+        partial[(int64_t(blockIdx.x) * 64 + i) * 2 + 1] = s_acc[i][1];
+    }
+}
+
+constexpr int kStatsBlocks = 1024;
+
+}  // namespace
+
+bool ms_config_supported(int block, int k) { return find_ms_kernels(block, k) != nullptr; }
+
+void DeviceGraph::free_ms_workspace() {
+    if (ms.graph_exec) (void)hipGraphExecDestroy(ms.graph_exec);
+    void* ptrs[] = {ms.seen, ms.next, ms.q_rb, ms.q_eoff, ms.q_v, ms.q_fm, ms.tile_start, ms.ctl, ms.lb[0],
+                    ms.lb[1], ms.sources, ms.levels, ms.partial};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (ms.h_sources) (void)hipHostFree(ms.h_sources);
+    if (ms.h_ctl) (void)hipHostFree(ms.h_ctl);
+    ms = MsWorkspace{};
+}
+
+void DeviceGraph::ensure_ms_workspace() {
+    const int T = ms_cfg.block * ms_cfg.edges_per_thread;
+    const int64_t tile_cap = (m + T - 1) / T + 2;
+    if (ms.ctl && ms.tile_edges == T) return;
+    if (ms.ctl) {
+        (void)hipFree(ms.tile_start);
+        ms.bytes -= sizeof(uint32_t) * ms.tile_cap;
+        ms.tile_start = nullptr;
+        GG_HIP(hipMalloc(&ms.tile_start, sizeof(uint32_t) * tile_cap));
+        ms.bytes += sizeof(uint32_t) * tile_cap;
+        ms.tile_cap = tile_cap;
+        ms.tile_edges = T;
+        if (ms.graph_exec) (void)hipGraphExecDestroy(ms.graph_exec);
+        ms.graph_exec = nullptr;
+        return;
+    }
+    const int64_t nchunks = (n + kUcChunk - 1) / kUcChunk;
+    size_t bytes = 0;
+    auto alloc = [&](auto** p, size_t sz) {
+        GG_HIP(hipMalloc(reinterpret_cast<void**>(p), sz));
+        bytes += sz;
+    };
+    try {
+        alloc(&ms.seen, sizeof(unsigned long long) * n);
+        alloc(&ms.next, sizeof(unsigned long long) * n);
+        alloc(&ms.q_rb, sizeof(int64_t) * n);
+        alloc(&ms.q_eoff, sizeof(uint32_t) * n);
+        alloc(&ms.q_v, sizeof(int32_t) * n);
+        alloc(&ms.q_fm, sizeof(unsigned long long) * n);
+        alloc(&ms.tile_start, sizeof(uint32_t) * tile_cap);
+        alloc(&ms.ctl, sizeof(unsigned long long) * kCtlWords);
+        alloc(&ms.lb[0], sizeof(unsigned long long) * nchunks);
+        alloc(&ms.lb[1], sizeof(unsigned long long) * nchunks);
+        alloc(&ms.sources, sizeof(int32_t) * 64);
+        alloc(&ms.partial, sizeof(unsigned long long) * kStatsBlocks * 64 * 2);
+        GG_HIP(hipHostMalloc(reinterpret_cast<void**>(&ms.h_sources), sizeof(int32_t) * 64, hipHostMallocDefault));
+        GG_HIP(hipHostMalloc(reinterpret_cast<void**>(&ms.h_ctl), sizeof(unsigned long long) * kCtlWords,
+                             hipHostMallocDefault));
+    } catch (...) {
+        free_ms_workspace();
+        throw;
+    }
+    ms.tile_cap = tile_cap;
+    ms.tile_edges = T;
+    ms.nchunks = nchunks;
+    ms.bytes = bytes;
+}
+
+size_t DeviceGraph::batch_bytes_estimate(int64_t n_vertices, int64_t m_entries, int k, bool host_levels) {
+    const int64_t nchunks = (n_vertices + kUcChunk - 1) / kUcChunk;
+    size_t b = sizeof(unsigned long long) * n_vertices * 3 + (sizeof(int64_t) + 8) * n_vertices +
+               sizeof(uint32_t) * (m_entries / 256 + 2) + 16 * nchunks + 1024 * 64 * 16;
+    if (host_levels) b += sizeof(int32_t) * n_vertices * k;
+    return b;
+}
+
+void DeviceGraph::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_batch_stats* st) {
+    if (k < 1 || k > 64) fail(GB_EINVAL, "a batch holds 1 to 64 sources");
+    if (!h_sources || !d_levels) fail(GB_EINVAL, "null sources or levels");
+    for (int i = 0; i < k; ++i)
+        if (h_sources[i] < 0 || h_sources[i] >= n)
+            fail(GB_ERANGE, "source " + std::to_string(h_sources[i]) + " outside [0, " + std::to_string(n) + ")");
+    const MsKernelSet* ks = find_ms_kernels(ms_cfg.block, ms_cfg.edges_per_thread);
+    if (!ks) fail(GB_EINVAL, "unsupported batch block / edges_per_thread combination");
+    DeviceGuard guard(device);
+    ensure_ms_workspace();
+
+    MsArgs a;
+    a.offsets = offsets;
+    a.cols = cols;
+    a.seen = ms.seen;
+    a.next = ms.next;
+    a.levels = d_levels;
+    a.sources = ms.sources;
+    a.q_rb = ms.q_rb;
+    a.q_eoff = ms.q_eoff;
+    a.q_v = ms.q_v;
+    a.q_fm = ms.q_fm;
+    a.tile_start = ms.tile_start;
+    a.ctl = ms.ctl;
+    a.lb[0] = ms.lb[0];
+    a.lb[1] = ms.lb[1];
+    a.n = n;
+    a.k = k;
+    a.nchunks = static_cast<uint32_t>(ms.nchunks);
+    a.tile_edges = static_cast<uint32_t>(ms_cfg.block * ms_cfg.edges_per_thread);
+    a.spin_limit = spin_limit_ticks;
+
+    int occ = 0;
+    GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(ks->expand),
+                                                       ms_cfg.block, 0));
+    if (occ < 1) fail(GB_EDEVICE, "batch kernel cannot be resident on a CU");
+    const int waves_per_block = (ms_cfg.block + 63) / 64;
+    const int per_cu = std::max(1, std::min(occ, 16 / waves_per_block));
+    const uint32_t grid = static_cast<uint32_t>(per_cu * num_cus);
+    const uint32_t ugrid = static_cast<uint32_t>(std::min<int64_t>(ms.nchunks, int64_t(num_cus) * 4));
+    const uint32_t cgrid = static_cast<uint32_t>(std::min<int64_t>((n * k / 4 + 255) / 256 + 1, int64_t(num_cus) * 8));
+
+    std::memcpy(ms.h_sources, h_sources, sizeof(int32_t) * k);
+    auto launch_prologue = [&]() {
+        GG_HIP(hipMemcpyAsync(ms.sources, ms.h_sources, sizeof(int32_t) * 64, hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(ms_clear_kernel, dim3(cgrid), dim3(256), 0, stream, a);
+        hipLaunchKernelGGL(ms_seed_kernel, dim3(1), dim3(64), 0, stream, a);
+    };
+    int L = 0;
+    auto launch_level = [&]() {
+        hipLaunchKernelGGL(ms_update_compact_kernel, dim3(ugrid), dim3(kUcBlock), 0, stream, a, L);
+        hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(ms_cfg.block), 0, stream, a, L);
+        ++L;
+    };
+    auto check_done = [&]() {
+        GG_HIP(hipMemcpyAsync(ms.h_ctl, ms.ctl, sizeof(unsigned long long) * kCtlWords, hipMemcpyDeviceToHost,
+                              stream));
+        GG_HIP(hipStreamSynchronize(stream));
+        return ms.h_ctl[MS_DONE] != 0;
+    };
+
+    const int depth_hint = std::max(ms.last_depth + 1, 4);
+    GG_HIP(hipEventRecord(ev_start, stream));
+    if (ms_cfg.use_graph) {
+        // One captured sequence: sources upload, clears, depth_hint levels.
+        // Captured arguments are fixed, so the graph is rebuilt when k, the
+        // output buffer, the kernel shape or the depth changes.
+        const bool reuse = ms.graph_exec && ms.graph_k == k && ms.graph_levels == d_levels &&
+                           ms.graph_depth == depth_hint && ms.graph_block == ms_cfg.block &&
+                           ms.graph_kpt == ms_cfg.edges_per_thread;
+        if (!reuse) {
+            if (ms.graph_exec) (void)hipGraphExecDestroy(ms.graph_exec);
+            ms.graph_exec = nullptr;
+            hipGraph_t g = nullptr;
+            GG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            try {
+                launch_prologue();
+                for (int i = 0; i < depth_hint; ++i) launch_level();
+            } catch (...) {
+                hipGraph_t dummy = nullptr;
+                (void)hipStreamEndCapture(stream, &dummy);
+                if (dummy) (void)hipGraphDestroy(dummy);
+                throw;
+            }
+            GG_HIP(hipStreamEndCapture(stream, &g));
+            hipError_t e = hipGraphInstantiate(&ms.graph_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            GG_HIP(e);
+            ms.graph_k = k;
+            ms.graph_levels = d_levels;
+            ms.graph_depth = depth_hint;
+            ms.graph_block = ms_cfg.block;
+            ms.graph_kpt = ms_cfg.edges_per_thread;
+            GG_HIP(hipEventRecord(ev_start, stream));
+        }
+        GG_HIP(hipGraphLaunch(ms.graph_exec, stream));
+        L = depth_hint;
+    } else {
+        launch_prologue();
+        for (int i = 0; i < depth_hint; ++i) launch_level();
+    }
+    GG_HIP(hipGetLastError());
+    // Deeper than the hint: continue level by level in batches of 8.
+    while (!check_done()) {
+        if (L > n + 2) fail(GB_EDEVICE, "internal error: more BFS levels than vertices");
+        for (int i = 0; i < 8; ++i) launch_level();
+        GG_HIP(hipGetLastError());
+    }
+    GG_HIP(hipEventRecord(ev_stop, stream));
+    GG_HIP(hipEventSynchronize(ev_stop));
+    if (ms.h_ctl[MS_ERR] & MS_ERR_TILE) fail(GB_EDEVICE, "internal error: inconsistent frontier tile index");
+    if (ms.h_ctl[MS_ERR] & MS_ERR_LOOKBACK) fail(GB_EDEVICE, "frontier compaction look-back timed out");
+    ms.last_depth = static_cast<int>(ms.h_ctl[MS_MAXLEV]) + 1;
+    if (st) {
+        float ms_elapsed = 0.f;
+        GG_HIP(hipEventElapsedTime(&ms_elapsed, ev_start, ev_stop));
+        std::memset(st, 0, sizeof(*st));
+        st->ms = ms_elapsed;
+        st->num_levels = ms.last_depth;
+        st->k = k;
+        hipLaunchKernelGGL(ms_stats_kernel, dim3(kStatsBlocks), dim3(256), 0, stream, a, ms.partial);
+        GG_HIP(hipGetLastError());
+        std::vector<unsigned long long> part(size_t(kStatsBlocks) * 64 * 2);
+        GG_HIP(hipMemcpyAsync(part.data(), ms.partial, sizeof(unsigned long long) * part.size(),
+                              hipMemcpyDeviceToHost, stream));
+        GG_HIP(hipStreamSynchronize(stream));
+        for (int b = 0; b < kStatsBlocks; ++b)
+            for (int i = 0; i < k; ++i) {
+                st->n_reached[i] += static_cast<int64_t>(part[(size_t(b) * 64 + i) * 2 + 0]);
+                st->m_reached[i] += static_cast<int64_t>(part[(size_t(b) * 64 + i) * 2 + 1]);
+            }
+    }
+}
+
+void DeviceGraph::bfs_batch_to_host(const int32_t* h_sources, int k, int32_t* h_levels, gb_batch_stats* st) {
+    if (!h_levels) fail(GB_EINVAL, "levels buffer is null");
+    if (k < 1 || k > 64) fail(GB_EINVAL, "a batch holds 1 to 64 sources");
+    DeviceGuard guard(device);
+    const size_t need = sizeof(int32_t) * size_t(n) * size_t(k);
+    if (ms.levels_bytes < need) {
+        if (ms.levels) (void)hipFree(ms.levels);
+        ms.levels = nullptr;
+        ms.levels_bytes = 0;
+        GG_HIP(hipMalloc(&ms.levels, need));
+        ms.levels_bytes = need;
+    }
+    bfs_batch(h_sources, k, ms.levels, st);
+    GG_HIP(hipMemcpyAsync(h_levels, ms.levels, need, hipMemcpyDeviceToHost, stream));
+    GG_HIP(hipStreamSynchronize(stream));
+}
+
+}  // namespace gg

@@ -149,6 +149,29 @@ int  gb_bfs_config(gb_dgraph* dg, int block, int edges_per_thread, int blocks_pe
 int  gb_bfs_get_config(const gb_dgraph* dg, int* block, int* edges_per_thread,
                        int* blocks_per_cu, int* mode, int64_t* dense_edges);
 
+/* ---- batched multi-source BFS (no reference counterpart: the reference
+ *      runs one source per gpuBFS object, 1D Partition/main.cu:17-35).
+ *      Up to 64 sources share one launch sequence with 64-bit masks; each
+ *      source's levels equal gb_bfs's for that source. ------------------- */
+typedef struct gb_batch_stats {
+    double  ms;              /* device time of the whole batch (hipEvent)       */
+    int32_t num_levels;      /* 1 + max level over the batch                    */
+    int32_t k;
+    int64_t n_reached[64];   /* per source                                       */
+    int64_t m_reached[64];   /* per source: sum of degrees of reached vertices   */
+} gb_batch_stats;
+
+/* sources: k host ints (1 <= k <= 64, repeats allowed).  levels: k rows of n
+ * int32, row i = levels from sources[i].  st (nullable) adds a statistics
+ * pass after the timed region. */
+int  gb_bfs_batch(gb_dgraph* dg, const int32_t* sources, int k, int32_t* levels, gb_batch_stats* st);
+int  gb_bfs_batch_device(gb_dgraph* dg, const int32_t* sources, int k, int32_t* d_levels,
+                         gb_batch_stats* st);
+/* Batch kernel shape (same pairs as gb_bfs_config) and whether a batch is
+ * replayed as one captured hipGraph (1) or launched kernel by kernel (0). */
+int  gb_batch_config(gb_dgraph* dg, int block, int edges_per_thread, int use_graph);
+int  gb_batch_get_config(const gb_dgraph* dg, int* block, int* edges_per_thread, int* use_graph);
+
 /* ---- device-wide exclusive scan (replaces PrefixSum::sum_scan_blelloch,
  *      1D Partition/src/prefixSum.cu:139-224).  d_out[i] = sum(d_in[0..i)).
  *      In-place (d_out == d_in) is allowed.  No allocation per call. ----- */

@@ -1,0 +1,118 @@
+"""GPU parity of the batched multi-source BFS (gb_bfs_batch): every source's
+level row must be bit-exact with the single-source path and the CPU oracle,
+with and without hipGraph replay, for batches of 1..64 sources (repeats
+allowed), on graphs with sinks, deep graphs and hubs."""
+import numpy as np
+import pytest
+
+import gpu_graphs_amd as gg
+from conftest import load_fixture
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(256, 1), (256, 2), (256, 4), (512, 2), (1024, 1), (64, 16)]
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def check_batch(G, dg, sources):
+    levels, st = dg.bfs_batch(sources)
+    want = O.bfs_multi(G.offsets, G.cols, sources)
+    np.testing.assert_array_equal(levels, want)
+    deg = np.diff(G.offsets)
+    for i, s in enumerate(sources):
+        reached = want[i] >= 0
+        assert st["n_reached"][i] == int(reached.sum())
+        assert st["m_reached"][i] == int(deg[reached].sum())
+    assert st["num_levels"] == int(want.max()) + 1
+    return levels
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+@pytest.mark.parametrize("block,k", SHAPES)
+def test_rmat_batch_shapes(gpu, block, k, use_graph):
+    G = gg.Graph.rmat(14, seed=51)
+    dg = gg.DeviceGraph(G, gpu)
+    dg.batch_config(block=block, edges_per_thread=k, use_graph=use_graph)
+    deg = np.diff(G.offsets)
+    rng = np.random.default_rng(block + k)
+    sources = rng.choice(np.nonzero(deg)[0], 64, replace=False).astype(np.int32)
+    log(f"rmat14 batch block={block} k={k} graph={use_graph}")
+    check_batch(G, dg, sources)
+
+
+@pytest.mark.parametrize("nsrc", [1, 2, 17, 63, 64])
+def test_batch_sizes_with_repeats_and_isolated(gpu, nsrc):
+    G = gg.Graph.rmat(13, seed=7)
+    dg = gg.DeviceGraph(G, gpu)
+    deg = np.diff(G.offsets)
+    rng = np.random.default_rng(nsrc)
+    pool = np.concatenate([np.nonzero(deg)[0][:40], np.nonzero(deg == 0)[0][:3]])
+    sources = rng.choice(pool, nsrc, replace=True).astype(np.int32)
+    check_batch(G, dg, sources)
+
+
+def test_batch_equals_single_source(gpu):
+    G = gg.Graph.rmat(16, seed=3)
+    dg = gg.DeviceGraph(G, gpu)
+    deg = np.diff(G.offsets)
+    sources = np.random.default_rng(0).choice(np.nonzero(deg)[0], 64, replace=False).astype(np.int32)
+    levels, _ = dg.bfs_batch(sources)
+    for i in range(0, 64, 7):
+        np.testing.assert_array_equal(levels[i], dg.bfs(int(sources[i]))[0])
+    np.testing.assert_array_equal(levels, O.bfs_multi(G.offsets, G.cols, sources))
+
+
+@pytest.mark.parametrize("name", ["wiki-Vote", "test2", "tree", "test_isolated_node"])
+def test_fixture_batches(gpu, name, golden_levels):
+    g = load_fixture(name)
+    dg = g.device_graph(gpu)
+    sources = golden_levels[f"{name}__sources"]
+    want = golden_levels[f"{name}__levels"]
+    for lo in range(0, len(sources), 64):
+        lv, _ = dg.bfs_batch(sources[lo:lo + 64])
+        np.testing.assert_array_equal(lv, want[lo:lo + 64])
+
+
+def test_deep_grid_beyond_depth_hint(gpu):
+    # hundreds of levels: the captured graph covers the depth hint, the rest
+    # runs as continuation launches; the second batch reuses the learned depth.
+    G = gg.Graph.grid(120, 90, 0.05, seed=9)
+    dg = gg.DeviceGraph(G, gpu)
+    sources = np.array([0, 5, 120 * 45 + 60, G.n - 1] * 4, dtype=np.int32)
+    for _ in range(2):
+        check_batch(G, dg, sources)
+
+
+def test_graph_replays_match_eager(gpu):
+    G = gg.Graph.rmat(15, seed=12)
+    dg = gg.DeviceGraph(G, gpu)
+    deg = np.diff(G.offsets)
+    rng = np.random.default_rng(4)
+    batches = [rng.choice(np.nonzero(deg)[0], 64, replace=False).astype(np.int32) for _ in range(3)]
+    dg.batch_config(use_graph=False)
+    eager = [dg.bfs_batch(b, stats=False)[0] for b in batches]
+    dg.batch_config(use_graph=True)
+    d_lv = gg.DeviceArray(gpu, np.int32, 64 * G.n)
+    for rep in range(4):
+        for b, want in zip(batches, eager):
+            dg.bfs_batch_device(b, d_lv)
+            np.testing.assert_array_equal(d_lv.numpy().reshape(64, G.n), want)
+            # eager single-source work between replays
+            np.testing.assert_array_equal(dg.bfs(int(b[rep]))[0], want[rep])
+
+
+def test_batch_errors(gpu):
+    G = gg.Graph.rmat(10, seed=1)
+    dg = gg.DeviceGraph(G, gpu)
+    with pytest.raises(gg.GBError) as e:
+        dg.bfs_batch(np.zeros(65, dtype=np.int32))
+    assert e.value.code == -1
+    with pytest.raises(gg.GBError) as e:
+        dg.bfs_batch(np.array([0, G.n], dtype=np.int32))
+    assert e.value.code == -5
+    with pytest.raises(gg.GBError):
+        dg.batch_config(block=100, edges_per_thread=1)
