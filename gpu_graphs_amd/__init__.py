@@ -25,7 +25,7 @@ from ._ffi import (GB_AUTO_N, GB_DROP_SELF_LOOPS, GB_KEEP_ORDER, GB_MODE_ASYNC, 
                    GB_SYMMETRIZE, BatchStats, GBError, Stats, check, lib)
 
 __all__ = [
-    "Graph", "DeviceGraph", "DeviceArray", "csr", "gpuBFS", "GBError", "Stats", "device_count",
+    "Graph", "DeviceGraph", "DeviceArray", "Server", "csr", "gpuBFS", "GBError", "Stats", "device_count",
     "exclusive_scan", "GB_AUTO_N", "GB_ONE_BASED", "GB_SYMMETRIZE", "GB_KEEP_ORDER", "GB_DROP_SELF_LOOPS",
     "GB_MODE_ASYNC", "GB_MODE_LEVEL_LOOP",
 ]
@@ -266,6 +266,77 @@ class DeviceGraph:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching BFS query server (gb_server_*).
+
+    Queries (graph id, source) are batched onto `lanes` concurrent GPU streams
+    as lanes and the device-memory budget allow; results stay in HBM until
+    released.  DeviceGraph objects added to the server are kept alive by it.
+    """
+
+    def __init__(self, device=0, max_batch=64, lanes=4, plain=False, memory_budget=0, queue_capacity=0):
+        cfg = _ffi.ServerCfg(device, max_batch, lanes, int(plain), memory_budget, queue_capacity)
+        h = C.c_void_p()
+        check(lib().gb_server_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self._graphs = []
+
+    def add_graph(self, dgraph):
+        gid = C.c_int()
+        check(lib().gb_server_add_graph(self._h, dgraph._h, C.byref(gid)))
+        self._graphs.append(dgraph)
+        return gid.value
+
+    def submit(self, gid, source):
+        t = C.c_uint64()
+        check(lib().gb_server_submit(self._h, int(gid), int(source), C.byref(t)))
+        return t.value
+
+    def poll(self, ticket, out=None):
+        """Returns (status, latency dict); fills `out` (int32[n]) when done."""
+        lat = _ffi.Latency()
+        rc = lib().gb_server_poll(self._h, ticket, _ptr(out), C.byref(lat))
+        return rc, lat.as_dict()
+
+    def wait(self, ticket, out=None, timeout_ms=-1):
+        """Returns (status, latency dict): GB_OK, GB_EAGAIN on timeout, or the
+        query's own failure status (GB_ENOMEM, GB_EDEVICE).  Raises for an
+        unknown ticket."""
+        lat = _ffi.Latency()
+        rc = lib().gb_server_wait(self._h, ticket, int(timeout_ms), _ptr(out), C.byref(lat))
+        if rc == _ffi.GB_EINVAL:
+            check(rc)
+        return rc, lat.as_dict()
+
+    def release(self, ticket):
+        check(lib().gb_server_release(self._h, ticket))
+
+    def drain(self, timeout_ms=-1):
+        rc = lib().gb_server_drain(self._h, int(timeout_ms))
+        if rc not in (_ffi.GB_OK, _ffi.GB_EAGAIN):
+            check(rc)
+        return rc == _ffi.GB_OK
+
+    def stats(self):
+        st = _ffi.ServerStats()
+        check(lib().gb_server_get_stats(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def reset_stats(self):
+        check(lib().gb_server_reset_stats(self._h))
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().gb_server_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 

@@ -70,6 +70,36 @@ class BatchStats(C.Structure):
                 "n_reached": list(self.n_reached)[:k], "m_reached": list(self.m_reached)[:k]}
 
 
+class ServerCfg(C.Structure):
+    _fields_ = [
+        ("device", C.c_int),
+        ("max_batch", C.c_int),
+        ("lanes", C.c_int),
+        ("plain", C.c_int),
+        ("memory_budget", C.c_int64),
+        ("queue_capacity", C.c_int64),
+    ]
+
+
+class Latency(C.Structure):
+    _fields_ = [("queue_ms", C.c_double), ("run_ms", C.c_double), ("total_ms", C.c_double),
+                ("batch_size", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64), ("rejected", C.c_int64),
+        ("batches", C.c_int64), ("mean_batch", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+        ("max_ms", C.c_double), ("qps", C.c_double), ("memory_used", C.c_int64), ("memory_budget", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 P = C.c_void_p
 PP = C.POINTER(C.c_void_p)
 i32p = C.POINTER(C.c_int32)
@@ -106,6 +136,17 @@ SIGNATURES = {
     "gb_bfs_batch_device": (C.c_int, [P, P, C.c_int, P, C.POINTER(BatchStats)]),
     "gb_batch_config": (C.c_int, [P, C.c_int, C.c_int, C.c_int]),
     "gb_batch_get_config": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "gb_server_create": (C.c_int, [C.POINTER(ServerCfg), PP]),
+    "gb_server_add_graph": (C.c_int, [P, P, C.POINTER(C.c_int)]),
+    "gb_server_submit": (C.c_int, [P, C.c_int, C.c_int32, C.POINTER(C.c_uint64)]),
+    "gb_server_poll": (C.c_int, [P, C.c_uint64, P, C.POINTER(Latency)]),
+    "gb_server_wait": (C.c_int, [P, C.c_uint64, C.c_int, P, C.POINTER(Latency)]),
+    "gb_server_result_device": (C.c_int, [P, C.c_uint64, C.POINTER(P)]),
+    "gb_server_release": (C.c_int, [P, C.c_uint64]),
+    "gb_server_drain": (C.c_int, [P, C.c_int]),
+    "gb_server_get_stats": (C.c_int, [P, C.POINTER(ServerStats)]),
+    "gb_server_reset_stats": (C.c_int, [P]),
+    "gb_server_free": (None, [P]),
     "gb_exclusive_scan_i32": (C.c_int, [C.c_int, P, P, C.c_int64]),
     "gb_dmalloc": (C.c_int, [C.c_int, C.c_size_t, PP]),
     "gb_dfree": (C.c_int, [C.c_int, P]),

@@ -32,10 +32,15 @@
 //    a packed (count << 32 | edges) counter gives both their queue slots and
 //    their edge offsets.
 //  * Dense levels skip the enqueue.  compact(L) rebuilds the queue from the
-//    bitmaps: new = visited & ~prev, with a single-pass exclusive scan of
-//    (count, edges) over 4096-vertex chunks -- per-thread serial sums, a
-//    wave64 scan, an LDS-staged block scan and a decoupled look-back across
-//    chunks (Merrill & Garland 2016) -- so no counter is shared.
+//    bitmaps: new = visited & ~prev over 4096-vertex chunks -- per-thread
+//    serial sums, a wave64 scan and an LDS-staged block scan give each new
+//    vertex its offset inside the chunk, and the chunk reserves its queue
+//    slots and edge range with one packed 64-bit atomicAdd, like a sparse
+//    tile.  No workgroup waits on another, so the kernels need no particular
+//    residency and can share the device with other streams' work.
+//  * Every kernel stops at once when the error word is set, and the expand
+//    tiles check each queue entry and column index against n and m, so an
+//    inconsistent queue is reported as an error, never dereferenced.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -59,7 +64,7 @@ enum : int {
     CTL_ERR = 7,     // internal consistency error (bit set)
     CTL_WORDS = 16
 };
-enum : unsigned long long { ERR_TILE = 1, ERR_LOOKBACK = 2 };
+enum : unsigned long long { ERR_TILE = 1, ERR_RANGE = 2 };
 
 constexpr int kCompactBlock = 256;
 constexpr int kCompactRounds = 16;                          // vertices per thread
@@ -77,14 +82,13 @@ struct BfsArgs {
     int32_t* q_v[2];
     uint32_t* tile_start[2];
     unsigned long long* ctl;
-    unsigned long long* lb[2];  // look-back status per compaction chunk, by level parity
     int64_t n;
+    int64_t m;
     uint32_t nwords;
     uint32_t nchunks;
     uint32_t dense_edges;
     uint32_t tile_edges;        // T of the expand kernel (tile_start granularity)
     int32_t source;
-    uint64_t spin_limit;        // wall-clock ticks a look-back may wait
 };
 
 template <int BLOCK, int K>
@@ -136,9 +140,12 @@ __device__ void expand_tile(const BfsArgs& a, TileLds<BLOCK, K>& s, int L, uint3
     if (c == 0) return;
 
     // Gather: K independent column loads per thread, then K bitmap probes.
+    // Edge e belongs to entry lo iff eoff[lo] <= e < eoff[lo + 1]; entries and
+    // columns that fail the range checks are dropped and reported.
     int32_t u[K];
     uint32_t owner[K];
     bool act[K];
+    bool bad = false;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const uint32_t e = base + k * BLOCK + tid;
@@ -148,9 +155,23 @@ __device__ void expand_tile(const BfsArgs& a, TileLds<BLOCK, K>& s, int L, uint3
         if (act[k]) {
             const uint32_t lo = owner_of(s.eoff, c, e);
             owner[k] = lo;
-            u[k] = a.cols[s.rb[lo] + (e - s.eoff[lo])];
+            const uint64_t pos = static_cast<uint64_t>(s.rb[lo]) + (e - s.eoff[lo]);
+            if (e < s.eoff[lo] || e >= s.eoff[lo + 1] || pos >= static_cast<uint64_t>(a.m)) {
+                act[k] = false;
+                bad = true;
+            } else {
+                u[k] = a.cols[pos];
+            }
         }
     }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (act[k] && static_cast<uint32_t>(u[k]) >= static_cast<uint64_t>(a.n)) {
+            act[k] = false;
+            bad = true;
+        }
+    }
+    if (bad) atomicOr(&a.ctl[CTL_ERR], ERR_RANGE);
     uint32_t word[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) word[k] = act[k] ? a.vis[static_cast<uint32_t>(u[k]) >> 5] : ~0u;
@@ -234,9 +255,18 @@ __device__ void expand_tile(const BfsArgs& a, TileLds<BLOCK, K>& s, int L, uint3
     }
     if (tid == 0) {
         s.win_deg[nwin] = running;
-        s.base = atomicAdd(&a.ctl[CTL_RING0 + (L + 1) % 3], (static_cast<unsigned long long>(nwin) << 32) | running);
+        const unsigned long long tot = (static_cast<unsigned long long>(nwin) << 32) | running;
+        s.base = atomicAdd(&a.ctl[CTL_RING0 + (L + 1) % 3], tot);
+        if (!reservation_fits(s.base, tot, a.n, a.m)) {
+            atomicOr(&a.ctl[CTL_ERR], ERR_RANGE);
+            s.nwin = 0;  // nothing is written
+        }
     }
     __syncthreads();
+    if (s.nwin == 0) {
+        __syncthreads();
+        return;
+    }
     const uint32_t slot0 = static_cast<uint32_t>(s.base >> 32);
     const uint32_t e0 = static_cast<uint32_t>(s.base);
     for (uint32_t i = tid; i < nwin; i += BLOCK) {
@@ -257,12 +287,17 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(BfsArgs a, int L) {
     __shared__ TileLds<BLOCK, K> s;
     constexpr uint32_t T = BLOCK * K;
     // Levels queued past the end of the search are no-ops: once done, the
-    // ring slots are no longer reset and would hold stale counts.
-    if (a.ctl[CTL_DONE]) return;
+    // ring slots are no longer reset and would hold stale counts.  After an
+    // error the queue is not trusted: nothing runs.
+    if (a.ctl[CTL_DONE] | a.ctl[CTL_ERR]) return;
     const unsigned long long cnt = a.ctl[CTL_RING0 + L % 3];  // final: written by earlier launches
     const uint32_t F = static_cast<uint32_t>(cnt >> 32), E = static_cast<uint32_t>(cnt);
     if (F == 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl[CTL_DONE] = 1;
+        return;
+    }
+    if (F > a.n || E > a.m) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.ctl[CTL_ERR], ERR_RANGE);
         return;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -280,10 +315,12 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(BfsArgs a, int L) {
 // Level L, step 2 (dense levels only): queue of level L+1 from the bitmaps.
 // Chunk c covers vertices [c*kChunk, (c+1)*kChunk); thread t takes vertices
 // c*kChunk + r*256 + t for r < 16, so the offsets loads of a wave are
-// contiguous.  The thread's new vertices occupy consecutive queue slots.
+// contiguous.  The thread's new vertices occupy consecutive queue slots; the
+// chunk's block goes where its atomicAdd on level L+1's counter puts it.
 __device__ void compact_chunk(const BfsArgs& a, int L, uint32_t c) {
     __shared__ unsigned long long s_wave[kCompactBlock / 64];
     __shared__ unsigned long long s_excl;
+    __shared__ uint32_t s_ok;
     __shared__ uint32_t s_zero[kCompactBlock / 64];
     const uint32_t tid = threadIdx.x;
     const int64_t vbase = int64_t(c) * kChunk;
@@ -320,18 +357,23 @@ __device__ void compact_chunk(const BfsArgs& a, int L, uint32_t c) {
         if (w < a.nwords) a.prev[w] = a.vis[w];
     }
     if (tid == 0) {
-        const uint32_t p = L & 1;
-        s_excl = lookback(a.lb[p], c, total, a.spin_limit, &a.ctl[CTL_ERR], ERR_LOOKBACK);
+        s_ok = 1;
+        if (total) {
+            s_excl = atomicAdd(&a.ctl[CTL_RING0 + (L + 1) % 3], total);
+            if (!reservation_fits(s_excl, total, a.n, a.m)) {
+                atomicOr(&a.ctl[CTL_ERR], ERR_RANGE);
+                s_ok = 0;
+            }
+        }
         uint32_t z = 0;
         for (int i = 0; i < kCompactBlock / 64; ++i) z += s_zero[i];
         if (z) {
             atomicAdd(&a.ctl[CTL_NREACH], static_cast<unsigned long long>(z));
             atomicMax(&a.ctl[CTL_MAXLEV], static_cast<unsigned long long>(L + 1));
         }
-        if (c == a.nchunks - 1) a.ctl[CTL_RING0 + (L + 1) % 3] = s_excl + total;  // level L+1 totals
     }
     __syncthreads();
-    if (!newmask) return;
+    if (!newmask || !s_ok) return;
     const unsigned long long pre = s_excl + excl_local;
     uint32_t slot = static_cast<uint32_t>(pre >> 32);
     uint32_t eo = static_cast<uint32_t>(pre);
@@ -352,14 +394,12 @@ __device__ void compact_chunk(const BfsArgs& a, int L, uint32_t c) {
     }
 }
 
+// Level L+1's counter ring[(L+1)%3] was zeroed by expand(L-1) and, on a
+// dense level, expand(L) left it alone: the chunks' reservations build it up.
 __global__ __launch_bounds__(kCompactBlock) void bfs_compact_kernel(BfsArgs a, int L) {
-    if (a.ctl[CTL_DONE]) return;
+    if (a.ctl[CTL_DONE] | a.ctl[CTL_ERR]) return;
     const unsigned long long cnt = a.ctl[CTL_RING0 + L % 3];
     const uint32_t F = static_cast<uint32_t>(cnt >> 32), E = static_cast<uint32_t>(cnt);
-    // The look-back status of level L+1 (other parity) starts from zero.
-    unsigned long long* other = a.lb[(L + 1) & 1];
-    for (uint32_t i = blockIdx.x * kCompactBlock + threadIdx.x; i < a.nchunks; i += gridDim.x * kCompactBlock)
-        other[i] = 0;
     if (F == 0 || E <= a.dense_edges) return;  // sparse level: expand() queued level L+1
     for (uint32_t c = blockIdx.x; c < a.nchunks; c += gridDim.x) {
         compact_chunk(a, L, c);
@@ -367,7 +407,7 @@ __global__ __launch_bounds__(kCompactBlock) void bfs_compact_kernel(BfsArgs a, i
     }
 }
 
-// levels = -1, parents = -1, bitmaps and look-back status = 0.
+// levels = -1, parents = -1, bitmaps = 0.
 __global__ void bfs_clear_kernel(BfsArgs a) {
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n; i += stride) {
@@ -376,10 +416,6 @@ __global__ void bfs_clear_kernel(BfsArgs a) {
         if (i < a.nwords) {
             a.vis[i] = 0u;
             a.prev[i] = 0u;
-        }
-        if (i < a.nchunks) {
-            a.lb[0][i] = 0;
-            a.lb[1][i] = 0;
         }
     }
 }
@@ -440,11 +476,31 @@ DeviceGraph::DeviceGraph(const HostGraph& g, int dev) : device(dev), n(g.n), m(g
     hipDeviceProp_t prop;
     GG_HIP(hipGetDeviceProperties(&prop, device));
     num_cus = prop.multiProcessorCount;
-    int clock_khz = 0;
-    GG_HIP(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, device));
-    if (clock_khz <= 0) clock_khz = 100000;
-    spin_limit_ticks = static_cast<uint64_t>(clock_khz) * 1000ull * kSpinTimeoutSeconds;
     cfg.dense_edges = default_dense_edges();
+    try {
+        GG_HIP(hipMalloc(&offsets, sizeof(int64_t) * (n + 1)));
+        GG_HIP(hipMalloc(&cols, sizeof(int32_t) * std::max<int64_t>(m, 1)));
+        graph_bytes = sizeof(int64_t) * (n + 1) + sizeof(int32_t) * std::max<int64_t>(m, 1);
+        GG_HIP(hipMemcpy(offsets, g.offsets.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
+        if (m) GG_HIP(hipMemcpy(cols, g.cols.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice));
+        runner = std::make_unique<Runner>(*this);
+    } catch (...) {
+        this->~DeviceGraph();
+        throw;
+    }
+}
+
+DeviceGraph::~DeviceGraph() {
+    (void)hipSetDevice(device);
+    runner.reset();
+    if (offsets) (void)hipFree(offsets);
+    if (cols) (void)hipFree(cols);
+    offsets = nullptr;
+    cols = nullptr;
+}
+
+Runner::Runner(DeviceGraph& graph) : g(graph), device(graph.device), n(graph.n), m(graph.m) {
+    DeviceGuard guard(device);
     try {
         GG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         GG_HIP(hipEventCreate(&ev_start));
@@ -454,26 +510,17 @@ DeviceGraph::DeviceGraph(const HostGraph& g, int dev) : device(dev), n(g.n), m(g
         GG_HIP(hipEventCreateWithFlags(&ev_batch[1], hipEventDisableTiming));
         GG_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_ctl), sizeof(unsigned long long) * 2 * kCtlWords,
                              hipHostMallocDefault));
-        GG_HIP(hipMalloc(&offsets, sizeof(int64_t) * (n + 1)));
-        GG_HIP(hipMalloc(&cols, sizeof(int32_t) * std::max<int64_t>(m, 1)));
-        graph_bytes = sizeof(int64_t) * (n + 1) + sizeof(int32_t) * std::max<int64_t>(m, 1);
-        GG_HIP(hipMemcpyAsync(offsets, g.offsets.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, stream));
-        if (m) GG_HIP(hipMemcpyAsync(cols, g.cols.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, stream));
-        GG_HIP(hipStreamSynchronize(stream));
     } catch (...) {
-        this->~DeviceGraph();
+        this->~Runner();
         throw;
     }
 }
 
-DeviceGraph::~DeviceGraph() {
+Runner::~Runner() {
     (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
     free_workspace();
     free_ms_workspace();
-    if (offsets) (void)hipFree(offsets);
-    if (cols) (void)hipFree(cols);
-    offsets = nullptr;
-    cols = nullptr;
     if (h_ctl) (void)hipHostFree(h_ctl);
     h_ctl = nullptr;
     for (hipEvent_t* e : {&ev_start, &ev_stop, &ev_kernel, &ev_batch[0], &ev_batch[1]}) {
@@ -484,17 +531,17 @@ DeviceGraph::~DeviceGraph() {
     stream = nullptr;
 }
 
-void DeviceGraph::free_workspace() {
-    void* ptrs[] = {ws.vis,           ws.prev,          ws.q_rb[0], ws.q_rb[1], ws.q_eoff[0], ws.q_eoff[1],
-                    ws.q_v[0],        ws.q_v[1],        ws.ctl,     ws.lb[0],   ws.lb[1],     ws.levels,
-                    ws.parents,       ws.tile_start[0], ws.tile_start[1]};
+void Runner::free_workspace() {
+    void* ptrs[] = {ws.vis,    ws.prev,    ws.q_rb[0],  ws.q_rb[1],       ws.q_eoff[0],    ws.q_eoff[1],
+                    ws.q_v[0], ws.q_v[1],  ws.ctl,      ws.levels,        ws.parents,      ws.tile_start[0],
+                    ws.tile_start[1]};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     ws = BfsWorkspace{};
 }
 
-void DeviceGraph::ensure_workspace() {
-    const int T = cfg.block * cfg.edges_per_thread;
+void Runner::ensure_workspace() {
+    const int T = g.cfg.block * g.cfg.edges_per_thread;
     const int64_t tile_cap = (m + T - 1) / T + 2;
     if (ws.ctl && ws.tile_edges == T) return;
     if (ws.ctl) {  // only the tile index depends on T
@@ -524,7 +571,6 @@ void DeviceGraph::ensure_workspace() {
             alloc(&ws.q_eoff[i], sizeof(uint32_t) * n);
             alloc(&ws.q_v[i], sizeof(int32_t) * n);
             alloc(&ws.tile_start[i], sizeof(uint32_t) * tile_cap);
-            alloc(&ws.lb[i], sizeof(unsigned long long) * nchunks);
         }
         alloc(&ws.ctl, sizeof(unsigned long long) * kCtlWords);
         alloc(&ws.levels, sizeof(int32_t) * n);
@@ -539,19 +585,19 @@ void DeviceGraph::ensure_workspace() {
     ws.bytes = bytes;
 }
 
-void DeviceGraph::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats* st) {
+void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats* st) {
     static_assert(kCtlWords == CTL_WORDS, "control block size");
     if (source < 0 || source >= n)
         fail(GB_ERANGE, "source " + std::to_string(source) + " outside [0, " + std::to_string(n) + ")");
     if (!d_levels) fail(GB_EINVAL, "levels buffer is null");
-    const KernelSet* ks = find_kernels(cfg.block, cfg.edges_per_thread);
+    const KernelSet* ks = find_kernels(g.cfg.block, g.cfg.edges_per_thread);
     if (!ks) fail(GB_EINVAL, "unsupported block / edges_per_thread combination");
     DeviceGuard guard(device);
     ensure_workspace();
 
     BfsArgs a;
-    a.offsets = offsets;
-    a.cols = cols;
+    a.offsets = g.offsets;
+    a.cols = g.cols;
     a.vis = ws.vis;
     a.prev = ws.prev;
     a.levels = d_levels;
@@ -561,31 +607,29 @@ void DeviceGraph::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_
         a.q_eoff[i] = ws.q_eoff[i];
         a.q_v[i] = ws.q_v[i];
         a.tile_start[i] = ws.tile_start[i];
-        a.lb[i] = ws.lb[i];
     }
     a.ctl = ws.ctl;
     a.n = n;
+    a.m = m;
     a.nwords = static_cast<uint32_t>((n + 31) / 32);
     a.nchunks = static_cast<uint32_t>(ws.nchunks);
-    a.dense_edges = static_cast<uint32_t>(std::min<int64_t>(cfg.dense_edges, 0xffffffffll));
-    a.tile_edges = static_cast<uint32_t>(cfg.block * cfg.edges_per_thread);
+    a.dense_edges = static_cast<uint32_t>(std::min<int64_t>(g.cfg.dense_edges, 0xffffffffll));
+    a.tile_edges = static_cast<uint32_t>(g.cfg.block * g.cfg.edges_per_thread);
     a.source = source;
-    a.spin_limit = spin_limit_ticks;
 
     int occ = 0;
-    GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(ks->expand), cfg.block, 0));
+    GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(ks->expand), g.cfg.block, 0));
     if (occ < 1) fail(GB_EDEVICE, "BFS kernel cannot be resident on a CU");
-    const int waves_per_block = (cfg.block + 63) / 64;
+    const int waves_per_block = (g.cfg.block + 63) / 64;
     const int default_per_cu = std::max(1, std::min(occ, 16 / waves_per_block));
-    const int per_cu = cfg.blocks_per_cu > 0 ? std::min(cfg.blocks_per_cu, occ) : default_per_cu;
-    const uint32_t grid = static_cast<uint32_t>(per_cu * num_cus);
-    // compact(): chunks are assigned statically and wait on lower-numbered
-    // chunks, so the whole grid must be resident: 4 blocks of 256 per CU.
-    const uint32_t cgrid = static_cast<uint32_t>(std::min<int64_t>(ws.nchunks, int64_t(num_cus) * 4));
-    const uint32_t clear_grid = static_cast<uint32_t>(std::min<int64_t>((n + 255) / 256, int64_t(num_cus) * 8));
+    const int per_cu = g.cfg.blocks_per_cu > 0 ? std::min(g.cfg.blocks_per_cu, occ) : default_per_cu;
+    const uint32_t grid = static_cast<uint32_t>(per_cu * g.num_cus);
+    // compact(): chunks are independent; 4 workgroups of 256 per CU.
+    const uint32_t cgrid = static_cast<uint32_t>(std::min<int64_t>(ws.nchunks, int64_t(g.num_cus) * 4));
+    const uint32_t clear_grid = static_cast<uint32_t>(std::min<int64_t>((n + 255) / 256, int64_t(g.num_cus) * 8));
 
     auto launch_level = [&](int L) {
-        hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(cfg.block), 0, stream, a, L);
+        hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(g.cfg.block), 0, stream, a, L);
         hipLaunchKernelGGL(bfs_compact_kernel, dim3(cgrid), dim3(kCompactBlock), 0, stream, a, L);
     };
 
@@ -595,7 +639,7 @@ void DeviceGraph::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_
     GG_HIP(hipGetLastError());
     GG_HIP(hipEventRecord(ev_kernel, stream));
     int L = 0;
-    if (cfg.mode == GB_MODE_LEVEL_LOOP) {
+    if (g.cfg.mode == GB_MODE_LEVEL_LOOP) {
         // One level at a time; the host reads the next frontier size.
         for (;; ++L) {
             if (L > n) fail(GB_EDEVICE, "internal error: more BFS levels than vertices");
@@ -604,6 +648,7 @@ void DeviceGraph::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_
             GG_HIP(hipMemcpyAsync(h_ctl, ws.ctl, sizeof(unsigned long long) * kCtlWords, hipMemcpyDeviceToHost,
                                   stream));
             GG_HIP(hipStreamSynchronize(stream));
+            if (h_ctl[CTL_ERR]) break;
             if ((h_ctl[CTL_RING0 + (L + 1) % 3] >> 32) == 0) {
                 ++L;
                 launch_level(L);  // sets the done flag and the final counters
@@ -625,7 +670,7 @@ void DeviceGraph::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_
         };
         auto done = [&](int slot) {
             GG_HIP(hipEventSynchronize(ev_batch[slot]));
-            return h_ctl[slot * kCtlWords + CTL_DONE] != 0;
+            return h_ctl[slot * kCtlWords + CTL_DONE] != 0 || h_ctl[slot * kCtlWords + CTL_ERR] != 0;
         };
         enqueue(last_num_levels > 0 ? last_num_levels + 1 : 8, 0);
         if (!done(0)) {
@@ -644,7 +689,7 @@ void DeviceGraph::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_
     GG_HIP(hipMemcpyAsync(fin, ws.ctl, sizeof(fin), hipMemcpyDeviceToHost, stream));
     GG_HIP(hipStreamSynchronize(stream));
     if (fin[CTL_ERR] & ERR_TILE) fail(GB_EDEVICE, "internal error: inconsistent frontier tile index");
-    if (fin[CTL_ERR] & ERR_LOOKBACK) fail(GB_EDEVICE, "frontier compaction look-back timed out");
+    if (fin[CTL_ERR] & ERR_RANGE) fail(GB_EDEVICE, "internal error: frontier entry or column index out of range");
     if (!fin[CTL_DONE]) fail(GB_EDEVICE, "internal error: BFS did not reach an empty frontier");
     last_num_levels = static_cast<int>(fin[CTL_MAXLEV]) + 1;
     if (st) {
@@ -660,7 +705,7 @@ void DeviceGraph::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_
     }
 }
 
-void DeviceGraph::bfs_to_host(int32_t source, int32_t* h_levels, int32_t* h_parents, gb_stats* st) {
+void Runner::bfs_to_host(int32_t source, int32_t* h_levels, int32_t* h_parents, gb_stats* st) {
     if (!h_levels) fail(GB_EINVAL, "levels buffer is null");
     DeviceGuard guard(device);
     ensure_workspace();

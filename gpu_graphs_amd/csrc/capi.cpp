@@ -9,22 +9,10 @@
 
 #include "common.hpp"
 #include "device_graph.hpp"
+#include "handles.hpp"
 #include "host_graph.hpp"
 
-struct gb_graph {
-    gg::HostGraph g;
-};
-struct gb_dgraph {
-    std::unique_ptr<gg::DeviceGraph> d;
-};
-
 namespace gg {
-
-namespace {
-thread_local std::string t_last_error;
-}
-void set_last_error(const std::string& msg) { t_last_error = msg; }
-const char* last_error() { return t_last_error.c_str(); }
 
 template <class F>
 int guarded(F&& fn) {

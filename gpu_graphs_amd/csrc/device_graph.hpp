@@ -9,7 +9,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <memory>
 #include <string>
 
 #include "common.hpp"
@@ -55,7 +57,6 @@ struct BfsWorkspace {
     uint32_t* q_eoff[2] = {nullptr, nullptr};   // frontier: exclusive prefix of degrees in the level
     int32_t* q_v[2] = {nullptr, nullptr};       // frontier: vertex id
     uint32_t* tile_start[2] = {nullptr, nullptr};  // frontier entry that owns edge t*T
-    unsigned long long* lb[2] = {nullptr, nullptr};  // compaction look-back status per chunk
     unsigned long long* ctl = nullptr;          // counters ring, stats, flags (see bfs.hip)
     int32_t* levels = nullptr;                  // levels staging for host-pointer calls
     int32_t* parents = nullptr;
@@ -82,7 +83,6 @@ struct MsWorkspace {
     unsigned long long* q_fm = nullptr;   // their frontier masks
     uint32_t* tile_start = nullptr;
     unsigned long long* ctl = nullptr;
-    unsigned long long* lb[2] = {nullptr, nullptr};
     int32_t* sources = nullptr;           // 64, device
     int32_t* h_sources = nullptr;         // 64, pinned (read by the captured copy)
     unsigned long long* h_ctl = nullptr;  // pinned snapshot of ctl
@@ -99,47 +99,43 @@ struct MsWorkspace {
     int32_t* graph_levels = nullptr;
 };
 
-struct DeviceGraph {
-    int device = 0;
-    int64_t n = 0, m = 0;
-    int64_t* offsets = nullptr;  // n + 1
-    int32_t* cols = nullptr;     // m
-    size_t graph_bytes = 0;
+struct DeviceGraph;
+
+// Execution context of BFS calls on one graph: a HIP stream, its events and
+// the single- and multi-source workspaces.  A DeviceGraph owns one, used by
+// the gb_bfs* calls; the query server creates one per lane so that batches on
+// the same graph can run concurrently on different streams.
+struct Runner {
+    DeviceGraph& g;
+    const int device;
+    const int64_t n, m;
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_kernel = nullptr;
     hipEvent_t ev_batch[2] = {nullptr, nullptr};
     static constexpr int kCtlWords = 16;
     unsigned long long* h_ctl = nullptr;  // pinned: 2 snapshots of the control block
-    int num_cus = 0;
-    // Longest a device-side wait (compaction look-back) spins before the BFS
-    // fails with an error; a legitimate wait takes microseconds.
-    static constexpr uint64_t kSpinTimeoutSeconds = 5;
-    uint64_t spin_limit_ticks = 0;
     int last_num_levels = 0;              // depth of the previous BFS: first batch size
-    BfsConfig cfg;
     BfsWorkspace ws;
-    MsConfig ms_cfg;
     MsWorkspace ms;
 
-    DeviceGraph(const HostGraph& g, int device);
-    ~DeviceGraph();
-    DeviceGraph(const DeviceGraph&) = delete;
-    DeviceGraph& operator=(const DeviceGraph&) = delete;
+    explicit Runner(DeviceGraph& graph);
+    ~Runner();
+    Runner(const Runner&) = delete;
+    Runner& operator=(const Runner&) = delete;
 
     // levels / parents are device pointers (parents may be null).
     void bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats* st);
-    // Host-pointer variant: runs into the handle's staging buffers, then
-    // copies levels (and parents) to the host.
+    // Host-pointer variant: runs into staging buffers, then copies levels
+    // (and parents) to the host.
     void bfs_to_host(int32_t source, int32_t* h_levels, int32_t* h_parents, gb_stats* st);
     // Batched multi-source BFS of k <= 64 sources (host array); levels is a
     // device buffer of k rows of n int32.
     void bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_batch_stats* st);
     void bfs_batch_to_host(const int32_t* h_sources, int k, int32_t* h_levels, gb_batch_stats* st);
-    // HBM a batch needs beyond the graph (for admission control).
-    static size_t batch_bytes_estimate(int64_t n, int64_t m, int k, bool host_levels);
+    // Allocates the batch workspace now (the server charges it to its budget).
+    void prepare_batch();
 
-    size_t memory_bytes() const { return graph_bytes + ws.bytes + ms.bytes + ms.levels_bytes; }
-    int64_t default_dense_edges() const { return std::max<int64_t>(int64_t(1) << 19, m / 256); }
+    size_t bytes() const { return ws.bytes + ms.bytes + ms.levels_bytes; }
 
    private:
     void ensure_workspace();
@@ -148,8 +144,44 @@ struct DeviceGraph {
     void free_ms_workspace();
 };
 
+struct DeviceGraph {
+    int device = 0;
+    int64_t n = 0, m = 0;
+    int64_t* offsets = nullptr;  // n + 1
+    int32_t* cols = nullptr;     // m
+    size_t graph_bytes = 0;
+    int num_cus = 0;
+    BfsConfig cfg;
+    MsConfig ms_cfg;
+    std::unique_ptr<Runner> runner;  // context of the gb_bfs* calls
+
+    DeviceGraph(const HostGraph& g, int device);
+    ~DeviceGraph();
+    DeviceGraph(const DeviceGraph&) = delete;
+    DeviceGraph& operator=(const DeviceGraph&) = delete;
+
+    void bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats* st) {
+        runner->bfs(source, d_levels, d_parents, st);
+    }
+    void bfs_to_host(int32_t source, int32_t* h_levels, int32_t* h_parents, gb_stats* st) {
+        runner->bfs_to_host(source, h_levels, h_parents, st);
+    }
+    void bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_batch_stats* st) {
+        runner->bfs_batch(h_sources, k, d_levels, st);
+    }
+    void bfs_batch_to_host(const int32_t* h_sources, int k, int32_t* h_levels, gb_batch_stats* st) {
+        runner->bfs_batch_to_host(h_sources, k, h_levels, st);
+    }
+
+    size_t memory_bytes() const { return graph_bytes + (runner ? runner->bytes() : 0); }
+    int64_t default_dense_edges() const { return std::max<int64_t>(int64_t(1) << 19, m / 256); }
+};
+
 bool bfs_config_supported(int block, int k);
 bool ms_config_supported(int block, int k);
+// HBM a batch workspace needs beyond the graph (plus the staging buffer when
+// results go to the host).
+size_t batch_bytes_estimate(int64_t n, int64_t m, int k, bool host_levels);
 
 // Device-wide exclusive scan of int32 (scan.hip).
 void exclusive_scan_i32(int device, const int32_t* d_in, int32_t* d_out, int64_t n);

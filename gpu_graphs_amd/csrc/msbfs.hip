@@ -10,8 +10,9 @@
 //   update_compact(L): for every vertex with next != 0: new = next & ~seen,
 //       seen |= new, level[s][v] = L for every bit s of new, and (degree > 0)
 //       an entry (vertex, first CSR entry, edge prefix, mask = new) in the
-//       queue of active vertices; the edge prefix comes from the same
-//       single-pass look-back scan as the single-source compaction.
+//       queue of active vertices; a block scan orders the entries of a
+//       4096-vertex chunk and one packed atomicAdd per chunk reserves their
+//       slots and edge range, as in the single-source compaction.
 //   expand(L): edge-parallel tiles over the queue; for edge (u, v) the bits
 //       mask(u) & ~seen(v) are OR-ed into next(v) with a 64-bit atomicOr whose
 //       result is not needed, so the atomics do not stall the wave.
@@ -34,8 +35,10 @@ namespace gg {
 
 namespace {
 
-enum : int { MS_CNT = 0, MS_DONE = 1, MS_ERR = 2, MS_MAXLEV = 3, MS_WORDS = 16 };
-enum : unsigned long long { MS_ERR_TILE = 1, MS_ERR_LOOKBACK = 2 };
+// ctl[] layout (64-bit words).  Level L's queue counter is cnt[L & 1]:
+// update_compact(L) builds it up and zeroes the other parity for level L+1.
+enum : int { MS_CNT0 = 0, MS_DONE = 2, MS_ERR = 3, MS_MAXLEV = 4, MS_WORDS = 16 };
+enum : unsigned long long { MS_ERR_TILE = 1, MS_ERR_RANGE = 2 };
 
 constexpr int kUcBlock = 256;
 constexpr int kUcRounds = 16;
@@ -54,12 +57,11 @@ struct MsArgs {
     unsigned long long* q_fm;    // frontier mask of each queue entry
     uint32_t* tile_start;
     unsigned long long* ctl;
-    unsigned long long* lb[2];
     int64_t n;
+    int64_t m;
     int k;
     uint32_t nchunks;
     uint32_t tile_edges;
-    uint64_t spin_limit;
 };
 
 template <int BLOCK, int K>
@@ -70,17 +72,13 @@ struct MsTileLds {
     unsigned long long fm[T + 1];
 };
 
-// seen = next = 0; levels = -1 (k rows of n); look-back status = 0.
+// seen = next = 0; levels = -1 (k rows of n).
 __global__ void ms_clear_kernel(MsArgs a) {
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
     const int64_t tid0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     for (int64_t i = tid0; i < a.n; i += stride) {
         a.seen[i] = 0;
         a.next[i] = 0;
-        if (i < a.nchunks) {
-            a.lb[0][i] = 0;
-            a.lb[1][i] = 0;
-        }
     }
     const int64_t total = a.n * a.k;
     int4* lv4 = reinterpret_cast<int4*>(a.levels);
@@ -114,6 +112,7 @@ __device__ __forceinline__ void write_levels(const MsArgs& a, int64_t v, unsigne
 __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c) {
     __shared__ unsigned long long s_wave[kUcBlock / 64];
     __shared__ unsigned long long s_excl;
+    __shared__ uint32_t s_ok;
     const uint32_t tid = threadIdx.x;
     const int64_t vbase = int64_t(c) * kUcChunk;
     uint32_t active = 0;  // bit r: vertex vbase + r*256 + tid goes to the queue
@@ -152,14 +151,20 @@ __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c) {
     unsigned long long total;
     const unsigned long long excl_local = block_exclusive_scan<kUcBlock>(mine, s_wave, total);
     if (tid == 0) {
-        s_excl = lookback(a.lb[L & 1], c, total, a.spin_limit, &a.ctl[MS_ERR], MS_ERR_LOOKBACK);
-        if (c == a.nchunks - 1) a.ctl[MS_CNT] = s_excl + total;
+        s_ok = 1;
+        if (total) {
+            s_excl = atomicAdd(&a.ctl[MS_CNT0 + (L & 1)], total);
+            if (!reservation_fits(s_excl, total, a.n, a.m)) {
+                atomicOr(&a.ctl[MS_ERR], MS_ERR_RANGE);
+                s_ok = 0;
+            }
+        }
         // Every writer of this level stores the same L and levels run in
         // order, so the final value is the deepest level reached.
         if (anynew) a.ctl[MS_MAXLEV] = static_cast<unsigned long long>(L);
     }
     __syncthreads();
-    if (!active) return;
+    if (!active || !s_ok) return;
     const unsigned long long pre = s_excl + excl_local;
     uint32_t slot = static_cast<uint32_t>(pre >> 32);
     uint32_t eo = static_cast<uint32_t>(pre);
@@ -183,9 +188,9 @@ __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c) {
 // Level L, step 1: assign level L to the bits found by expand(L-1) and queue
 // the vertices that carry them.
 __global__ __launch_bounds__(kUcBlock) void ms_update_compact_kernel(MsArgs a, int L) {
-    if (a.ctl[MS_DONE]) return;
-    unsigned long long* other = a.lb[(L + 1) & 1];
-    for (uint32_t i = blockIdx.x * kUcBlock + threadIdx.x; i < a.nchunks; i += gridDim.x * kUcBlock) other[i] = 0;
+    if (a.ctl[MS_DONE] | a.ctl[MS_ERR]) return;
+    // cnt[(L+1) & 1] was last read by expand(L-1), which has finished.
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl[MS_CNT0 + ((L + 1) & 1)] = 0;
     for (uint32_t c = blockIdx.x; c < a.nchunks; c += gridDim.x) {
         ms_uc_chunk(a, L, c);
         __syncthreads();
@@ -197,11 +202,15 @@ template <int BLOCK, int K>
 __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(MsArgs a, int L) {
     __shared__ MsTileLds<BLOCK, K> s;
     constexpr uint32_t T = BLOCK * K;
-    if (a.ctl[MS_DONE]) return;
-    const unsigned long long cnt = a.ctl[MS_CNT];
+    if (a.ctl[MS_DONE] | a.ctl[MS_ERR]) return;
+    const unsigned long long cnt = a.ctl[MS_CNT0 + (L & 1)];
     const uint32_t F = static_cast<uint32_t>(cnt >> 32), E = static_cast<uint32_t>(cnt);
     if (F == 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl[MS_DONE] = 1;
+        return;
+    }
+    if (F > a.n || E > a.m) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.ctl[MS_ERR], MS_ERR_RANGE);
         return;
     }
     const uint32_t ntiles = static_cast<uint32_t>((uint64_t(E) + T - 1) / T);
@@ -228,8 +237,12 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(MsArgs a, int L) {
         }
         __syncthreads();
         if (c) {
+            // Edge e belongs to entry o iff eoff[o] <= e < eoff[o + 1];
+            // entries and columns failing the range checks are dropped and
+            // reported.
             int32_t v[K];
             unsigned long long fm[K];
+            bool bad = false;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 const uint32_t e = base + k * BLOCK + tid;
@@ -237,10 +250,27 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(MsArgs a, int L) {
                 fm[k] = 0;
                 if (e < end) {
                     const uint32_t o = owner_of(s.eoff, c, e);
-                    fm[k] = s.fm[o];
-                    v[k] = a.cols[s.rb[o] + (e - s.eoff[o])];
+                    const uint64_t pos = static_cast<uint64_t>(s.rb[o]) + (e - s.eoff[o]);
+                    if (e < s.eoff[o] || e >= s.eoff[o + 1] || pos >= static_cast<uint64_t>(a.m)) {
+                        bad = true;
+                    } else {
+                        fm[k] = s.fm[o];
+                        v[k] = a.cols[pos];
+                    }
                 }
             }
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (v[k] >= 0 && v[k] >= a.n) {
+                    v[k] = -1;
+                    bad = true;
+                }
+                if (v[k] < -1) {
+                    v[k] = -1;
+                    bad = true;
+                }
+            }
+            if (bad) atomicOr(&a.ctl[MS_ERR], MS_ERR_RANGE);
             unsigned long long sv[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) sv[k] = v[k] >= 0 ? a.seen[v[k]] : ~0ull;
@@ -316,10 +346,10 @@ constexpr int kStatsBlocks = 1024;
 
 bool ms_config_supported(int block, int k) { return find_ms_kernels(block, k) != nullptr; }
 
-void DeviceGraph::free_ms_workspace() {
+void Runner::free_ms_workspace() {
     if (ms.graph_exec) (void)hipGraphExecDestroy(ms.graph_exec);
-    void* ptrs[] = {ms.seen, ms.next, ms.q_rb, ms.q_eoff, ms.q_v, ms.q_fm, ms.tile_start, ms.ctl, ms.lb[0],
-                    ms.lb[1], ms.sources, ms.levels, ms.partial};
+    void* ptrs[] = {ms.seen,       ms.next, ms.q_rb,    ms.q_eoff,  ms.q_v,    ms.q_fm,
+                    ms.tile_start, ms.ctl,  ms.sources, ms.levels, ms.partial};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (ms.h_sources) (void)hipHostFree(ms.h_sources);
@@ -327,8 +357,8 @@ void DeviceGraph::free_ms_workspace() {
     ms = MsWorkspace{};
 }
 
-void DeviceGraph::ensure_ms_workspace() {
-    const int T = ms_cfg.block * ms_cfg.edges_per_thread;
+void Runner::ensure_ms_workspace() {
+    const int T = g.ms_cfg.block * g.ms_cfg.edges_per_thread;
     const int64_t tile_cap = (m + T - 1) / T + 2;
     if (ms.ctl && ms.tile_edges == T) return;
     if (ms.ctl) {
@@ -358,8 +388,6 @@ void DeviceGraph::ensure_ms_workspace() {
         alloc(&ms.q_fm, sizeof(unsigned long long) * n);
         alloc(&ms.tile_start, sizeof(uint32_t) * tile_cap);
         alloc(&ms.ctl, sizeof(unsigned long long) * kCtlWords);
-        alloc(&ms.lb[0], sizeof(unsigned long long) * nchunks);
-        alloc(&ms.lb[1], sizeof(unsigned long long) * nchunks);
         alloc(&ms.sources, sizeof(int32_t) * 64);
         alloc(&ms.partial, sizeof(unsigned long long) * kStatsBlocks * 64 * 2);
         GG_HIP(hipHostMalloc(reinterpret_cast<void**>(&ms.h_sources), sizeof(int32_t) * 64, hipHostMallocDefault));
@@ -375,7 +403,12 @@ void DeviceGraph::ensure_ms_workspace() {
     ms.bytes = bytes;
 }
 
-size_t DeviceGraph::batch_bytes_estimate(int64_t n_vertices, int64_t m_entries, int k, bool host_levels) {
+void Runner::prepare_batch() {
+    DeviceGuard guard(device);
+    ensure_ms_workspace();
+}
+
+size_t batch_bytes_estimate(int64_t n_vertices, int64_t m_entries, int k, bool host_levels) {
     const int64_t nchunks = (n_vertices + kUcChunk - 1) / kUcChunk;
     size_t b = sizeof(unsigned long long) * n_vertices * 3 + (sizeof(int64_t) + 8) * n_vertices +
                sizeof(uint32_t) * (m_entries / 256 + 2) + 16 * nchunks + 1024 * 64 * 16;
@@ -383,20 +416,20 @@ size_t DeviceGraph::batch_bytes_estimate(int64_t n_vertices, int64_t m_entries, 
     return b;
 }
 
-void DeviceGraph::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_batch_stats* st) {
+void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_batch_stats* st) {
     if (k < 1 || k > 64) fail(GB_EINVAL, "a batch holds 1 to 64 sources");
     if (!h_sources || !d_levels) fail(GB_EINVAL, "null sources or levels");
     for (int i = 0; i < k; ++i)
         if (h_sources[i] < 0 || h_sources[i] >= n)
             fail(GB_ERANGE, "source " + std::to_string(h_sources[i]) + " outside [0, " + std::to_string(n) + ")");
-    const MsKernelSet* ks = find_ms_kernels(ms_cfg.block, ms_cfg.edges_per_thread);
+    const MsKernelSet* ks = find_ms_kernels(g.ms_cfg.block, g.ms_cfg.edges_per_thread);
     if (!ks) fail(GB_EINVAL, "unsupported batch block / edges_per_thread combination");
     DeviceGuard guard(device);
     ensure_ms_workspace();
 
     MsArgs a;
-    a.offsets = offsets;
-    a.cols = cols;
+    a.offsets = g.offsets;
+    a.cols = g.cols;
     a.seen = ms.seen;
     a.next = ms.next;
     a.levels = d_levels;
@@ -407,23 +440,21 @@ void DeviceGraph::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, 
     a.q_fm = ms.q_fm;
     a.tile_start = ms.tile_start;
     a.ctl = ms.ctl;
-    a.lb[0] = ms.lb[0];
-    a.lb[1] = ms.lb[1];
     a.n = n;
+    a.m = m;
     a.k = k;
     a.nchunks = static_cast<uint32_t>(ms.nchunks);
-    a.tile_edges = static_cast<uint32_t>(ms_cfg.block * ms_cfg.edges_per_thread);
-    a.spin_limit = spin_limit_ticks;
+    a.tile_edges = static_cast<uint32_t>(g.ms_cfg.block * g.ms_cfg.edges_per_thread);
 
     int occ = 0;
     GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(ks->expand),
-                                                       ms_cfg.block, 0));
+                                                       g.ms_cfg.block, 0));
     if (occ < 1) fail(GB_EDEVICE, "batch kernel cannot be resident on a CU");
-    const int waves_per_block = (ms_cfg.block + 63) / 64;
+    const int waves_per_block = (g.ms_cfg.block + 63) / 64;
     const int per_cu = std::max(1, std::min(occ, 16 / waves_per_block));
-    const uint32_t grid = static_cast<uint32_t>(per_cu * num_cus);
-    const uint32_t ugrid = static_cast<uint32_t>(std::min<int64_t>(ms.nchunks, int64_t(num_cus) * 4));
-    const uint32_t cgrid = static_cast<uint32_t>(std::min<int64_t>((n * k / 4 + 255) / 256 + 1, int64_t(num_cus) * 8));
+    const uint32_t grid = static_cast<uint32_t>(per_cu * g.num_cus);
+    const uint32_t ugrid = static_cast<uint32_t>(std::min<int64_t>(ms.nchunks, int64_t(g.num_cus) * 4));
+    const uint32_t cgrid = static_cast<uint32_t>(std::min<int64_t>((n * k / 4 + 255) / 256 + 1, int64_t(g.num_cus) * 8));
 
     std::memcpy(ms.h_sources, h_sources, sizeof(int32_t) * k);
     auto launch_prologue = [&]() {
@@ -434,29 +465,29 @@ void DeviceGraph::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, 
     int L = 0;
     auto launch_level = [&]() {
         hipLaunchKernelGGL(ms_update_compact_kernel, dim3(ugrid), dim3(kUcBlock), 0, stream, a, L);
-        hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(ms_cfg.block), 0, stream, a, L);
+        hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(g.ms_cfg.block), 0, stream, a, L);
         ++L;
     };
     auto check_done = [&]() {
         GG_HIP(hipMemcpyAsync(ms.h_ctl, ms.ctl, sizeof(unsigned long long) * kCtlWords, hipMemcpyDeviceToHost,
                               stream));
         GG_HIP(hipStreamSynchronize(stream));
-        return ms.h_ctl[MS_DONE] != 0;
+        return ms.h_ctl[MS_DONE] != 0 || ms.h_ctl[MS_ERR] != 0;
     };
 
     const int depth_hint = std::max(ms.last_depth + 1, 4);
     GG_HIP(hipEventRecord(ev_start, stream));
-    if (ms_cfg.use_graph) {
+    if (g.ms_cfg.use_graph) {
         // One captured sequence: sources upload, clears, depth_hint levels.
         // Captured arguments are fixed, so the graph is rebuilt when k, the
         // output buffer, the kernel shape or the depth changes.
         const bool reuse = ms.graph_exec && ms.graph_k == k && ms.graph_levels == d_levels &&
-                           ms.graph_depth == depth_hint && ms.graph_block == ms_cfg.block &&
-                           ms.graph_kpt == ms_cfg.edges_per_thread;
+                           ms.graph_depth == depth_hint && ms.graph_block == g.ms_cfg.block &&
+                           ms.graph_kpt == g.ms_cfg.edges_per_thread;
         if (!reuse) {
             if (ms.graph_exec) (void)hipGraphExecDestroy(ms.graph_exec);
             ms.graph_exec = nullptr;
-            hipGraph_t g = nullptr;
+            hipGraph_t captured = nullptr;
             GG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             try {
                 launch_prologue();
@@ -467,15 +498,15 @@ void DeviceGraph::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, 
                 if (dummy) (void)hipGraphDestroy(dummy);
                 throw;
             }
-            GG_HIP(hipStreamEndCapture(stream, &g));
-            hipError_t e = hipGraphInstantiate(&ms.graph_exec, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
+            GG_HIP(hipStreamEndCapture(stream, &captured));
+            hipError_t e = hipGraphInstantiate(&ms.graph_exec, captured, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(captured);
             GG_HIP(e);
             ms.graph_k = k;
             ms.graph_levels = d_levels;
             ms.graph_depth = depth_hint;
-            ms.graph_block = ms_cfg.block;
-            ms.graph_kpt = ms_cfg.edges_per_thread;
+            ms.graph_block = g.ms_cfg.block;
+            ms.graph_kpt = g.ms_cfg.edges_per_thread;
             GG_HIP(hipEventRecord(ev_start, stream));
         }
         GG_HIP(hipGraphLaunch(ms.graph_exec, stream));
@@ -494,7 +525,8 @@ void DeviceGraph::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, 
     GG_HIP(hipEventRecord(ev_stop, stream));
     GG_HIP(hipEventSynchronize(ev_stop));
     if (ms.h_ctl[MS_ERR] & MS_ERR_TILE) fail(GB_EDEVICE, "internal error: inconsistent frontier tile index");
-    if (ms.h_ctl[MS_ERR] & MS_ERR_LOOKBACK) fail(GB_EDEVICE, "frontier compaction look-back timed out");
+    if (ms.h_ctl[MS_ERR] & MS_ERR_RANGE)
+        fail(GB_EDEVICE, "internal error: frontier entry or column index out of range");
     ms.last_depth = static_cast<int>(ms.h_ctl[MS_MAXLEV]) + 1;
     if (st) {
         float ms_elapsed = 0.f;
@@ -517,7 +549,7 @@ void DeviceGraph::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, 
     }
 }
 
-void DeviceGraph::bfs_batch_to_host(const int32_t* h_sources, int k, int32_t* h_levels, gb_batch_stats* st) {
+void Runner::bfs_batch_to_host(const int32_t* h_sources, int k, int32_t* h_levels, gb_batch_stats* st) {
     if (!h_levels) fail(GB_EINVAL, "levels buffer is null");
     if (k < 1 || k > 64) fail(GB_EINVAL, "a batch holds 1 to 64 sources");
     DeviceGuard guard(device);

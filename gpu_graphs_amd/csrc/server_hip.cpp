@@ -1,0 +1,254 @@
+// HIP executor of the query server and its C ABI (gb_server_*).
+//
+// Lane i owns one Runner (stream + workspaces) per graph it has served, so
+// batches on different lanes -- on the same graph or on different graphs --
+// run concurrently on the device.  Result buffers are plain device
+// allocations; the server pools and budgets them.
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "device_graph.hpp"
+#include "handles.hpp"
+#include "server.hpp"
+
+namespace gg {
+
+class HipExecutor : public Executor {
+   public:
+    HipExecutor(int device, int lanes) : device_(device), runners_(lanes) {
+        DeviceGuard guard(device_);
+        GG_HIP(hipStreamCreateWithFlags(&fetch_stream_, hipStreamNonBlocking));
+    }
+    ~HipExecutor() override {
+        (void)hipSetDevice(device_);
+        runners_.clear();
+        if (fetch_stream_) (void)hipStreamDestroy(fetch_stream_);
+    }
+
+    int add(DeviceGraph* g) {
+        if (g->device != device_) fail(GB_EINVAL, "graph lives on another device than the server");
+        std::lock_guard<std::mutex> lk(mu_);
+        graphs_.push_back(g);
+        return static_cast<int>(graphs_.size()) - 1;
+    }
+
+    int64_t vertices(int gid) override { return graph(gid)->n; }
+
+    size_t workspace_bytes(int gid, bool plain) override {
+        DeviceGraph* g = graph(gid);
+        if (plain) {  // single-source workspace: two queues, bitmaps, tile index, staging
+            return size_t(g->n) * (2 * (8 + 4 + 4) + 8) + size_t(g->m / 64 + 64) * 8 + (size_t(g->n) / 4);
+        }
+        return batch_bytes_estimate(g->n, g->m, 64, false);
+    }
+
+    void* alloc_result(size_t bytes) override {
+        DeviceGuard guard(device_);
+        void* p = nullptr;
+        GG_HIP(hipMalloc(&p, bytes ? bytes : 1));
+        return p;
+    }
+
+    // Called from the scheduler thread and destructors: never throws, also
+    // after a device error has made every HIP call fail.
+    void free_result(void* p) noexcept override {
+        if (!p) return;
+        int prev = -1;
+        const bool restore = hipGetDevice(&prev) == hipSuccess && prev != device_;
+        if (restore) (void)hipSetDevice(device_);
+        (void)hipFree(p);
+        if (restore) (void)hipSetDevice(prev);
+    }
+
+    void run(int lane, int gid, const int32_t* sources, int k, int32_t* result, bool plain) override {
+        DeviceGraph* g = graph(gid);
+        std::unique_ptr<Runner>& r = runners_[lane][gid];  // only this lane's thread touches runners_[lane]
+        if (!r) r = std::make_unique<Runner>(*g);
+        if (plain) {
+            for (int i = 0; i < k; ++i) r->bfs(sources[i], result + int64_t(i) * g->n, nullptr, nullptr);
+        } else {
+            r->bfs_batch(sources, k, result, nullptr);
+        }
+    }
+
+    void fetch(const int32_t* row, int32_t* host, int64_t n) override {
+        DeviceGuard guard(device_);
+        std::lock_guard<std::mutex> lk(fetch_mu_);
+        GG_HIP(hipMemcpyAsync(host, row, sizeof(int32_t) * n, hipMemcpyDeviceToHost, fetch_stream_));
+        GG_HIP(hipStreamSynchronize(fetch_stream_));
+    }
+
+    int device() const { return device_; }
+
+   private:
+    DeviceGraph* graph(int gid) {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (gid < 0 || gid >= static_cast<int>(graphs_.size())) fail(GB_EINVAL, "unknown graph id");
+        return graphs_[gid];
+    }
+
+    int device_;
+    std::mutex mu_, fetch_mu_;
+    std::vector<DeviceGraph*> graphs_;
+    std::vector<std::map<int, std::unique_ptr<Runner>>> runners_;
+    hipStream_t fetch_stream_ = nullptr;
+};
+
+}  // namespace gg
+
+struct gb_server {
+    gg::HipExecutor* exec = nullptr;  // owned by srv
+    std::unique_ptr<gg::Server> srv;
+};
+
+namespace {
+
+template <class F>
+int guarded(F&& fn) {
+    try {
+        fn();
+        return GB_OK;
+    } catch (const gg::Error& e) {
+        gg::set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        gg::set_last_error("host allocation failed");
+        return GB_ENOMEM;
+    } catch (const std::exception& e) {
+        gg::set_last_error(e.what());
+        return GB_EINVAL;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int gb_server_create(const gb_server_cfg* cfg, gb_server** out) {
+    return guarded([&] {
+        if (!cfg || !out) gg::fail(GB_EINVAL, "null argument");
+        *out = nullptr;
+        gg::ServerConfig sc;
+        sc.max_batch = cfg->max_batch > 0 ? cfg->max_batch : 64;
+        sc.lanes = cfg->lanes > 0 ? cfg->lanes : 4;
+        sc.plain = cfg->plain != 0;
+        sc.queue_capacity = cfg->queue_capacity > 0 ? cfg->queue_capacity : (int64_t(1) << 20);
+        int64_t budget = cfg->memory_budget;
+        if (budget <= 0) {  // default: half of the device memory free now
+            gg::DeviceGuard guard(cfg->device);
+            size_t free_b = 0, total_b = 0;
+            GG_HIP(hipMemGetInfo(&free_b, &total_b));
+            budget = static_cast<int64_t>(free_b / 2);
+        }
+        sc.memory_budget = budget;
+        auto h = std::make_unique<gb_server>();
+        auto exec = std::make_unique<gg::HipExecutor>(cfg->device, sc.lanes);
+        h->exec = exec.get();
+        h->srv = std::make_unique<gg::Server>(sc, std::move(exec));
+        *out = h.release();
+    });
+}
+
+int gb_server_add_graph(gb_server* s, gb_dgraph* dg, int* gid) {
+    return guarded([&] {
+        if (!s || !dg || !gid) gg::fail(GB_EINVAL, "null argument");
+        const int id = s->exec->add(dg->d.get());
+        *gid = s->srv->add_graph(id);
+    });
+}
+
+int gb_server_submit(gb_server* s, int gid, int32_t source, uint64_t* ticket) {
+    return guarded([&] {
+        if (!s || !ticket) gg::fail(GB_EINVAL, "null argument");
+        *ticket = s->srv->submit(gid, source);
+    });
+}
+
+static void to_latency(const gg::QueryLatency& q, gb_latency* lat) {
+    if (!lat) return;
+    lat->queue_ms = q.queue_ms;
+    lat->run_ms = q.run_ms;
+    lat->total_ms = q.total_ms;
+    lat->batch_size = q.batch_size;
+}
+
+int gb_server_poll(gb_server* s, uint64_t ticket, int32_t* levels, gb_latency* lat) {
+    int st = GB_OK;
+    int rc = guarded([&] {
+        if (!s) gg::fail(GB_EINVAL, "null server");
+        gg::QueryLatency q;
+        st = s->srv->poll(ticket, levels, &q, nullptr);
+        to_latency(q, lat);
+    });
+    return rc != GB_OK ? rc : st;
+}
+
+int gb_server_wait(gb_server* s, uint64_t ticket, int timeout_ms, int32_t* levels, gb_latency* lat) {
+    int st = GB_OK;
+    int rc = guarded([&] {
+        if (!s) gg::fail(GB_EINVAL, "null server");
+        gg::QueryLatency q;
+        st = s->srv->wait(ticket, timeout_ms, levels, &q);
+        to_latency(q, lat);
+    });
+    return rc != GB_OK ? rc : st;
+}
+
+int gb_server_result_device(gb_server* s, uint64_t ticket, const int32_t** d_levels) {
+    int st = GB_OK;
+    int rc = guarded([&] {
+        if (!s || !d_levels) gg::fail(GB_EINVAL, "null argument");
+        st = s->srv->poll(ticket, nullptr, nullptr, d_levels);
+    });
+    return rc != GB_OK ? rc : st;
+}
+
+int gb_server_release(gb_server* s, uint64_t ticket) {
+    return guarded([&] {
+        if (!s) gg::fail(GB_EINVAL, "null server");
+        s->srv->release(ticket);
+    });
+}
+
+int gb_server_drain(gb_server* s, int timeout_ms) {
+    int st = GB_OK;
+    int rc = guarded([&] {
+        if (!s) gg::fail(GB_EINVAL, "null server");
+        st = s->srv->drain(timeout_ms) ? GB_OK : GB_EAGAIN;
+    });
+    return rc != GB_OK ? rc : st;
+}
+
+int gb_server_get_stats(gb_server* s, gb_server_stats* out) {
+    return guarded([&] {
+        if (!s || !out) gg::fail(GB_EINVAL, "null argument");
+        const gg::ServerStats st = s->srv->stats();
+        out->submitted = st.submitted;
+        out->completed = st.completed;
+        out->failed = st.failed;
+        out->rejected = st.rejected;
+        out->batches = st.batches;
+        out->mean_batch = st.mean_batch;
+        out->p50_ms = st.p50_ms;
+        out->p99_ms = st.p99_ms;
+        out->max_ms = st.max_ms;
+        out->qps = st.qps;
+        out->memory_used = st.memory_used;
+        out->memory_budget = st.memory_budget;
+    });
+}
+
+int gb_server_reset_stats(gb_server* s) {
+    return guarded([&] {
+        if (!s) gg::fail(GB_EINVAL, "null server");
+        s->srv->reset_stats();
+    });
+}
+
+void gb_server_free(gb_server* s) { delete s; }
+
+}  // extern "C"

@@ -172,6 +172,57 @@ int  gb_bfs_batch_device(gb_dgraph* dg, const int32_t* sources, int k, int32_t* 
 int  gb_batch_config(gb_dgraph* dg, int block, int edges_per_thread, int use_graph);
 int  gb_batch_get_config(const gb_dgraph* dg, int* block, int* edges_per_thread, int* use_graph);
 
+/* ---- continuous-batching query server (no reference counterpart: the
+ *      reference answers one (graph, source) per process run, 1D Partition/
+ *      main.cu:10-82).  Queries on any added graph are batched (up to
+ *      max_batch per graph) onto `lanes` concurrent GPU streams as lanes and
+ *      device memory free up; admission is by a device-memory budget that
+ *      covers lane workspaces and result buffers.  Results stay in device
+ *      memory until gb_server_release.  The gb_dgraph handles must outlive
+ *      the server.  All gb_server_* calls are thread-safe. -------------- */
+typedef struct gb_server_cfg {
+    int     device;
+    int     max_batch;       /* 1..64 (0 = 64)                                    */
+    int     lanes;           /* concurrent batches, 1..16 (0 = 4)                 */
+    int     plain;           /* 1: one single-source BFS per query, no batching   */
+    int64_t memory_budget;   /* bytes (0 = half of the free device memory)        */
+    int64_t queue_capacity;  /* queued queries before submit returns GB_EBUSY (0 = 2^20) */
+} gb_server_cfg;
+
+typedef struct gb_latency {
+    double queue_ms;   /* arrival -> batch start */
+    double run_ms;     /* batch start -> done    */
+    double total_ms;   /* arrival -> done        */
+    int    batch_size;
+} gb_latency;
+
+typedef struct gb_server_stats {
+    int64_t submitted, completed, failed, rejected, batches;
+    double  mean_batch;
+    double  p50_ms, p99_ms, max_ms;   /* arrival -> done of completed queries */
+    double  qps;                      /* completed / (last done - first arrival) */
+    int64_t memory_used, memory_budget;
+} gb_server_stats;
+
+int  gb_server_create(const gb_server_cfg* cfg, gb_server** out);
+int  gb_server_add_graph(gb_server* srv, gb_dgraph* dg, int* gid);
+/* GB_EBUSY when the queue is full, GB_ERANGE for a bad source. */
+int  gb_server_submit(gb_server* srv, int gid, int32_t source, uint64_t* ticket);
+/* GB_OK when done (levels, if not null, receives n int32 on the host),
+ * GB_EAGAIN while pending, or the query's failure status (GB_ENOMEM when its
+ * batch could never fit the budget, GB_EDEVICE on a device error). */
+int  gb_server_poll(gb_server* srv, uint64_t ticket, int32_t* levels, gb_latency* lat);
+int  gb_server_wait(gb_server* srv, uint64_t ticket, int timeout_ms, int32_t* levels, gb_latency* lat);
+/* Device pointer of a finished query's n levels (valid until release). */
+int  gb_server_result_device(gb_server* srv, uint64_t ticket, const int32_t** d_levels);
+/* Frees a finished query's result (or cancels a queued one). */
+int  gb_server_release(gb_server* srv, uint64_t ticket);
+/* Waits until nothing is queued or running (GB_EAGAIN on timeout). */
+int  gb_server_drain(gb_server* srv, int timeout_ms);
+int  gb_server_get_stats(gb_server* srv, gb_server_stats* st);
+int  gb_server_reset_stats(gb_server* srv);
+void gb_server_free(gb_server* srv);
+
 /* ---- device-wide exclusive scan (replaces PrefixSum::sum_scan_blelloch,
  *      1D Partition/src/prefixSum.cu:139-224).  d_out[i] = sum(d_in[0..i)).
  *      In-place (d_out == d_in) is allowed.  No allocation per call. ----- */

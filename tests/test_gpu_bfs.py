@@ -113,6 +113,42 @@ def test_rmat_levels_match_oracle(gpu, scale, seed):
         assert st["m_reached"] == ost["m_reached"]
 
 
+def test_concurrent_handles_on_large_graph(gpu):
+    # Four threads, four handles of one 8.4M-vertex graph (2048 compaction
+    # chunks), BFS at the same time on four streams: the level kernels share
+    # the CUs, and two handles rebuild every level's queue by compaction.
+    import threading
+
+    G = gg.Graph.rmat(23, edge_factor=2, seed=21)
+    deg = np.diff(G.offsets)
+    rng = np.random.default_rng(21)
+    srcs = [int(s) for s in rng.choice(np.nonzero(deg)[0], 3, replace=False)]
+    want = [O.bfs(G.offsets, G.cols, s)[0] for s in srcs]
+    dgs = [gg.DeviceGraph(G, gpu) for _ in range(4)]
+    dgs[0].config(dense_edges=1)
+    dgs[1].config(dense_edges=1)
+    log(f"rmat23 ef2 n={G.n} m={G.m}, 4 threads x {len(srcs)} sources")
+    results, errors = {}, []
+
+    def worker(i):
+        try:
+            for rep in range(2):
+                for j, s in enumerate(srcs):
+                    results[(i, rep, j)] = dgs[i].bfs(s)[0]
+        except Exception as e:  # noqa: BLE001 - reported below
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(i,)) for i in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    assert len(results) == 4 * 2 * len(srcs)
+    for (i, rep, j), lv in results.items():
+        np.testing.assert_array_equal(lv, want[j], err_msg=f"handle {i} rep {rep} source {srcs[j]}")
+
+
 def test_isolated_and_degree_zero_sources(gpu):
     G = gg.Graph.rmat(12, seed=3)
     deg = np.diff(G.offsets)
