@@ -21,13 +21,15 @@ import ctypes as C
 import numpy as np
 
 from . import _ffi
-from ._ffi import (GB_AUTO_N, GB_DROP_SELF_LOOPS, GB_KEEP_ORDER, GB_MODE_ASYNC, GB_MODE_LEVEL_LOOP, GB_ONE_BASED,
-                   GB_SYMMETRIZE, BatchStats, GBError, Stats, check, lib)
+from ._ffi import (GB_AUTO_N, GB_DROP_SELF_LOOPS, GB_EAGAIN, GB_EBUSY, GB_EDEVICE, GB_EINVAL, GB_EIO, GB_ENOMEM,
+                   GB_ERANGE, GB_KEEP_ORDER, GB_MODE_ASYNC, GB_MODE_LEVEL_LOOP, GB_OK, GB_ONE_BASED, GB_SYMMETRIZE,
+                   BatchStats, GBError, Stats, check, lib)
 
 __all__ = [
     "Graph", "DeviceGraph", "DeviceArray", "Server", "csr", "gpuBFS", "GBError", "Stats", "device_count",
     "exclusive_scan", "GB_AUTO_N", "GB_ONE_BASED", "GB_SYMMETRIZE", "GB_KEEP_ORDER", "GB_DROP_SELF_LOOPS",
-    "GB_MODE_ASYNC", "GB_MODE_LEVEL_LOOP",
+    "GB_MODE_ASYNC", "GB_MODE_LEVEL_LOOP", "GB_OK", "GB_EINVAL", "GB_EIO", "GB_ENOMEM", "GB_EDEVICE", "GB_ERANGE",
+    "GB_EBUSY", "GB_EAGAIN",
 ]
 
 
@@ -180,6 +182,16 @@ class DeviceArray:
         check(lib().gb_memcpy_dtoh(self.device, _ptr(out), self.ptr, out.nbytes))
         return out
 
+    def numpy_slice(self, start, count):
+        """Copies elements [start, start + count) to a new host array."""
+        start, count = int(start), int(count)
+        if start < 0 or count < 0 or start + count > self.count:
+            raise ValueError("slice outside the array")
+        out = np.empty(count, dtype=self.dtype)
+        src = C.c_void_p(self.ptr.value + start * self.dtype.itemsize)
+        check(lib().gb_memcpy_dtoh(self.device, _ptr(out), src, out.nbytes))
+        return out
+
     def free(self):
         if self.ptr is not None and self.ptr.value:
             lib().gb_dfree(self.device, self.ptr)
@@ -234,13 +246,14 @@ class DeviceGraph:
                                   C.byref(st)))
         return st.as_dict()
 
-    def batch_config(self, block=256, edges_per_thread=2, use_graph=True):
-        check(lib().gb_batch_config(self._h, block, edges_per_thread, int(use_graph)))
+    def batch_config(self, block=256, edges_per_thread=2, use_graph=True, sparse_edges=0):
+        check(lib().gb_batch_config(self._h, block, edges_per_thread, int(use_graph), int(sparse_edges)))
 
     def get_batch_config(self):
-        b, k, g = C.c_int(), C.c_int(), C.c_int()
-        check(lib().gb_batch_get_config(self._h, C.byref(b), C.byref(k), C.byref(g)))
-        return {"block": b.value, "edges_per_thread": k.value, "use_graph": bool(g.value)}
+        b, k, g, s = C.c_int(), C.c_int(), C.c_int(), C.c_int64()
+        check(lib().gb_batch_get_config(self._h, C.byref(b), C.byref(k), C.byref(g), C.byref(s)))
+        return {"block": b.value, "edges_per_thread": k.value, "use_graph": bool(g.value),
+                "sparse_edges": s.value}
 
     def bfs_batch(self, sources, stats=True):
         """Multi-source BFS of up to 64 sources: returns (levels[k, n], stats or None)."""

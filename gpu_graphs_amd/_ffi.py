@@ -134,8 +134,9 @@ SIGNATURES = {
                                     C.POINTER(C.c_int), i64p]),
     "gb_bfs_batch": (C.c_int, [P, P, C.c_int, P, C.POINTER(BatchStats)]),
     "gb_bfs_batch_device": (C.c_int, [P, P, C.c_int, P, C.POINTER(BatchStats)]),
-    "gb_batch_config": (C.c_int, [P, C.c_int, C.c_int, C.c_int]),
-    "gb_batch_get_config": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "gb_batch_config": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int64]),
+    "gb_batch_get_config": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int64)]),
     "gb_server_create": (C.c_int, [C.POINTER(ServerCfg), PP]),
     "gb_server_add_graph": (C.c_int, [P, P, C.POINTER(C.c_int)]),
     "gb_server_submit": (C.c_int, [P, C.c_int, C.c_int32, C.POINTER(C.c_uint64)]),

@@ -692,9 +692,11 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     if (fin[CTL_ERR] & ERR_RANGE) fail(GB_EDEVICE, "internal error: frontier entry or column index out of range");
     if (!fin[CTL_DONE]) fail(GB_EDEVICE, "internal error: BFS did not reach an empty frontier");
     last_num_levels = static_cast<int>(fin[CTL_MAXLEV]) + 1;
+    float ms = 0.f;
+    GG_HIP(hipEventElapsedTime(&ms, ev_start, ev_stop));
+    last_ms = ms;
     if (st) {
-        float ms = 0.f, kms = 0.f;
-        GG_HIP(hipEventElapsedTime(&ms, ev_start, ev_stop));
+        float kms = 0.f;
         GG_HIP(hipEventElapsedTime(&kms, ev_kernel, ev_stop));
         st->ms = ms;
         st->kernel_ms = kms;

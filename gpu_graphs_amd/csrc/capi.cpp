@@ -290,22 +290,28 @@ int gb_bfs_batch_device(gb_dgraph* dg, const int32_t* sources, int k, int32_t* d
     });
 }
 
-int gb_batch_config(gb_dgraph* dg, int block, int edges_per_thread, int use_graph) {
+int gb_batch_config(gb_dgraph* dg, int block, int edges_per_thread, int use_graph, int64_t sparse_edges) {
     return guarded([&] {
         need(dg, "null device graph");
         need(gg::ms_config_supported(block, edges_per_thread), "unsupported block / edges_per_thread combination");
+        need(sparse_edges >= 0, "sparse_edges must be >= 0");
         dg->d->ms_cfg.block = block;
         dg->d->ms_cfg.edges_per_thread = edges_per_thread;
         dg->d->ms_cfg.use_graph = use_graph != 0;
+        dg->d->ms_cfg.sparse_edges = sparse_edges;
     });
 }
 
-int gb_batch_get_config(const gb_dgraph* dg, int* block, int* edges_per_thread, int* use_graph) {
+int gb_batch_get_config(const gb_dgraph* dg, int* block, int* edges_per_thread, int* use_graph,
+                        int64_t* sparse_edges) {
     return guarded([&] {
         need(dg, "null device graph");
         if (block) *block = dg->d->ms_cfg.block;
         if (edges_per_thread) *edges_per_thread = dg->d->ms_cfg.edges_per_thread;
         if (use_graph) *use_graph = dg->d->ms_cfg.use_graph ? 1 : 0;
+        if (sparse_edges)
+            *sparse_edges = dg->d->ms_cfg.sparse_edges > 0 ? dg->d->ms_cfg.sparse_edges
+                                                           : dg->d->default_ms_sparse_edges();
     });
 }
 

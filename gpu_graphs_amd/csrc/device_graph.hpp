@@ -70,7 +70,17 @@ struct BfsWorkspace {
 struct MsConfig {
     int block = 256;
     int edges_per_thread = 2;
-    bool use_graph = true;  // replay one captured hipGraph per batch
+    bool use_graph = true;      // replay one captured hipGraph per batch
+    int64_t sparse_edges = 0;   // levels with at most this many frontier edges pass a touched list
+};
+
+// Per-batch parameters, read by the kernels from device memory so that one
+// captured sequence serves every batch (any k, any output buffer).
+struct MsParams {
+    int32_t sources[64];
+    int32_t k;
+    int32_t pad;
+    int32_t* levels;  // k rows of n int32
 };
 
 // Device workspace of the batched multi-source BFS.
@@ -82,10 +92,11 @@ struct MsWorkspace {
     int32_t* q_v = nullptr;
     unsigned long long* q_fm = nullptr;   // their frontier masks
     uint32_t* tile_start = nullptr;
+    int32_t* touched = nullptr;           // n: vertices first reached in a sparse level
     unsigned long long* ctl = nullptr;
-    int32_t* sources = nullptr;           // 64, device
-    int32_t* h_sources = nullptr;         // 64, pinned (read by the captured copy)
-    unsigned long long* h_ctl = nullptr;  // pinned snapshot of ctl
+    MsParams* params = nullptr;           // device
+    MsParams* h_params = nullptr;         // pinned (read by the captured copy)
+    unsigned long long* h_ctl = nullptr;  // pinned: 2 snapshots of ctl
     unsigned long long* partial = nullptr;  // per-workgroup statistics
     int32_t* levels = nullptr;            // staging for host-pointer calls
     size_t levels_bytes = 0;
@@ -95,8 +106,8 @@ struct MsWorkspace {
     int last_depth = 0;
     size_t bytes = 0;
     hipGraphExec_t graph_exec = nullptr;  // captured batch sequence
-    int graph_k = 0, graph_depth = 0, graph_block = 0, graph_kpt = 0;
-    int32_t* graph_levels = nullptr;
+    int graph_depth = 0, graph_block = 0, graph_kpt = 0;
+    int64_t graph_sparse = -1;
 };
 
 struct DeviceGraph;
@@ -131,6 +142,7 @@ struct Runner {
     // Batched multi-source BFS of k <= 64 sources (host array); levels is a
     // device buffer of k rows of n int32.
     void bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_batch_stats* st);
+    double last_ms = 0;  // device time of the last bfs / bfs_batch call
     void bfs_batch_to_host(const int32_t* h_sources, int k, int32_t* h_levels, gb_batch_stats* st);
     // Allocates the batch workspace now (the server charges it to its budget).
     void prepare_batch();
@@ -175,6 +187,7 @@ struct DeviceGraph {
 
     size_t memory_bytes() const { return graph_bytes + (runner ? runner->bytes() : 0); }
     int64_t default_dense_edges() const { return std::max<int64_t>(int64_t(1) << 19, m / 256); }
+    int64_t default_ms_sparse_edges() const { return std::max<int64_t>(int64_t(1) << 16, n / 16); }
 };
 
 bool bfs_config_supported(int block, int k);

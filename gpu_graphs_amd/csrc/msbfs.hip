@@ -14,13 +14,19 @@
 //       4096-vertex chunk and one packed atomicAdd per chunk reserves their
 //       slots and edge range, as in the single-source compaction.
 //   expand(L): edge-parallel tiles over the queue; for edge (u, v) the bits
-//       mask(u) & ~seen(v) are OR-ed into next(v) with a 64-bit atomicOr whose
-//       result is not needed, so the atomics do not stall the wave.
-// Level writes: lanes of a wave hold 64 consecutive vertices; the wave loops
-// over the union of their new bits and stores level[s][v] with the lanes that
-// have bit s, so each store instruction covers one 256-byte run of row s.
-// The whole sequence (source upload, clears, levels) can be captured once
-// into a hipGraph and replayed per batch.
+//       mask(u) & ~seen(v) are OR-ed into next(v) with a 64-bit atomicOr.
+// Dense levels post the atomicOr (its result is not needed, so the wave does
+// not stall) and the next update_compact scans all n vertices.  Sparse levels
+// (few frontier edges) use the returned old value: the edge that turns next(v)
+// from 0 to non-zero appends v to a touched list, and the next update_compact
+// visits only the listed vertices -- a level then costs O(frontier), which is
+// what deep, thin graphs (road networks) need.
+// Level writes: the wave loops over the union of its lanes' new bits and
+// stores level[s][v] with the lanes that have bit s; on a scan of consecutive
+// vertices each store instruction covers one 256-byte run of row s.
+// Sources, k and the output buffer are read from device memory (MsParams), so
+// the whole sequence (parameter upload, clears, levels) is captured once into
+// a hipGraph per kernel shape and replayed for every batch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,33 +41,44 @@ namespace gg {
 
 namespace {
 
-// ctl[] layout (64-bit words).  Level L's queue counter is cnt[L & 1]:
-// update_compact(L) builds it up and zeroes the other parity for level L+1.
-enum : int { MS_CNT0 = 0, MS_DONE = 2, MS_ERR = 3, MS_MAXLEV = 4, MS_WORDS = 16 };
+// ctl[] layout (64-bit words).  Level L's queue counter is cnt[L & 1]
+// (count << 32 | edges); tcnt[L & 1] is the length of the touched list that
+// feeds update_compact(L) and tvalid[L & 1] says whether that list is in use.
+enum : int {
+    MS_CNT0 = 0,
+    MS_DONE = 2,
+    MS_ERR = 3,
+    MS_MAXLEV = 4,
+    MS_TCNT0 = 5,
+    MS_TVALID0 = 7,
+    MS_WORDS = 16
+};
 enum : unsigned long long { MS_ERR_TILE = 1, MS_ERR_RANGE = 2 };
 
 constexpr int kUcBlock = 256;
 constexpr int kUcRounds = 16;
 constexpr int kUcChunk = kUcBlock * kUcRounds;
+constexpr int kMaxCapturedLevels = 32;  // deeper searches continue with direct launches
+constexpr int kMaxLevelBatch = 128;     // levels enqueued between done checks
 
 struct MsArgs {
     const int64_t* __restrict__ offsets;
     const int32_t* __restrict__ cols;
     unsigned long long* seen;
     unsigned long long* next;
-    int32_t* levels;             // [k][n]
-    const int32_t* sources;      // k, device
+    const MsParams* p;           // sources, k, output rows (device memory)
     int64_t* q_rb;
     uint32_t* q_eoff;
     int32_t* q_v;
     unsigned long long* q_fm;    // frontier mask of each queue entry
     uint32_t* tile_start;
+    int32_t* touched;
     unsigned long long* ctl;
     int64_t n;
     int64_t m;
-    int k;
     uint32_t nchunks;
     uint32_t tile_edges;
+    uint32_t sparse_edges;
 };
 
 template <int BLOCK, int K>
@@ -80,49 +97,67 @@ __global__ void ms_clear_kernel(MsArgs a) {
         a.seen[i] = 0;
         a.next[i] = 0;
     }
-    const int64_t total = a.n * a.k;
-    int4* lv4 = reinterpret_cast<int4*>(a.levels);
+    int32_t* levels = a.p->levels;
+    const int64_t total = a.n * a.p->k;
+    int4* lv4 = reinterpret_cast<int4*>(levels);
     // 16-byte stores only on a 16-byte aligned buffer (caller-owned memory).
-    const int64_t n4 = (reinterpret_cast<uintptr_t>(a.levels) & 15) ? 0 : total / 4;
+    const int64_t n4 = (reinterpret_cast<uintptr_t>(levels) & 15) ? 0 : total / 4;
     const int4 minus1 = make_int4(-1, -1, -1, -1);
     for (int64_t i = tid0; i < n4; i += stride) lv4[i] = minus1;
-    for (int64_t i = n4 * 4 + tid0; i < total; i += stride) a.levels[i] = -1;
+    for (int64_t i = n4 * 4 + tid0; i < total; i += stride) levels[i] = -1;
 }
 
+// Level 0: next(source i) |= bit i; each distinct source once on the touched
+// list of level 0.
 __global__ void ms_seed_kernel(MsArgs a) {
     if (threadIdx.x == 0)
         for (int i = 0; i < MS_WORDS; ++i) a.ctl[i] = 0;
-    if (threadIdx.x < static_cast<uint32_t>(a.k))
-        atomicOr(&a.next[a.sources[threadIdx.x]], 1ull << threadIdx.x);
+    __syncthreads();
+    if (threadIdx.x < static_cast<uint32_t>(a.p->k)) {
+        const int32_t s = a.p->sources[threadIdx.x];
+        const unsigned long long old = atomicOr(&a.next[s], 1ull << threadIdx.x);
+        if (old == 0) a.touched[atomicAdd(&a.ctl[MS_TCNT0], 1ull)] = s;
+    }
+    if (threadIdx.x == 0) a.ctl[MS_TVALID0] = 1;
 }
 
 // Stores level L for the new bits of this lane's vertex: the wave walks the
 // union of its lanes' bits, one predicated store per source present.
-__device__ __forceinline__ void write_levels(const MsArgs& a, int64_t v, unsigned long long nv, int L) {
+__device__ __forceinline__ void write_levels(int32_t* levels, int64_t n, int64_t v, unsigned long long nv, int L) {
     unsigned long long any = nv;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) any |= __shfl_xor(any, o, 64);
     while (any) {
         const int s = __ffsll(static_cast<long long>(any)) - 1;
         any &= any - 1;
-        if ((nv >> s) & 1ull) a.levels[int64_t(s) * a.n + v] = L;
+        if ((nv >> s) & 1ull) levels[int64_t(s) * n + v] = L;
     }
 }
 
-__device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c) {
+// Chunk c of update_compact(L): thread t takes slots c*kUcChunk + r*256 + t
+// (r < 16) of either the vertex range (scan) or the touched list.
+__device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, bool list, uint32_t count) {
     __shared__ unsigned long long s_wave[kUcBlock / 64];
     __shared__ unsigned long long s_excl;
     __shared__ uint32_t s_ok;
     const uint32_t tid = threadIdx.x;
-    const int64_t vbase = int64_t(c) * kUcChunk;
-    uint32_t active = 0;  // bit r: vertex vbase + r*256 + tid goes to the queue
+    const int64_t ibase = int64_t(c) * kUcChunk;
+    int32_t* levels = a.p->levels;
+    uint32_t active = 0;  // bit r: the vertex of slot r goes to the queue
     uint32_t kept = 0, edges = 0;
     unsigned long long newbits[kUcRounds];
+    int32_t verts[kUcRounds];
 #pragma unroll
     for (int r = 0; r < kUcRounds; ++r) {
-        const int64_t v = vbase + r * kUcBlock + tid;
+        const int64_t i = ibase + r * kUcBlock + tid;
+        int64_t v = -1;
+        if (list) {
+            if (i < count) v = a.touched[i];
+        } else if (i < a.n) {
+            v = i;
+        }
         unsigned long long nv = 0;
-        if (v < a.n) {
+        if (v >= 0) {
             const unsigned long long nx = a.next[v];
             if (nx) {
                 const unsigned long long sv = a.seen[v];
@@ -132,8 +167,9 @@ __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c) {
             }
         }
         newbits[r] = nv;
+        verts[r] = static_cast<int32_t>(v);
         // every lane of the wave takes part in the shuffles of write_levels
-        if (__ballot(nv != 0)) write_levels(a, v, nv, L);
+        if (__ballot(nv != 0)) write_levels(levels, a.n, v, nv, L);
         if (nv) {
             const uint32_t d = static_cast<uint32_t>(a.offsets[v + 1] - a.offsets[v]);
             if (d) {
@@ -171,7 +207,7 @@ __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c) {
 #pragma unroll
     for (int r = 0; r < kUcRounds; ++r) {
         if ((active >> r) & 1u) {
-            const int64_t v = vbase + r * kUcBlock + tid;
+            const int64_t v = verts[r];
             const int64_t rb = a.offsets[v];
             const uint32_t d = static_cast<uint32_t>(a.offsets[v + 1] - rb);
             a.q_rb[slot] = rb;
@@ -189,10 +225,22 @@ __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c) {
 // the vertices that carry them.
 __global__ __launch_bounds__(kUcBlock) void ms_update_compact_kernel(MsArgs a, int L) {
     if (a.ctl[MS_DONE] | a.ctl[MS_ERR]) return;
-    // cnt[(L+1) & 1] was last read by expand(L-1), which has finished.
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl[MS_CNT0 + ((L + 1) & 1)] = 0;
-    for (uint32_t c = blockIdx.x; c < a.nchunks; c += gridDim.x) {
-        ms_uc_chunk(a, L, c);
+    const int par = L & 1;
+    const bool list = a.ctl[MS_TVALID0 + par] != 0;
+    const unsigned long long count = list ? a.ctl[MS_TCNT0 + par] : 0;
+    // cnt[(L+1) & 1] was last read by expand(L-1) and tcnt[(L+1) & 1] by
+    // update_compact(L-1): both restart from zero for level L+1.
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.ctl[MS_CNT0 + ((L + 1) & 1)] = 0;
+        a.ctl[MS_TCNT0 + ((L + 1) & 1)] = 0;
+    }
+    if (count > static_cast<unsigned long long>(a.n)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.ctl[MS_ERR], MS_ERR_RANGE);
+        return;
+    }
+    const uint32_t nch = list ? static_cast<uint32_t>((count + kUcChunk - 1) / kUcChunk) : a.nchunks;
+    for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        ms_uc_chunk(a, L, c, list, static_cast<uint32_t>(count));
         __syncthreads();
     }
 }
@@ -213,7 +261,11 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(MsArgs a, int L) {
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.ctl[MS_ERR], MS_ERR_RANGE);
         return;
     }
+    const bool sparse = E <= a.sparse_edges;
+    const int npar = (L + 1) & 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl[MS_TVALID0 + npar] = sparse ? 1 : 0;
     const uint32_t ntiles = static_cast<uint32_t>((uint64_t(E) + T - 1) / T);
+    const uint32_t lane = threadIdx.x & 63u;
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint32_t tid = threadIdx.x;
         const uint32_t base = t * T;
@@ -261,11 +313,7 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(MsArgs a, int L) {
             }
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                if (v[k] >= 0 && v[k] >= a.n) {
-                    v[k] = -1;
-                    bad = true;
-                }
-                if (v[k] < -1) {
+                if (v[k] < -1 || v[k] >= a.n) {
                     v[k] = -1;
                     bad = true;
                 }
@@ -274,10 +322,37 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(MsArgs a, int L) {
             unsigned long long sv[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) sv[k] = v[k] >= 0 ? a.seen[v[k]] : ~0ull;
+            if (!sparse) {
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const unsigned long long m = fm[k] & ~sv[k];
-                if (m) __hip_atomic_fetch_or(&a.next[v[k]], m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (int k = 0; k < K; ++k) {
+                    const unsigned long long mk = fm[k] & ~sv[k];
+                    if (mk) __hip_atomic_fetch_or(&a.next[v[k]], mk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            } else {
+                bool first[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const unsigned long long mk = fm[k] & ~sv[k];
+                    first[k] = false;
+                    if (mk)
+                        first[k] = __hip_atomic_fetch_or(&a.next[v[k]], mk, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT) == 0;
+                }
+                // Wave-aggregated appends: one counter atomic per wave and k.
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const uint64_t fb = wave_ballot(first[k]);
+                    if (!fb) continue;
+                    unsigned long long pos = 0;
+                    if (lane == __ffsll(static_cast<long long>(fb)) - 1)
+                        pos = atomicAdd(&a.ctl[MS_TCNT0 + npar], static_cast<unsigned long long>(__popcll(fb)));
+                    pos = __shfl(pos, __ffsll(static_cast<long long>(fb)) - 1, 64);
+                    if (first[k]) {
+                        const unsigned long long slot = pos + rank_below(fb);
+                        if (slot < static_cast<unsigned long long>(a.n)) a.touched[slot] = v[k];
+                        else atomicOr(&a.ctl[MS_ERR], MS_ERR_RANGE);
+                    }
+                }
             }
         }
         __syncthreads();
@@ -334,7 +409,7 @@ __global__ __launch_bounds__(256) void ms_stats_kernel(MsArgs a, unsigned long l
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < a.k; i += blockDim.x) {
+    for (int i = threadIdx.x; i < a.p->k; i += blockDim.x) {
         partial[(int64_t(blockIdx.x) * 64 + i) * 2 + 0] = s_acc[i][0];
         partial[(int64_t(blockIdx.x) * 64 + i) * 2 + 1] = s_acc[i][1];
     }
@@ -348,11 +423,11 @@ bool ms_config_supported(int block, int k) { return find_ms_kernels(block, k) !=
 
 void Runner::free_ms_workspace() {
     if (ms.graph_exec) (void)hipGraphExecDestroy(ms.graph_exec);
-    void* ptrs[] = {ms.seen,       ms.next, ms.q_rb,    ms.q_eoff,  ms.q_v,    ms.q_fm,
-                    ms.tile_start, ms.ctl,  ms.sources, ms.levels, ms.partial};
+    void* ptrs[] = {ms.seen,       ms.next,    ms.q_rb,   ms.q_eoff,  ms.q_v,     ms.q_fm,
+                    ms.tile_start, ms.touched, ms.ctl,    ms.params,  ms.levels,  ms.partial};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
-    if (ms.h_sources) (void)hipHostFree(ms.h_sources);
+    if (ms.h_params) (void)hipHostFree(ms.h_params);
     if (ms.h_ctl) (void)hipHostFree(ms.h_ctl);
     ms = MsWorkspace{};
 }
@@ -387,11 +462,12 @@ void Runner::ensure_ms_workspace() {
         alloc(&ms.q_v, sizeof(int32_t) * n);
         alloc(&ms.q_fm, sizeof(unsigned long long) * n);
         alloc(&ms.tile_start, sizeof(uint32_t) * tile_cap);
+        alloc(&ms.touched, sizeof(int32_t) * n);
         alloc(&ms.ctl, sizeof(unsigned long long) * kCtlWords);
-        alloc(&ms.sources, sizeof(int32_t) * 64);
+        alloc(&ms.params, sizeof(MsParams));
         alloc(&ms.partial, sizeof(unsigned long long) * kStatsBlocks * 64 * 2);
-        GG_HIP(hipHostMalloc(reinterpret_cast<void**>(&ms.h_sources), sizeof(int32_t) * 64, hipHostMallocDefault));
-        GG_HIP(hipHostMalloc(reinterpret_cast<void**>(&ms.h_ctl), sizeof(unsigned long long) * kCtlWords,
+        GG_HIP(hipHostMalloc(reinterpret_cast<void**>(&ms.h_params), sizeof(MsParams), hipHostMallocDefault));
+        GG_HIP(hipHostMalloc(reinterpret_cast<void**>(&ms.h_ctl), sizeof(unsigned long long) * 2 * kCtlWords,
                              hipHostMallocDefault));
     } catch (...) {
         free_ms_workspace();
@@ -409,14 +485,14 @@ void Runner::prepare_batch() {
 }
 
 size_t batch_bytes_estimate(int64_t n_vertices, int64_t m_entries, int k, bool host_levels) {
-    const int64_t nchunks = (n_vertices + kUcChunk - 1) / kUcChunk;
-    size_t b = sizeof(unsigned long long) * n_vertices * 3 + (sizeof(int64_t) + 8) * n_vertices +
-               sizeof(uint32_t) * (m_entries / 256 + 2) + 16 * nchunks + 1024 * 64 * 16;
+    size_t b = sizeof(unsigned long long) * n_vertices * 3 + (sizeof(int64_t) + 12) * n_vertices +
+               sizeof(uint32_t) * (m_entries / 256 + 2) + 1024 * 64 * 16 + 4096;
     if (host_levels) b += sizeof(int32_t) * n_vertices * k;
     return b;
 }
 
 void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_batch_stats* st) {
+    static_assert(kCtlWords >= MS_WORDS, "control block size");
     if (k < 1 || k > 64) fail(GB_EINVAL, "a batch holds 1 to 64 sources");
     if (!h_sources || !d_levels) fail(GB_EINVAL, "null sources or levels");
     for (int i = 0; i < k; ++i)
@@ -427,24 +503,25 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     DeviceGuard guard(device);
     ensure_ms_workspace();
 
+    const int64_t sparse_cfg = g.ms_cfg.sparse_edges > 0 ? g.ms_cfg.sparse_edges : g.default_ms_sparse_edges();
     MsArgs a;
     a.offsets = g.offsets;
     a.cols = g.cols;
     a.seen = ms.seen;
     a.next = ms.next;
-    a.levels = d_levels;
-    a.sources = ms.sources;
+    a.p = ms.params;
     a.q_rb = ms.q_rb;
     a.q_eoff = ms.q_eoff;
     a.q_v = ms.q_v;
     a.q_fm = ms.q_fm;
     a.tile_start = ms.tile_start;
+    a.touched = ms.touched;
     a.ctl = ms.ctl;
     a.n = n;
     a.m = m;
-    a.k = k;
     a.nchunks = static_cast<uint32_t>(ms.nchunks);
     a.tile_edges = static_cast<uint32_t>(g.ms_cfg.block * g.ms_cfg.edges_per_thread);
+    a.sparse_edges = static_cast<uint32_t>(std::min<int64_t>(sparse_cfg, 0xffffffffll));
 
     int occ = 0;
     GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(ks->expand),
@@ -454,11 +531,14 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     const int per_cu = std::max(1, std::min(occ, 16 / waves_per_block));
     const uint32_t grid = static_cast<uint32_t>(per_cu * g.num_cus);
     const uint32_t ugrid = static_cast<uint32_t>(std::min<int64_t>(ms.nchunks, int64_t(g.num_cus) * 4));
-    const uint32_t cgrid = static_cast<uint32_t>(std::min<int64_t>((n * k / 4 + 255) / 256 + 1, int64_t(g.num_cus) * 8));
+    const uint32_t cgrid = static_cast<uint32_t>(std::min<int64_t>((n * 16 + 255) / 256 + 1, int64_t(g.num_cus) * 8));
 
-    std::memcpy(ms.h_sources, h_sources, sizeof(int32_t) * k);
+    std::memset(ms.h_params, 0, sizeof(MsParams));
+    std::memcpy(ms.h_params->sources, h_sources, sizeof(int32_t) * k);
+    ms.h_params->k = k;
+    ms.h_params->levels = d_levels;
     auto launch_prologue = [&]() {
-        GG_HIP(hipMemcpyAsync(ms.sources, ms.h_sources, sizeof(int32_t) * 64, hipMemcpyHostToDevice, stream));
+        GG_HIP(hipMemcpyAsync(ms.params, ms.h_params, sizeof(MsParams), hipMemcpyHostToDevice, stream));
         hipLaunchKernelGGL(ms_clear_kernel, dim3(cgrid), dim3(256), 0, stream, a);
         hipLaunchKernelGGL(ms_seed_kernel, dim3(1), dim3(64), 0, stream, a);
     };
@@ -468,22 +548,15 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
         hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(g.ms_cfg.block), 0, stream, a, L);
         ++L;
     };
-    auto check_done = [&]() {
-        GG_HIP(hipMemcpyAsync(ms.h_ctl, ms.ctl, sizeof(unsigned long long) * kCtlWords, hipMemcpyDeviceToHost,
-                              stream));
-        GG_HIP(hipStreamSynchronize(stream));
-        return ms.h_ctl[MS_DONE] != 0 || ms.h_ctl[MS_ERR] != 0;
-    };
 
-    const int depth_hint = std::max(ms.last_depth + 1, 4);
+    const int depth_hint = std::min(std::max(ms.last_depth + 1, 4), kMaxCapturedLevels);
     GG_HIP(hipEventRecord(ev_start, stream));
     if (g.ms_cfg.use_graph) {
-        // One captured sequence: sources upload, clears, depth_hint levels.
-        // Captured arguments are fixed, so the graph is rebuilt when k, the
-        // output buffer, the kernel shape or the depth changes.
-        const bool reuse = ms.graph_exec && ms.graph_k == k && ms.graph_levels == d_levels &&
-                           ms.graph_depth == depth_hint && ms.graph_block == g.ms_cfg.block &&
-                           ms.graph_kpt == g.ms_cfg.edges_per_thread;
+        // One captured sequence: parameter upload, clears, depth_hint levels.
+        // Everything batch-specific is read from device memory, so the graph
+        // is rebuilt only when the kernel shape or the depth hint changes.
+        const bool reuse = ms.graph_exec && ms.graph_depth == depth_hint && ms.graph_block == g.ms_cfg.block &&
+                           ms.graph_kpt == g.ms_cfg.edges_per_thread && ms.graph_sparse == a.sparse_edges;
         if (!reuse) {
             if (ms.graph_exec) (void)hipGraphExecDestroy(ms.graph_exec);
             ms.graph_exec = nullptr;
@@ -502,11 +575,10 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
             hipError_t e = hipGraphInstantiate(&ms.graph_exec, captured, nullptr, nullptr, 0);
             (void)hipGraphDestroy(captured);
             GG_HIP(e);
-            ms.graph_k = k;
-            ms.graph_levels = d_levels;
             ms.graph_depth = depth_hint;
             ms.graph_block = g.ms_cfg.block;
             ms.graph_kpt = g.ms_cfg.edges_per_thread;
+            ms.graph_sparse = a.sparse_edges;
             GG_HIP(hipEventRecord(ev_start, stream));
         }
         GG_HIP(hipGraphLaunch(ms.graph_exec, stream));
@@ -516,21 +588,47 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
         for (int i = 0; i < depth_hint; ++i) launch_level();
     }
     GG_HIP(hipGetLastError());
-    // Deeper than the hint: continue level by level in batches of 8.
-    while (!check_done()) {
-        if (L > n + 2) fail(GB_EDEVICE, "internal error: more BFS levels than vertices");
-        for (int i = 0; i < 8; ++i) launch_level();
+    // Deeper than the hint: batches of levels of doubling size (at most
+    // kMaxLevelBatch) are pipelined -- the done flag of batch b-1 is checked
+    // while batch b runs; levels past the end are no-ops.
+    auto snapshot = [&](int slot) {
+        GG_HIP(hipMemcpyAsync(ms.h_ctl + slot * kCtlWords, ms.ctl, sizeof(unsigned long long) * kCtlWords,
+                              hipMemcpyDeviceToHost, stream));
+        GG_HIP(hipEventRecord(ev_batch[slot], stream));
+    };
+    auto done = [&](int slot) {
+        GG_HIP(hipEventSynchronize(ev_batch[slot]));
+        const unsigned long long* c = ms.h_ctl + slot * kCtlWords;
+        return c[MS_DONE] != 0 || c[MS_ERR] != 0;
+    };
+    snapshot(0);
+    if (!done(0)) {
+        int batch = 8;
+        for (int i = 0; i < batch; ++i) launch_level();
         GG_HIP(hipGetLastError());
+        snapshot(1);
+        for (int b = 2;; ++b) {
+            if (L > n + 2 * kMaxLevelBatch) fail(GB_EDEVICE, "internal error: more BFS levels than vertices");
+            batch = std::min(batch * 2, kMaxLevelBatch);
+            for (int i = 0; i < batch; ++i) launch_level();
+            GG_HIP(hipGetLastError());
+            snapshot(b & 1);
+            if (done((b - 1) & 1)) break;
+        }
     }
     GG_HIP(hipEventRecord(ev_stop, stream));
-    GG_HIP(hipEventSynchronize(ev_stop));
-    if (ms.h_ctl[MS_ERR] & MS_ERR_TILE) fail(GB_EDEVICE, "internal error: inconsistent frontier tile index");
-    if (ms.h_ctl[MS_ERR] & MS_ERR_RANGE)
+    unsigned long long fin[kCtlWords];
+    GG_HIP(hipMemcpyAsync(fin, ms.ctl, sizeof(fin), hipMemcpyDeviceToHost, stream));
+    GG_HIP(hipStreamSynchronize(stream));
+    if (fin[MS_ERR] & MS_ERR_TILE) fail(GB_EDEVICE, "internal error: inconsistent frontier tile index");
+    if (fin[MS_ERR] & MS_ERR_RANGE)
         fail(GB_EDEVICE, "internal error: frontier entry or column index out of range");
-    ms.last_depth = static_cast<int>(ms.h_ctl[MS_MAXLEV]) + 1;
+    if (!fin[MS_DONE]) fail(GB_EDEVICE, "internal error: batch did not reach an empty frontier");
+    ms.last_depth = static_cast<int>(fin[MS_MAXLEV]) + 1;
+    float ms_elapsed = 0.f;
+    GG_HIP(hipEventElapsedTime(&ms_elapsed, ev_start, ev_stop));
+    last_ms = ms_elapsed;
     if (st) {
-        float ms_elapsed = 0.f;
-        GG_HIP(hipEventElapsedTime(&ms_elapsed, ev_start, ev_stop));
         std::memset(st, 0, sizeof(*st));
         st->ms = ms_elapsed;
         st->num_levels = ms.last_depth;

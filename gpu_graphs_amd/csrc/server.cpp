@@ -145,13 +145,36 @@ bool Server::try_dispatch(std::unique_lock<std::mutex>&) {
     if (lane < 0) return false;
 
     const int64_t n = exec_->vertices(gid);
-    const int64_t buf_bytes = int64_t(cfg_.max_batch) * n * int64_t(sizeof(int32_t));
+    const int64_t row_bytes = n * int64_t(sizeof(int32_t));
+    int k = 0;  // queries of this graph that can go into the batch
+    for (uint64_t t : pq) {
+        auto qi = queries_.find(t);
+        if (qi != queries_.end() && qi->second.status == GB_EAGAIN && !qi->second.batch && ++k == cfg_.max_batch)
+            break;
+    }
+    if (k == 0) {  // only stale entries
+        pq.clear();
+        return true;
+    }
     const int64_t ws_bytes =
         lanes_[lane].has_workspace.count(gid) ? 0 : static_cast<int64_t>(exec_->workspace_bytes(gid, cfg_.plain));
     std::vector<Buffer>& pool = free_buffers_[gid];
-    int64_t need = ws_bytes + (pool.empty() ? buf_bytes : 0);
-    if (used_ + need > cfg_.memory_budget) {
-        // Give back pooled result buffers of other graphs first.
+    auto pow2 = [](int x) {
+        int p = 1;
+        while (p < x) p <<= 1;
+        return p;
+    };
+    // Smallest pooled buffer of this graph with at least `rows` rows.
+    auto pooled = [&](int rows) {
+        int best = -1;
+        for (int i = 0; i < static_cast<int>(pool.size()); ++i)
+            if (pool[i].rows >= rows && (best < 0 || pool[i].rows < pool[best].rows)) best = i;
+        return best;
+    };
+    auto need_for = [&](int rows) { return ws_bytes + (pooled(rows) >= 0 ? 0 : pow2(rows) * row_bytes); };
+    if (used_ + need_for(k) > cfg_.memory_budget) {
+        // Give back pooled result buffers: other graphs' first, then this
+        // graph's that are too small for the batch.
         for (auto& kv : free_buffers_) {
             if (kv.first == gid) continue;
             for (auto& b : kv.second) {
@@ -160,14 +183,25 @@ bool Server::try_dispatch(std::unique_lock<std::mutex>&) {
             }
             kv.second.clear();
         }
+        for (auto it = pool.begin(); it != pool.end();) {
+            if (it->rows < k) {
+                exec_->free_result(it->ptr);
+                used_ -= static_cast<int64_t>(it->bytes);
+                it = pool.erase(it);
+            } else {
+                ++it;
+            }
+        }
     }
+    while (k > 1 && used_ + need_for(k) > cfg_.memory_budget) k = pow2(k) / 2;  // shrink to what fits
+    const int64_t need = need_for(k);
     if (used_ + need > cfg_.memory_budget) {
         bool held = false;  // result buffers that a completion or release will return
         for (auto& kv : queries_)
             if (kv.second.batch && kv.second.status != GB_EAGAIN) held = true;
         if (in_flight_ == 0 && !held) {
             // Cannot fit even with everything else idle: fail this graph's queries.
-            const std::string err = "a batch on graph " + std::to_string(gid) + " needs " + std::to_string(need) +
+            const std::string err = "a query on graph " + std::to_string(gid) + " needs " + std::to_string(need) +
                                     " bytes; the budget is " + std::to_string(cfg_.memory_budget) + " with " +
                                     std::to_string(used_) + " in use";
             for (uint64_t t : pq) {
@@ -187,7 +221,7 @@ bool Server::try_dispatch(std::unique_lock<std::mutex>&) {
     auto b = std::make_shared<Batch>();
     b->gid = gid;
     b->lane = lane;
-    while (!pq.empty() && static_cast<int>(b->tickets.size()) < cfg_.max_batch) {
+    while (!pq.empty() && static_cast<int>(b->tickets.size()) < k) {
         const uint64_t t = pq.front();
         pq.pop_front();
         auto qi = queries_.find(t);
@@ -203,11 +237,13 @@ bool Server::try_dispatch(std::unique_lock<std::mutex>&) {
     if (b->tickets.empty()) return true;
     used_ += need;
     if (ws_bytes) lanes_[lane].has_workspace[gid] = true;
-    if (!pool.empty()) {
-        b->buf = pool.back();
-        pool.pop_back();
+    const int use = pooled(static_cast<int>(b->tickets.size()));
+    if (use >= 0) {
+        b->buf = pool[use];
+        pool.erase(pool.begin() + use);
     } else {
-        b->buf.bytes = static_cast<size_t>(buf_bytes);
+        b->buf.rows = pow2(static_cast<int>(b->tickets.size()));
+        b->buf.bytes = static_cast<size_t>(b->buf.rows * row_bytes);
         b->buf.gid = gid;
     }
     b->live = static_cast<int>(b->tickets.size());

@@ -7,9 +7,11 @@
 // batch at a time on its own GPU stream, so batches on different lanes run
 // concurrently on the device, and queries that arrive while lanes are busy
 // join the next batch as soon as a lane and memory free up.  Admission is by
-// a device-memory budget: a batch needs a result buffer (max_batch rows of
-// n int32) and, the first time a lane serves a graph, that lane's workspace.
-// Results stay in device memory until the client releases them.
+// a device-memory budget: a batch of k queries needs a result buffer of k
+// rows of n int32 (rounded up to a power of two, pooled per graph and size)
+// and, the first time a lane serves a graph, that lane's workspace.  When
+// the budget cannot hold the whole batch it shrinks to what fits.  Results
+// stay in device memory until the client releases them.
 //
 // The device work is behind the Executor interface; the HIP executor runs the
 // multi-source BFS (or, in plain mode, one single-source BFS per query), and
@@ -107,6 +109,7 @@ class Server {
         void* ptr = nullptr;
         size_t bytes = 0;
         int gid = -1;
+        int rows = 0;
     };
     struct Batch {
         int gid = -1;

@@ -4,6 +4,12 @@
 // batches on different lanes -- on the same graph or on different graphs --
 // run concurrently on the device.  Result buffers are plain device
 // allocations; the server pools and budgets them.
+//
+// A batch of k queries runs either as one multi-source BFS or as k
+// single-source BFS, whichever the measured per-graph costs say is cheaper:
+// a multi-source level scans per-vertex masks, so on deep, thin graphs a
+// small batch can cost more than its queries one by one.  The costs are
+// running averages of the device time of past runs on that graph.
 #include <hip/hip_runtime.h>
 
 #include <map>
@@ -40,10 +46,9 @@ class HipExecutor : public Executor {
 
     size_t workspace_bytes(int gid, bool plain) override {
         DeviceGraph* g = graph(gid);
-        if (plain) {  // single-source workspace: two queues, bitmaps, tile index, staging
-            return size_t(g->n) * (2 * (8 + 4 + 4) + 8) + size_t(g->m / 64 + 64) * 8 + (size_t(g->n) / 4);
-        }
-        return batch_bytes_estimate(g->n, g->m, 64, false);
+        // single-source workspace: two queues, bitmaps, tile index, staging
+        const size_t single = size_t(g->n) * (2 * (8 + 4 + 4) + 8) + size_t(g->m / 64 + 64) * 8 + (size_t(g->n) / 4);
+        return plain ? single : single + batch_bytes_estimate(g->n, g->m, 64, false);
     }
 
     void* alloc_result(size_t bytes) override {
@@ -68,10 +73,16 @@ class HipExecutor : public Executor {
         DeviceGraph* g = graph(gid);
         std::unique_ptr<Runner>& r = runners_[lane][gid];  // only this lane's thread touches runners_[lane]
         if (!r) r = std::make_unique<Runner>(*g);
-        if (plain) {
-            for (int i = 0; i < k; ++i) r->bfs(sources[i], result + int64_t(i) * g->n, nullptr, nullptr);
+        if (plain || prefer_single(gid, k)) {
+            double total = 0;
+            for (int i = 0; i < k; ++i) {
+                r->bfs(sources[i], result + int64_t(i) * g->n, nullptr, nullptr);
+                total += r->last_ms;
+            }
+            record(gid, true, total / k);
         } else {
             r->bfs_batch(sources, k, result, nullptr);
+            record(gid, false, r->last_ms);
         }
     }
 
@@ -85,6 +96,28 @@ class HipExecutor : public Executor {
     int device() const { return device_; }
 
    private:
+    struct Cost {
+        double single_ms = -1;  // per query
+        double batch_ms = -1;   // per multi-source batch
+    };
+
+    // Unknown costs are learned first: a lone query runs single-source, a
+    // batch of two or more runs multi-source.
+    bool prefer_single(int gid, int k) {
+        std::lock_guard<std::mutex> lk(cost_mu_);
+        const Cost& c = cost_[gid];
+        if (k == 1 && c.single_ms < 0) return true;
+        if (c.batch_ms < 0) return k == 1;
+        if (c.single_ms < 0) return false;
+        return k * c.single_ms <= c.batch_ms;
+    }
+
+    void record(int gid, bool single, double ms) {
+        std::lock_guard<std::mutex> lk(cost_mu_);
+        double& avg = single ? cost_[gid].single_ms : cost_[gid].batch_ms;
+        avg = avg < 0 ? ms : 0.7 * avg + 0.3 * ms;
+    }
+
     DeviceGraph* graph(int gid) {
         std::lock_guard<std::mutex> lk(mu_);
         if (gid < 0 || gid >= static_cast<int>(graphs_.size())) fail(GB_EINVAL, "unknown graph id");
@@ -92,7 +125,8 @@ class HipExecutor : public Executor {
     }
 
     int device_;
-    std::mutex mu_, fetch_mu_;
+    std::mutex mu_, fetch_mu_, cost_mu_;
+    std::map<int, Cost> cost_;
     std::vector<DeviceGraph*> graphs_;
     std::vector<std::map<int, std::unique_ptr<Runner>>> runners_;
     hipStream_t fetch_stream_ = nullptr;

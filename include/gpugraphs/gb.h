@@ -167,10 +167,14 @@ typedef struct gb_batch_stats {
 int  gb_bfs_batch(gb_dgraph* dg, const int32_t* sources, int k, int32_t* levels, gb_batch_stats* st);
 int  gb_bfs_batch_device(gb_dgraph* dg, const int32_t* sources, int k, int32_t* d_levels,
                          gb_batch_stats* st);
-/* Batch kernel shape (same pairs as gb_bfs_config) and whether a batch is
- * replayed as one captured hipGraph (1) or launched kernel by kernel (0). */
-int  gb_batch_config(gb_dgraph* dg, int block, int edges_per_thread, int use_graph);
-int  gb_batch_get_config(const gb_dgraph* dg, int* block, int* edges_per_thread, int* use_graph);
+/* Batch kernel shape (same pairs as gb_bfs_config), whether a batch is
+ * replayed as one captured hipGraph (1) or launched kernel by kernel (0), and
+ * the frontier size (CSR entries) up to which a level hands a list of the
+ * vertices it reached to the next one instead of a scan of all n vertices
+ * (0 = default, max(2^16, n/16)). */
+int  gb_batch_config(gb_dgraph* dg, int block, int edges_per_thread, int use_graph, int64_t sparse_edges);
+int  gb_batch_get_config(const gb_dgraph* dg, int* block, int* edges_per_thread, int* use_graph,
+                         int64_t* sparse_edges);
 
 /* ---- continuous-batching query server (no reference counterpart: the
  *      reference answers one (graph, source) per process run, 1D Partition/

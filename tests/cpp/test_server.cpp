@@ -159,7 +159,7 @@ static void test_budget_limits_concurrency_and_held_results() {
     cfg.lanes = 3;
     cfg.max_batch = 4;
     const int64_t n = 1000;
-    cfg.memory_budget = 4 * n * 4 + 1000 + 100;  // one result buffer + one lane workspace
+    cfg.memory_budget = n * 4 + 1000 + 100;  // one result row + one lane workspace
     Fixture f(cfg, {{0, n}}, 20);
     uint64_t a = f.srv->submit(0, 1);
     CHECK(f.srv->wait(a, 5000, nullptr, nullptr) == GB_OK);
@@ -171,15 +171,46 @@ static void test_budget_limits_concurrency_and_held_results() {
     CHECK(f.srv->wait(b, 5000, row.data(), nullptr) == GB_OK);
     CHECK(row_ok(f.ex, 0, 99, row));
     f.srv->release(b);
-    // Many queries: never more than one batch in flight under this budget.
+    // Many queries: batches shrink to one row, never more than one in flight.
     std::vector<uint64_t> many;
     for (int i = 0; i < 24; ++i) many.push_back(f.srv->submit(0, i));
     for (uint64_t t : many) {
-        CHECK(f.srv->wait(t, 10000, nullptr, nullptr) == GB_OK);
+        QueryLatency lat;
+        CHECK(f.srv->wait(t, 10000, nullptr, &lat) == GB_OK);
+        CHECK(lat.batch_size == 1);
         f.srv->release(t);
     }
     CHECK(f.ex->max_running.load() == 1);
     CHECK(f.srv->stats().memory_used <= cfg.memory_budget);
+}
+
+static void test_batch_shrinks_to_budget() {
+    // Room for 4 result rows and one workspace: 10 queued queries are served
+    // in batches of at most 4 rows, whatever is held or pooled meanwhile.
+    ServerConfig cfg;
+    cfg.lanes = 1;
+    const int64_t n = 1000;
+    cfg.memory_budget = 4 * n * 4 + 1000 + 100;
+    Fixture f(cfg, {{0, n}}, 30);
+    uint64_t first = f.srv->submit(0, 1);
+    std::this_thread::sleep_for(std::chrono::milliseconds(10));
+    std::vector<uint64_t> later;
+    for (int i = 0; i < 10; ++i) later.push_back(f.srv->submit(0, 20 + i));
+    CHECK(f.srv->wait(first, 5000, nullptr, nullptr) == GB_OK);
+    f.srv->release(first);
+    int served = 0;
+    for (size_t i = 0; i < later.size(); ++i) {
+        std::vector<int32_t> row(n);
+        QueryLatency lat;
+        CHECK(f.srv->wait(later[i], 5000, row.data(), &lat) == GB_OK);
+        CHECK(row_ok(f.ex, 0, int32_t(20 + i), row));
+        CHECK(lat.batch_size >= 1 && lat.batch_size <= 4);
+        served += 1;
+        f.srv->release(later[i]);
+    }
+    CHECK(served == 10);
+    CHECK(f.srv->stats().memory_used <= cfg.memory_budget);
+    for (int k : f.ex->batch_sizes) CHECK(k <= 4);
 }
 
 static void test_query_that_never_fits() {
@@ -306,6 +337,7 @@ int main() {
                  {"continuous_admission", test_continuous_admission},
                  {"plain_mode", test_plain_mode},
                  {"budget", test_budget_limits_concurrency_and_held_results},
+                 {"batch_shrinks_to_budget", test_batch_shrinks_to_budget},
                  {"never_fits", test_query_that_never_fits},
                  {"injected_failures", test_injected_failures},
                  {"backpressure_and_cancel", test_backpressure_and_cancel},

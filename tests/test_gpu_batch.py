@@ -87,6 +87,43 @@ def test_deep_grid_beyond_depth_hint(gpu):
         check_batch(G, dg, sources)
 
 
+# sparse_edges: 1 = every level scans all vertices, 2**32-1 = every level
+# hands a touched list to the next, 0 = the default threshold.
+@pytest.mark.parametrize("sparse", [1, 2**32 - 1, 0])
+@pytest.mark.parametrize("graph", ["rmat", "grid"])
+def test_level_modes(gpu, graph, sparse):
+    if graph == "rmat":
+        G = gg.Graph.rmat(14, seed=61)
+    else:
+        G = gg.Graph.grid(150, 70, 0.05, seed=62)
+    dg = gg.DeviceGraph(G, gpu)
+    dg.batch_config(sparse_edges=sparse)
+    assert dg.get_batch_config()["sparse_edges"] == (sparse or max(1 << 16, G.n // 16))
+    deg = np.diff(G.offsets)
+    rng = np.random.default_rng(sparse % 1000)
+    sources = rng.choice(np.nonzero(deg)[0], 40, replace=False).astype(np.int32)
+    log(f"{graph} n={G.n} sparse_edges={sparse}")
+    for _ in range(2):  # second run replays the captured sequence
+        check_batch(G, dg, sources)
+
+
+def test_one_capture_serves_any_k_and_buffer(gpu):
+    # Sources, k and the output buffer live in device memory: batches of
+    # different sizes into different buffers replay one captured sequence.
+    G = gg.Graph.rmat(13, seed=63)
+    dg = gg.DeviceGraph(G, gpu)
+    deg = np.diff(G.offsets)
+    rng = np.random.default_rng(63)
+    cand = np.nonzero(deg)[0]
+    bufs = [gg.DeviceArray(gpu, np.int32, 64 * G.n) for _ in range(2)]
+    for rep, k in enumerate([64, 5, 1, 33, 64]):
+        sources = rng.choice(cand, k, replace=False).astype(np.int32)
+        d = bufs[rep % 2]
+        dg.bfs_batch_device(sources, d)
+        got = d.numpy_slice(0, k * G.n).reshape(k, G.n)
+        np.testing.assert_array_equal(got, O.bfs_multi(G.offsets, G.cols, sources), err_msg=f"k={k}")
+
+
 def test_graph_replays_match_eager(gpu):
     G = gg.Graph.rmat(15, seed=12)
     dg = gg.DeviceGraph(G, gpu)
