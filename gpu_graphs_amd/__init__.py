@@ -22,13 +22,15 @@ import numpy as np
 
 from . import _ffi
 from ._ffi import (GB_AUTO_N, GB_DROP_SELF_LOOPS, GB_EAGAIN, GB_EBUSY, GB_EDEVICE, GB_EINVAL, GB_EIO, GB_ENOMEM,
-                   GB_ERANGE, GB_KEEP_ORDER, GB_MODE_ASYNC, GB_MODE_LEVEL_LOOP, GB_OK, GB_ONE_BASED, GB_SYMMETRIZE,
+                   GB_ERANGE, GB_KEEP_ORDER, GB_MODE_ASYNC, GB_MODE_GRAPH, GB_MODE_LEVEL_LOOP, GB_OK, GB_ONE_BASED, GB_SYMMETRIZE,
                    BatchStats, GBError, Stats, check, lib)
+
+GB_DEGREE_BUCKETS = 34
 
 __all__ = [
     "Graph", "DeviceGraph", "DeviceArray", "Server", "csr", "gpuBFS", "GBError", "Stats", "device_count",
     "exclusive_scan", "GB_AUTO_N", "GB_ONE_BASED", "GB_SYMMETRIZE", "GB_KEEP_ORDER", "GB_DROP_SELF_LOOPS",
-    "GB_MODE_ASYNC", "GB_MODE_LEVEL_LOOP", "GB_OK", "GB_EINVAL", "GB_EIO", "GB_ENOMEM", "GB_EDEVICE", "GB_ERANGE",
+    "GB_MODE_ASYNC", "GB_MODE_LEVEL_LOOP", "GB_MODE_GRAPH", "GB_OK", "GB_EINVAL", "GB_EIO", "GB_ENOMEM", "GB_EDEVICE", "GB_ERANGE",
     "GB_EBUSY", "GB_EAGAIN",
 ]
 
@@ -135,6 +137,24 @@ class Graph:
     def degrees(self):
         return np.diff(self.offsets)
 
+    def degree_buckets(self):
+        """(offsets[35], vertices[n]): bucket 0 = degree 0, bucket b >= 1 =
+        degree in [2^(b-1), 2^b), ids ascending inside a bucket."""
+        off = np.zeros(GB_DEGREE_BUCKETS + 1, dtype=np.int64)
+        v = C.c_void_p()
+        check(lib().gb_graph_degree_buckets(self._h, off.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(v)))
+        n = self.n
+        if n == 0:
+            return off, np.zeros(0, dtype=np.int32)
+        arr = np.ctypeslib.as_array(C.cast(v, C.POINTER(C.c_int32)), shape=(n,)).copy()
+        return off, arr
+
+    @property
+    def is_symmetric(self):
+        s = C.c_int()
+        check(lib().gb_graph_is_symmetric(self._h, C.byref(s)))
+        return bool(s.value)
+
     def write_edgelist(self, path):
         check(lib().gb_graph_write_edgelist(self._h, str(path).encode()))
 
@@ -215,7 +235,7 @@ class DeviceGraph:
         self.n = graph.n
         self.m = graph.m
 
-    def config(self, block=256, edges_per_thread=4, blocks_per_cu=0, mode=GB_MODE_ASYNC, dense_edges=0):
+    def config(self, block=256, edges_per_thread=4, blocks_per_cu=0, mode=GB_MODE_GRAPH, dense_edges=0):
         """Kernel tunables; dense_edges=0 selects the default threshold."""
         check(lib().gb_bfs_config(self._h, block, edges_per_thread, blocks_per_cu, mode, dense_edges))
 
@@ -225,6 +245,28 @@ class DeviceGraph:
         check(lib().gb_bfs_get_config(self._h, C.byref(b), C.byref(k), C.byref(bpc), C.byref(mode), C.byref(dense)))
         return {"block": b.value, "edges_per_thread": k.value, "blocks_per_cu": bpc.value, "mode": mode.value,
                 "dense_edges": dense.value}
+
+    def direction_config(self, enable=True, alpha=0, beta=0):
+        """Bottom-up levels (symmetric graphs only); alpha/beta 0 keep the current values."""
+        check(lib().gb_bfs_direction_config(self._h, int(enable), int(alpha), int(beta)))
+
+    def get_direction_config(self):
+        e, a, b, s = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(lib().gb_bfs_get_direction_config(self._h, C.byref(e), C.byref(a), C.byref(b), C.byref(s)))
+        return {"enable": bool(e.value), "alpha": a.value, "beta": b.value, "symmetric": bool(s.value)}
+
+    def launch_hints(self):
+        """(depth, bu_lo, bu_hi): levels queued before the first done check and
+        the levels that get a bottom-up launch, learned from earlier searches."""
+        d, lo, hi = C.c_int(), C.c_int(), C.c_int()
+        check(lib().gb_bfs_launch_hints(self._h, C.byref(d), C.byref(lo), C.byref(hi)))
+        return d.value, lo.value, hi.value
+
+    def last_bottom_up(self):
+        """(first, last) level the last bfs() searched bottom-up, (-1, -1) if none."""
+        f, l = C.c_int(), C.c_int()
+        check(lib().gb_bfs_last_bottom_up(self._h, C.byref(f), C.byref(l)))
+        return f.value, l.value
 
     def memory_bytes(self):
         b = C.c_int64()

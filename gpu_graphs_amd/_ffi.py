@@ -26,6 +26,7 @@ GB_AUTO_N = 16
 
 GB_MODE_ASYNC = 0
 GB_MODE_LEVEL_LOOP = 1
+GB_MODE_GRAPH = 2
 
 _NAMES = {
     GB_EINVAL: "GB_EINVAL", GB_EIO: "GB_EIO", GB_ENOMEM: "GB_ENOMEM", GB_EDEVICE: "GB_EDEVICE",
@@ -118,6 +119,8 @@ SIGNATURES = {
     "gb_graph_generate_grid": (C.c_int, [C.c_int64, C.c_int64, C.c_double, C.c_uint64, PP]),
     "gb_graph_info": (C.c_int, [P, i64p, i64p]),
     "gb_graph_csr": (C.c_int, [P, C.POINTER(P), C.POINTER(P)]),
+    "gb_graph_degree_buckets": (C.c_int, [P, i64p, C.POINTER(P)]),
+    "gb_graph_is_symmetric": (C.c_int, [P, C.POINTER(C.c_int)]),
     "gb_graph_write_edgelist": (C.c_int, [P, C.c_char_p]),
     "gb_graph_save_binary": (C.c_int, [P, C.c_char_p]),
     "gb_graph_load_binary": (C.c_int, [C.c_char_p, PP]),
@@ -132,6 +135,11 @@ SIGNATURES = {
     "gb_bfs_config": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64]),
     "gb_bfs_get_config": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                     C.POINTER(C.c_int), i64p]),
+    "gb_bfs_direction_config": (C.c_int, [P, C.c_int, C.c_int, C.c_int]),
+    "gb_bfs_get_direction_config": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                              C.POINTER(C.c_int)]),
+    "gb_bfs_last_bottom_up": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "gb_bfs_launch_hints": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "gb_bfs_batch": (C.c_int, [P, P, C.c_int, P, C.POINTER(BatchStats)]),
     "gb_bfs_batch_device": (C.c_int, [P, P, C.c_int, P, C.POINTER(BatchStats)]),
     "gb_batch_config": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int64]),

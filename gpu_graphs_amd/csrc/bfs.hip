@@ -1,4 +1,4 @@
-// Single-source top-down BFS for gfx950.
+// Single-source BFS for gfx950, direction-optimising (top-down / bottom-up).
 //
 // Replaces the reference's level loop (1D Partition/src/gpuBFS.cu:66-100) and
 // its kernels `linear_bfs` (gpuBFS_kernels.cu:16-80: thread per queue entry,
@@ -14,15 +14,15 @@
 // kernel launches ~1.5 us, a software grid barrier 17-57 us.  Hence no grid
 // barrier and no per-tile atomics on a shared counter in large levels:
 //
-//  * A level L runs as two launches, `expand(L)` then `compact(L)`, issued
-//    back to back with no host synchronisation; the host only checks a done
-//    flag once per batch of levels.
-//  * The frontier of a level is a queue of entries (vertex, first CSR entry,
-//    exclusive prefix of degrees).  expand(L) cuts the level's edges into
-//    tiles of T = BLOCK*K edges; tile_start[t] names the entry owning edge
-//    t*T, so each workgroup loads its entries with one lookup and every
-//    thread owns K edges of the tile (coalesced column reads) whatever the
-//    degree distribution: hubs are spread over many workgroups.
+//  * A level L runs as `expand(L)`, optionally `bottom_up(L)`, and
+//    `compact(L)`, issued back to back with no host synchronisation; the
+//    host only checks a done flag once per batch of levels.
+//  * Top-down: the frontier of a level is a queue of entries (vertex, first
+//    CSR entry, exclusive prefix of degrees).  expand(L) cuts the level's
+//    edges into tiles of T = BLOCK*K edges; tile_start[t] names the entry
+//    owning edge t*T, so each workgroup loads its entries with one lookup and
+//    every thread owns K edges of the tile (coalesced column reads) whatever
+//    the degree distribution: hubs are spread over many workgroups.
 //  * Claims: plain load of the visited-bitmap word (may be stale, never
 //    wrongly "visited"), then atomicOr; the winner writes level L+1 and the
 //    parent.  Levels are deterministic.
@@ -36,15 +36,33 @@
 //    serial sums, a wave64 scan and an LDS-staged block scan give each new
 //    vertex its offset inside the chunk, and the chunk reserves its queue
 //    slots and edge range with one packed 64-bit atomicAdd, like a sparse
-//    tile.  No workgroup waits on another, so the kernels need no particular
+//    tile.  It also stores the new vertices as a frontier bitmap.
+//  * Bottom-up (Beamer, Asanovic, Patterson, SC 2012), on symmetric graphs:
+//    when a level's frontier edges exceed the unexplored edges / alpha, each
+//    unvisited vertex looks for a neighbour in the frontier bitmap and stops
+//    at the first one.  A wave owns 64 consecutive vertices (two bitmap
+//    words, so it updates them with plain stores); rows of at most
+//    kBuLowDegree entries are scanned by their lane alone, longer rows by the
+//    whole wave 64 entries at a time with a ballot -- the degree tiers of a
+//    thread or a wavefront per vertex.  Isolated vertices are marked visited
+//    up front.  Search goes back top-down when the frontier's edges * beta
+//    fall below the edges still unexplored (a bottom-up level rescans
+//    those); compact() then turns the frontier bitmap into a queue.
+//  * Directions are chosen on the device from counters that no kernel of
+//    the level modifies, so expand(L), bottom_up(L) and compact(L) agree.
+//    The host launches bottom_up(L) only for levels where searches of this
+//    graph have gone bottom-up (a window learned per handle); elsewhere the
+//    level runs top-down, which is always correct.
+//  * No workgroup waits on another, so the kernels need no particular
 //    residency and can share the device with other streams' work.
-//  * Every kernel stops at once when the error word is set, and the expand
-//    tiles check each queue entry and column index against n and m, so an
+//  * Every kernel stops at once when the error word is set, and the tiles
+//    check each queue entry and column index against n and m, so an
 //    inconsistent queue is reported as an error, never dereferenced.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "device_graph.hpp"
 #include "frontier.hpp"
@@ -56,40 +74,72 @@ namespace {
 
 // ctl[] layout (64-bit words).
 enum : int {
-    CTL_RING0 = 0,   // 3 packed (count << 32 | edges) frontier counters: level L uses ring[L % 3]
-    CTL_NREACH = 3,  // reached vertices
-    CTL_MREACH = 4,  // sum of degrees of reached vertices
-    CTL_MAXLEV = 5,  // largest level assigned
-    CTL_DONE = 6,    // set by the first expand() that finds an empty frontier
-    CTL_ERR = 7,     // internal consistency error (bit set)
-    CTL_WORDS = 16
+    CTL_RING0 = 0,     // 3 words: (count << 32 | edges) of level L's frontier at ring[L % 3]
+    CTL_NREACH = 3,    // reached vertices
+    CTL_MREACH = 4,    // CSR entries of the frontiers expanded so far (compact(L) adds level L's)
+    CTL_MAXLEV = 5,    // largest level assigned
+    CTL_DONE = 6,      // set by the first expand() that finds an empty frontier
+    CTL_ERR = 7,       // internal consistency error (bit set)
+    CTL_REP0 = 8,      // 3 words: forms of level L's frontier at rep[L % 3] (REP_* bits)
+    CTL_DIR0 = 11,     // 3 words: direction of level L at dir[L % 3]
+    CTL_QRES = 14,     // reservations when a frontier bitmap is turned into a queue
+    CTL_MAXE = 15,     // largest frontier seen (CSR entries)
+    CTL_BUFIRST = 16,  // first / last level searched bottom-up (~0 / 0 if none)
+    CTL_BULAST = 17,
+    CTL_EXP0 = 18,     // 2 words: CSR entries of the frontiers of levels < L at exp[L & 1]
+    CTL_DCNT0 = 20,    // 2 words: (count << 32 | edges) of the winners of a dense level L at dcnt[(L+1) & 1]
+    CTL_WORDS = 32
 };
 enum : unsigned long long { ERR_TILE = 1, ERR_RANGE = 2 };
+enum : unsigned long long { REP_QUEUE = 1, REP_BITMAP = 2 };
+enum : unsigned long long { DIR_TD = 1, DIR_BU = 2 };
 
-constexpr int kCompactBlock = 256;
+constexpr int kCompactBlock = 1024;
 constexpr int kCompactRounds = 16;                          // vertices per thread
 constexpr int kChunk = kCompactBlock * kCompactRounds;      // vertices per compaction chunk
+constexpr int kBuBlock = 1024;
+constexpr int kMaxCapturedLevels = 32;                      // deeper searches continue with direct launches
+constexpr uint32_t kBuLowDegree = 32;                       // longer rows are scanned by the whole wave
 
 struct BfsArgs {
     const int64_t* __restrict__ offsets;
     const int32_t* __restrict__ cols;
     uint32_t* vis;    // claimed vertices
     uint32_t* prev;   // vertices already placed in (or handled by) a frontier queue
-    int32_t* levels;
-    int32_t* parents;
+    uint32_t* fb[2];  // frontier bitmaps, by level parity
+    const uint32_t* iso;  // vertices of degree 0 (symmetric graphs), or null
     int64_t* q_rb[2];
     uint32_t* q_eoff[2];
     int32_t* q_v[2];
     uint32_t* tile_start[2];
     unsigned long long* ctl;
+    const BfsParams* p;         // source, levels, parents of this call (device memory)
     int64_t n;
     int64_t m;
     uint32_t nwords;
     uint32_t nchunks;
     uint32_t dense_edges;
     uint32_t tile_edges;        // T of the expand kernel (tile_start granularity)
-    int32_t source;
+    uint32_t bottom_up;         // direction optimisation enabled
+    uint32_t alpha, beta;
 };
+
+// Beamer's test for a frontier of E CSR entries after `explored` entries of
+// earlier frontiers: frontier edges * alpha against the unexplored edges.
+__device__ __forceinline__ bool go_bottom_up(const BfsArgs& a, int64_t explored, uint32_t E) {
+    const int64_t unexplored = a.m - explored - static_cast<int64_t>(E);
+    return static_cast<int64_t>(E) * a.alpha > unexplored;
+}
+
+// Direction of level L.  Reads only words that no kernel of level L writes.
+__device__ __forceinline__ unsigned long long choose_dir(const BfsArgs& a, int L, uint32_t E, bool bu_launched) {
+    if (!a.bottom_up || !bu_launched || L == 0) return DIR_TD;
+    const unsigned long long rep = a.ctl[CTL_REP0 + L % 3];
+    if (!(rep & REP_BITMAP)) return DIR_TD;
+    if (a.ctl[CTL_DIR0 + (L - 1) % 3] == DIR_BU) return (rep & REP_QUEUE) ? DIR_TD : DIR_BU;
+    // After a top-down level: frontier edges against the unexplored edges.
+    return go_bottom_up(a, static_cast<int64_t>(a.ctl[CTL_EXP0 + (L & 1)]), E) ? DIR_BU : DIR_TD;
+}
 
 template <int BLOCK, int K>
 struct TileLds {
@@ -108,8 +158,9 @@ struct TileLds {
 
 // Expands tile t of level L: frontier edges [t*T, min((t+1)*T, E)).
 template <int BLOCK, int K>
-__device__ void expand_tile(const BfsArgs& a, TileLds<BLOCK, K>& s, int L, uint32_t t, uint32_t ntiles,
-                            uint32_t F, uint32_t E, bool dense) {
+__device__ void expand_tile(const BfsArgs& a, int32_t* levels, int32_t* parents, TileLds<BLOCK, K>& s, int L,
+                            uint32_t t, uint32_t ntiles, uint32_t F, uint32_t E, bool dense, bool bitmap_next,
+                            uint32_t& dn_cnt, uint64_t& dn_edges, uint32_t& dn_zero) {
     constexpr uint32_t T = BLOCK * K;
     const int cur = L & 1, nxt = cur ^ 1;
     const uint32_t tid = threadIdx.x;
@@ -191,11 +242,26 @@ __device__ void expand_tile(const BfsArgs& a, TileLds<BLOCK, K>& s, int L, uint3
         const uint32_t uu = static_cast<uint32_t>(u[k]);
         won[k] = !(old[k] & (1u << (uu & 31u)));
         if (won[k]) {
-            a.levels[uu] = L + 1;
-            if (a.parents) a.parents[uu] = s.v[owner[k]];
+            levels[uu] = L + 1;
+            if (parents) parents[uu] = s.v[owner[k]];
         }
     }
-    if (dense) {  // compact() builds the next queue from the bitmaps
+    if (dense) {
+        // compact() builds the next queue (or bitmap) from the bitmaps; it
+        // needs the next level's totals first, so the winners are counted here.
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (won[k]) {
+                const uint32_t uu = static_cast<uint32_t>(u[k]);
+                const uint32_t d = static_cast<uint32_t>(a.offsets[uu + 1] - a.offsets[uu]);
+                if (d) {
+                    ++dn_cnt;
+                    dn_edges += d;
+                } else {
+                    ++dn_zero;
+                }
+            }
+        }
         __syncthreads();
         return;
     }
@@ -207,6 +273,7 @@ __device__ void expand_tile(const BfsArgs& a, TileLds<BLOCK, K>& s, int L, uint3
         wrb[k] = wre[k] = 0;
         if (won[k]) {
             atomicOr(&a.prev[uu >> 5], 1u << (uu & 31u));  // queued (or degree 0): compact() must skip it
+            if (bitmap_next) atomicOr(&a.fb[nxt][uu >> 5], 1u << (uu & 31u));
             wrb[k] = a.offsets[uu];
             wre[k] = a.offsets[uu + 1];
         }
@@ -281,9 +348,12 @@ __device__ void expand_tile(const BfsArgs& a, TileLds<BLOCK, K>& s, int L, uint3
     __syncthreads();
 }
 
-// Level L, step 1: expand the frontier queue of level L.
+// Level L, step 1: pick the direction and, top-down, expand the queue of
+// level L.  `bu_launched`: bottom_up(L) follows in the stream; `bu_next`:
+// bottom_up(L+1) will be launched, so a sparse level also records its
+// winners in the frontier bitmap fb[(L+1) & 1] (zeroed by compact(L-1)).
 template <int BLOCK, int K>
-__global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(BfsArgs a, int L) {
+__global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(BfsArgs a, int L, int bu_launched, int bu_next) {
     __shared__ TileLds<BLOCK, K> s;
     constexpr uint32_t T = BLOCK * K;
     // Levels queued past the end of the search are no-ops: once done, the
@@ -300,76 +370,242 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(BfsArgs a, int L) {
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.ctl[CTL_ERR], ERR_RANGE);
         return;
     }
+    const unsigned long long dir = choose_dir(a, L, E, bu_launched != 0);
+    const bool dense = E > a.dense_edges;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         // ring[(L+2)%3] held level L-1's counter; it next counts level L+2.
         a.ctl[CTL_RING0 + (L + 2) % 3] = 0;
-        atomicAdd(&a.ctl[CTL_MREACH], static_cast<unsigned long long>(E));
         atomicAdd(&a.ctl[CTL_NREACH], static_cast<unsigned long long>(F));
         atomicMax(&a.ctl[CTL_MAXLEV], static_cast<unsigned long long>(L));
+        atomicMax(&a.ctl[CTL_MAXE], static_cast<unsigned long long>(E));
+        a.ctl[CTL_DIR0 + L % 3] = dir;
+        a.ctl[CTL_REP0 + (L + 1) % 3] =
+            dir == DIR_BU ? REP_BITMAP : (dense ? 0ull : (bu_next ? REP_QUEUE | REP_BITMAP : REP_QUEUE));
+        a.ctl[CTL_QRES] = 0;
     }
-    const bool dense = E > a.dense_edges;
+    if (dir == DIR_BU) return;  // bottom_up(L) runs this level
+    int32_t* const levels = a.p->levels;
+    int32_t* const parents = a.p->parents;
     const uint32_t ntiles = static_cast<uint32_t>((uint64_t(E) + T - 1) / T);
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) expand_tile<BLOCK, K>(a, s, L, t, ntiles, F, E, dense);
+    uint32_t dn_cnt = 0, dn_zero = 0;
+    uint64_t dn_edges = 0;
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x)
+        expand_tile<BLOCK, K>(a, levels, parents, s, L, t, ntiles, F, E, dense, bu_next != 0, dn_cnt, dn_edges,
+                              dn_zero);
+    if (dense) {
+        // One atomic per workgroup: the next level's (count, edges) and the
+        // degree-0 winners (reached, never queued).
+        const unsigned long long mine =
+            (static_cast<unsigned long long>(wave_sum(dn_cnt)) << 32) | wave_sum(dn_edges);
+        const uint32_t z = wave_sum(dn_zero);
+        __shared__ unsigned long long s_tot[BLOCK / 64];
+        __shared__ uint32_t s_z[BLOCK / 64];
+        if ((threadIdx.x & 63) == 0) {
+            s_tot[threadIdx.x >> 6] = mine;
+            s_z[threadIdx.x >> 6] = z;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long tot = 0;
+            uint32_t zt = 0;
+            for (int i = 0; i < BLOCK / 64; ++i) {
+                tot += s_tot[i];
+                zt += s_z[i];
+            }
+            if (tot) atomicAdd(&a.ctl[CTL_DCNT0 + ((L + 1) & 1)], tot);
+            if (zt) {
+                atomicAdd(&a.ctl[CTL_NREACH], static_cast<unsigned long long>(zt));
+                atomicMax(&a.ctl[CTL_MAXLEV], static_cast<unsigned long long>(L + 1));
+            }
+        }
+    }
 }
 
-// Level L, step 2 (dense levels only): queue of level L+1 from the bitmaps.
-// Chunk c covers vertices [c*kChunk, (c+1)*kChunk); thread t takes vertices
-// c*kChunk + r*256 + t for r < 16, so the offsets loads of a wave are
+// Level L bottom-up: every unvisited vertex looks for a neighbour in the
+// frontier bitmap of level L; the vertices found form the bitmap of level
+// L+1, and their (count, CSR entries) go to ring[(L+1) % 3].
+__global__ __launch_bounds__(kBuBlock) void bfs_bottom_up_kernel(BfsArgs a, int L) {
+    __shared__ unsigned long long s_acc[kBuBlock / 64];
+    if (a.ctl[CTL_DONE] | a.ctl[CTL_ERR]) return;
+    const unsigned long long cnt = a.ctl[CTL_RING0 + L % 3];
+    const uint32_t F = static_cast<uint32_t>(cnt >> 32), E = static_cast<uint32_t>(cnt);
+    if (F == 0 || F > a.n || E > a.m) return;  // expand(L) has set the done or error word
+    if (choose_dir(a, L, E, true) != DIR_BU) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicMin(&a.ctl[CTL_BUFIRST], static_cast<unsigned long long>(L));
+        atomicMax(&a.ctl[CTL_BULAST], static_cast<unsigned long long>(L));
+    }
+    const uint32_t* fin = a.fb[L & 1];
+    uint32_t* fout = a.fb[(L + 1) & 1];
+    int32_t* const levels = a.p->levels;
+    int32_t* const parents = a.p->parents;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t ngroups = (static_cast<uint64_t>(a.n) + 63) / 64;
+    const uint64_t wstride = static_cast<uint64_t>(gridDim.x) * (kBuBlock / 64);
+    uint32_t my_cnt = 0;
+    uint64_t my_edges = 0;
+    bool bad = false;
+    for (uint64_t g = static_cast<uint64_t>(blockIdx.x) * (kBuBlock / 64) + (threadIdx.x >> 6); g < ngroups;
+         g += wstride) {
+        const int64_t v = static_cast<int64_t>(g * 64 + lane);
+        const uint32_t w = static_cast<uint32_t>(g * 2 + (lane >> 5));
+        const bool in_range = v < a.n;
+        const uint32_t vw = in_range ? a.vis[w] : ~0u;
+        const bool unvisited = in_range && !((vw >> (lane & 31u)) & 1u);
+        const uint64_t um = wave_ballot(unvisited);
+        if (!um) {
+            if ((lane & 31u) == 0 && w < a.nwords) fout[w] = 0u;
+            continue;
+        }
+        int64_t rb = 0, re = 0;
+        if (unvisited) {
+            rb = a.offsets[v];
+            re = a.offsets[v + 1];
+        }
+        const uint32_t deg = static_cast<uint32_t>(re - rb);
+        bool found = false;
+        int32_t par = -1;
+        // Short rows: the lane alone, 4 entries per step.
+        if (unvisited && deg <= kBuLowDegree) {
+            for (int64_t i = rb; i < re && !found; i += 4) {
+                int32_t u[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) u[j] = (i + j < re) ? a.cols[i + j] : -1;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (u[j] < 0 || found) continue;
+                    if (static_cast<uint32_t>(u[j]) >= static_cast<uint64_t>(a.n)) {
+                        bad = true;
+                        continue;
+                    }
+                    if ((fin[static_cast<uint32_t>(u[j]) >> 5] >> (u[j] & 31)) & 1u) {
+                        found = true;
+                        par = u[j];
+                    }
+                }
+            }
+        }
+        // Long rows: the whole wave, 64 entries per step, first hit wins.
+        uint64_t hm = wave_ballot(unvisited && deg > kBuLowDegree);
+        while (hm) {
+            const int hl = __ffsll(static_cast<long long>(hm)) - 1;
+            hm &= hm - 1;
+            const int64_t hrb = __shfl(rb, hl, 64);
+            const int64_t hre = __shfl(re, hl, 64);
+            int32_t hpar = -1;
+            for (int64_t b0 = hrb; b0 < hre; b0 += 64) {
+                const int64_t i = b0 + lane;
+                int32_t u = -1;
+                bool hit = false;
+                if (i < hre) {
+                    u = a.cols[i];
+                    if (static_cast<uint32_t>(u) >= static_cast<uint64_t>(a.n)) {
+                        bad = true;
+                    } else {
+                        hit = (fin[static_cast<uint32_t>(u) >> 5] >> (u & 31)) & 1u;
+                    }
+                }
+                const uint64_t hb = wave_ballot(hit);
+                if (hb) {
+                    hpar = __shfl(u, __ffsll(static_cast<long long>(hb)) - 1, 64);
+                    break;
+                }
+            }
+            if (static_cast<int>(lane) == hl && hpar >= 0) {
+                found = true;
+                par = hpar;
+            }
+        }
+        const uint64_t nb = wave_ballot(found);
+        if (found) {
+            levels[v] = L + 1;
+            if (parents) parents[v] = par;
+            ++my_cnt;
+            my_edges += deg;
+        }
+        // The wave owns words 2g and 2g+1: plain stores.
+        if ((lane & 31u) == 0 && w < a.nwords) {
+            const uint32_t bits = static_cast<uint32_t>(nb >> (lane & 32u));
+            fout[w] = bits;
+            if (bits) {
+                a.vis[w] = vw | bits;
+                a.prev[w] = vw | bits;
+            }
+        }
+    }
+    if (bad) atomicOr(&a.ctl[CTL_ERR], ERR_RANGE);
+    const unsigned long long mine = (static_cast<unsigned long long>(wave_sum(my_cnt)) << 32) | wave_sum(my_edges);
+    if (lane == 0) s_acc[threadIdx.x >> 6] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tot = 0;
+        for (int i = 0; i < kBuBlock / 64; ++i) tot += s_acc[i];
+        if (tot) atomicAdd(&a.ctl[CTL_RING0 + (L + 1) % 3], tot);
+    }
+}
+
+// Level L, step 3: queue of level L+1 from a bitmap, when level L+1 runs
+// top-down and its queue was not built during level L.  Chunk c covers
+// vertices [c*kChunk, (c+1)*kChunk); thread t takes vertices
+// c*kChunk + r*kCompactBlock + t for r < 16, so the offsets loads of a wave are
 // contiguous.  The thread's new vertices occupy consecutive queue slots; the
-// chunk's block goes where its atomicAdd on level L+1's counter puts it.
-__device__ void compact_chunk(const BfsArgs& a, int L, uint32_t c) {
+// chunk's block goes where its atomicAdd on the counter puts it.
+//   from_bitmap = false (dense top-down level): new = vis & ~prev; also
+//       stores the new vertices in fb[(L+1) & 1] and sets prev = vis;
+//       reserves on ring[(L+1) % 3].
+//   from_bitmap = true (after a bottom-up level): new = fb[(L+1) & 1];
+//       ring[(L+1) % 3] already holds the totals, so it reserves on QRES.
+__device__ void compact_chunk(const BfsArgs& a, int L, uint32_t c, bool from_bitmap) {
     __shared__ unsigned long long s_wave[kCompactBlock / 64];
     __shared__ unsigned long long s_excl;
     __shared__ uint32_t s_ok;
-    __shared__ uint32_t s_zero[kCompactBlock / 64];
     const uint32_t tid = threadIdx.x;
     const int64_t vbase = int64_t(c) * kChunk;
-    uint32_t newmask = 0;  // bit r: vertex vbase + r*256 + tid is new
-    uint32_t kept = 0, zero = 0, edges = 0;
+    const uint32_t* fnew = a.fb[(L + 1) & 1];
+    uint32_t newmask = 0;  // bit r: vertex vbase + r*kCompactBlock + tid is new (degree > 0)
+    uint32_t kept = 0, edges = 0;
 #pragma unroll
     for (int r = 0; r < kCompactRounds; ++r) {
         const int64_t v = vbase + r * kCompactBlock + tid;
         if (v < a.n) {
             const uint32_t w = static_cast<uint32_t>(v >> 5);
-            const uint32_t nb = a.vis[w] & ~a.prev[w];
+            const uint32_t nb = from_bitmap ? fnew[w] : (a.vis[w] & ~a.prev[w]);
             if ((nb >> (v & 31)) & 1u) {
                 const uint32_t d = static_cast<uint32_t>(a.offsets[v + 1] - a.offsets[v]);
                 if (d) {
                     newmask |= 1u << r;
                     ++kept;
                     edges += d;
-                } else {
-                    ++zero;
                 }
             }
         }
     }
-    // Block scan of (kept, edges) packed in 64 bits; zero-degree count summed.
+    // Block scan of (kept, edges) packed in 64 bits.  Degree-0 new vertices
+    // were counted when they were claimed.
     const unsigned long long mine = (static_cast<unsigned long long>(kept) << 32) | edges;
     unsigned long long total;
     const unsigned long long excl_local = block_exclusive_scan<kCompactBlock>(mine, s_wave, total);
-    const uint32_t zsum = wave_sum(zero);
-    if ((tid & 63) == 0) s_zero[tid >> 6] = zsum;
-    // prev |= vis for the chunk's words: every read of prev above happened
-    // before the block scan's barriers.
-    for (uint32_t i = tid; i < kChunk / 32; i += kCompactBlock) {
-        const int64_t w = vbase / 32 + i;
-        if (w < a.nwords) a.prev[w] = a.vis[w];
+    if (!from_bitmap) {
+        // fb[(L+1)&1] = new, prev |= vis for the chunk's words: every read of
+        // prev above happened before the block scan's barriers.
+        uint32_t* fout = a.fb[(L + 1) & 1];
+        for (uint32_t i = tid; i < kChunk / 32; i += kCompactBlock) {
+            const int64_t w = vbase / 32 + i;
+            if (w < a.nwords) {
+                const uint32_t vw = a.vis[w];
+                fout[w] = vw & ~a.prev[w];
+                a.prev[w] = vw;
+            }
+        }
     }
     if (tid == 0) {
         s_ok = 1;
         if (total) {
-            s_excl = atomicAdd(&a.ctl[CTL_RING0 + (L + 1) % 3], total);
+            s_excl = atomicAdd(&a.ctl[from_bitmap ? CTL_QRES : CTL_RING0 + (L + 1) % 3], total);
             if (!reservation_fits(s_excl, total, a.n, a.m)) {
                 atomicOr(&a.ctl[CTL_ERR], ERR_RANGE);
                 s_ok = 0;
             }
-        }
-        uint32_t z = 0;
-        for (int i = 0; i < kCompactBlock / 64; ++i) z += s_zero[i];
-        if (z) {
-            atomicAdd(&a.ctl[CTL_NREACH], static_cast<unsigned long long>(z));
-            atomicMax(&a.ctl[CTL_MAXLEV], static_cast<unsigned long long>(L + 1));
         }
     }
     __syncthreads();
@@ -394,54 +630,118 @@ __device__ void compact_chunk(const BfsArgs& a, int L, uint32_t c) {
     }
 }
 
-// Level L+1's counter ring[(L+1)%3] was zeroed by expand(L-1) and, on a
-// dense level, expand(L) left it alone: the chunks' reservations build it up.
-__global__ __launch_bounds__(kCompactBlock) void bfs_compact_kernel(BfsArgs a, int L) {
+// `bu_next`: bottom_up(L+1) will be launched; `bu_next2`: bottom_up(L+2)
+// will be, so fb[L & 1] (read by bottom_up(L) if at all, next written as
+// level L+2's bitmap by a sparse expand(L+1)) is zeroed here.  Level L+1's
+// counter ring[(L+1)%3] was zeroed by expand(L-1); a dense top-down level L
+// left it alone and the chunks' reservations build it up.
+__global__ __launch_bounds__(kCompactBlock) void bfs_compact_kernel(BfsArgs a, int L, int bu_next, int bu_next2) {
     if (a.ctl[CTL_DONE] | a.ctl[CTL_ERR]) return;
+    if (bu_next2) {
+        uint32_t* fz = a.fb[L & 1];
+        for (uint32_t i = blockIdx.x * kCompactBlock + threadIdx.x; i < a.nwords; i += gridDim.x * kCompactBlock)
+            fz[i] = 0u;
+    }
     const unsigned long long cnt = a.ctl[CTL_RING0 + L % 3];
     const uint32_t F = static_cast<uint32_t>(cnt >> 32), E = static_cast<uint32_t>(cnt);
-    if (F == 0 || E <= a.dense_edges) return;  // sparse level: expand() queued level L+1
+    if (F == 0) return;
+    const int64_t explored = static_cast<int64_t>(a.ctl[CTL_EXP0 + (L & 1)]) + E;  // levels <= L
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(&a.ctl[CTL_MREACH], static_cast<unsigned long long>(E));
+        a.ctl[CTL_EXP0 + ((L + 1) & 1)] = static_cast<unsigned long long>(explored);
+        a.ctl[CTL_DCNT0 + (L & 1)] = 0;  // read by compact(L-1); counts level L+2 next
+    }
+    if (a.ctl[CTL_DIR0 + L % 3] == DIR_BU) {
+        // Back top-down when level L+1's edges * beta are below the edges
+        // still unexplored after it: a bottom-up level rescans those, while
+        // a nearly finished search is cheaper to end bottom-up than to turn
+        // the bitmap into a queue.
+        const unsigned long long nx = a.ctl[CTL_RING0 + (L + 1) % 3];
+        const uint32_t F1 = static_cast<uint32_t>(nx >> 32), E1 = static_cast<uint32_t>(nx);
+        const int64_t unexplored_after = a.m - explored - static_cast<int64_t>(E1);
+        const bool to_top_down = !bu_next || static_cast<int64_t>(E1) * a.beta < unexplored_after;
+        if (F1 == 0 || !to_top_down) return;  // empty, or level L+1 stays bottom-up
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.ctl[CTL_REP0 + (L + 1) % 3], REP_QUEUE);
+        for (uint32_t c = blockIdx.x; c < a.nchunks; c += gridDim.x) {
+            compact_chunk(a, L, c, true);
+            __syncthreads();
+        }
+        return;
+    }
+    if (E <= a.dense_edges) return;  // sparse level: expand() queued level L+1
+    // Dense top-down level: expand(L) counted level L+1.  If level L+1 will
+    // go bottom-up (the test choose_dir(L+1) will make), only its bitmap is
+    // needed; otherwise the queue too.
+    const unsigned long long nx = a.ctl[CTL_DCNT0 + ((L + 1) & 1)];
+    const uint32_t F1 = static_cast<uint32_t>(nx >> 32), E1 = static_cast<uint32_t>(nx);
+    if (bu_next && a.bottom_up && F1 > 0 && go_bottom_up(a, explored, E1)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            a.ctl[CTL_RING0 + (L + 1) % 3] = nx;
+            a.ctl[CTL_REP0 + (L + 1) % 3] = REP_BITMAP;
+        }
+        uint32_t* fout = a.fb[(L + 1) & 1];
+        for (uint32_t i = blockIdx.x * kCompactBlock + threadIdx.x; i < a.nwords; i += gridDim.x * kCompactBlock) {
+            const uint32_t vw = a.vis[i];
+            fout[i] = vw & ~a.prev[i];
+            a.prev[i] = vw;
+        }
+        return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl[CTL_REP0 + (L + 1) % 3] = REP_QUEUE | REP_BITMAP;
     for (uint32_t c = blockIdx.x; c < a.nchunks; c += gridDim.x) {
-        compact_chunk(a, L, c);
+        compact_chunk(a, L, c, false);
         __syncthreads();
     }
 }
 
-// levels = -1, parents = -1, bitmaps = 0.
+// levels = -1, parents = -1; visited and queued bitmaps = the isolated
+// vertices (symmetric graphs: nothing can reach them) or 0; frontier bitmaps
+// = 0 when bottom-up levels may run.
 __global__ void bfs_clear_kernel(BfsArgs a) {
+    int32_t* const levels = a.p->levels;
+    int32_t* const parents = a.p->parents;
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n; i += stride) {
-        a.levels[i] = -1;
-        if (a.parents) a.parents[i] = -1;
+        levels[i] = -1;
+        if (parents) parents[i] = -1;
         if (i < a.nwords) {
-            a.vis[i] = 0u;
-            a.prev[i] = 0u;
+            const uint32_t w = a.iso ? a.iso[i] : 0u;
+            a.vis[i] = w;
+            a.prev[i] = w;
+            if (a.bottom_up) {
+                a.fb[0][i] = 0u;
+                a.fb[1][i] = 0u;
+            }
         }
     }
 }
 
 // Level 0: the source alone (claimed, level 0, parent = itself).
 __global__ void bfs_seed_kernel(BfsArgs a) {
-    const int32_t s = a.source;
+    const int32_t s = a.p->source;
+    int32_t* const levels = a.p->levels;
+    int32_t* const parents = a.p->parents;
     const int64_t rb = a.offsets[s];
     const uint32_t d = static_cast<uint32_t>(a.offsets[s + 1] - rb);
     if (threadIdx.x == 0) {
         const uint32_t bit = 1u << (static_cast<uint32_t>(s) & 31u);
-        a.vis[static_cast<uint32_t>(s) >> 5] = bit;
-        a.prev[static_cast<uint32_t>(s) >> 5] = bit;
-        a.levels[s] = 0;
-        if (a.parents) a.parents[s] = s;
+        a.vis[static_cast<uint32_t>(s) >> 5] |= bit;
+        a.prev[static_cast<uint32_t>(s) >> 5] |= bit;
+        levels[s] = 0;
+        if (parents) parents[s] = s;
         a.q_rb[0][0] = rb;
         a.q_eoff[0][0] = 0;
         a.q_v[0][0] = s;
         for (int i = 0; i < CTL_WORDS; ++i) a.ctl[i] = 0;
         a.ctl[CTL_RING0] = d ? ((1ull << 32) | d) : 0ull;
         a.ctl[CTL_NREACH] = d ? 0 : 1;  // expand(0) counts a source with edges
+        a.ctl[CTL_REP0] = REP_QUEUE;
+        a.ctl[CTL_BUFIRST] = ~0ull;
     }
     for (uint64_t tt = threadIdx.x; tt * a.tile_edges < d; tt += blockDim.x) a.tile_start[0][tt] = 0;
 }
 
-using ExpandFn = void (*)(BfsArgs, int);
+using ExpandFn = void (*)(BfsArgs, int, int, int);
 
 struct KernelSet {
     int block, k;
@@ -477,12 +777,24 @@ DeviceGraph::DeviceGraph(const HostGraph& g, int dev) : device(dev), n(g.n), m(g
     GG_HIP(hipGetDeviceProperties(&prop, device));
     num_cus = prop.multiProcessorCount;
     cfg.dense_edges = default_dense_edges();
+    symmetric = is_symmetric(g);
     try {
         GG_HIP(hipMalloc(&offsets, sizeof(int64_t) * (n + 1)));
         GG_HIP(hipMalloc(&cols, sizeof(int32_t) * std::max<int64_t>(m, 1)));
         graph_bytes = sizeof(int64_t) * (n + 1) + sizeof(int32_t) * std::max<int64_t>(m, 1);
         GG_HIP(hipMemcpy(offsets, g.offsets.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
         if (m) GG_HIP(hipMemcpy(cols, g.cols.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice));
+        if (symmetric) {
+            // Degree-0 vertices of a symmetric graph have no in-edges either:
+            // the searches mark them visited up front.
+            const int64_t nwords = (n + 31) / 32;
+            std::vector<uint32_t> h(nwords, 0u);
+            for (int64_t v = 0; v < n; ++v)
+                if (g.offsets[v + 1] == g.offsets[v]) h[v >> 5] |= 1u << (v & 31);
+            GG_HIP(hipMalloc(&iso, sizeof(uint32_t) * nwords));
+            graph_bytes += sizeof(uint32_t) * nwords;
+            GG_HIP(hipMemcpy(iso, h.data(), sizeof(uint32_t) * nwords, hipMemcpyHostToDevice));
+        }
         runner = std::make_unique<Runner>(*this);
     } catch (...) {
         this->~DeviceGraph();
@@ -495,8 +807,10 @@ DeviceGraph::~DeviceGraph() {
     runner.reset();
     if (offsets) (void)hipFree(offsets);
     if (cols) (void)hipFree(cols);
+    if (iso) (void)hipFree(iso);
     offsets = nullptr;
     cols = nullptr;
+    iso = nullptr;
 }
 
 Runner::Runner(DeviceGraph& graph) : g(graph), device(graph.device), n(graph.n), m(graph.m) {
@@ -532,11 +846,13 @@ Runner::~Runner() {
 }
 
 void Runner::free_workspace() {
-    void* ptrs[] = {ws.vis,    ws.prev,    ws.q_rb[0],  ws.q_rb[1],       ws.q_eoff[0],    ws.q_eoff[1],
-                    ws.q_v[0], ws.q_v[1],  ws.ctl,      ws.levels,        ws.parents,      ws.tile_start[0],
-                    ws.tile_start[1]};
+    if (ws.graph_exec) (void)hipGraphExecDestroy(ws.graph_exec);
+    void* ptrs[] = {ws.vis,    ws.prev,    ws.fb[0],  ws.fb[1],      ws.q_rb[0],       ws.q_rb[1],
+                    ws.q_eoff[0], ws.q_eoff[1], ws.q_v[0], ws.q_v[1], ws.ctl, ws.levels, ws.parents,
+                    ws.tile_start[0], ws.tile_start[1], ws.params};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    if (ws.h_params) (void)hipHostFree(ws.h_params);
     ws = BfsWorkspace{};
 }
 
@@ -544,7 +860,9 @@ void Runner::ensure_workspace() {
     const int T = g.cfg.block * g.cfg.edges_per_thread;
     const int64_t tile_cap = (m + T - 1) / T + 2;
     if (ws.ctl && ws.tile_edges == T) return;
-    if (ws.ctl) {  // only the tile index depends on T
+    if (ws.ctl) {  // only the tile index depends on T (and the captured sequence uses it)
+        if (ws.graph_exec) (void)hipGraphExecDestroy(ws.graph_exec);
+        ws.graph_exec = nullptr;
         (void)hipFree(ws.tile_start[0]);
         (void)hipFree(ws.tile_start[1]);
         ws.bytes -= 2 * sizeof(uint32_t) * ws.tile_cap;
@@ -566,6 +884,8 @@ void Runner::ensure_workspace() {
     try {
         alloc(&ws.vis, sizeof(uint32_t) * nwords);
         alloc(&ws.prev, sizeof(uint32_t) * nwords);
+        alloc(&ws.fb[0], sizeof(uint32_t) * nwords);
+        alloc(&ws.fb[1], sizeof(uint32_t) * nwords);
         for (int i = 0; i < 2; ++i) {
             alloc(&ws.q_rb[i], sizeof(int64_t) * n);
             alloc(&ws.q_eoff[i], sizeof(uint32_t) * n);
@@ -575,6 +895,8 @@ void Runner::ensure_workspace() {
         alloc(&ws.ctl, sizeof(unsigned long long) * kCtlWords);
         alloc(&ws.levels, sizeof(int32_t) * n);
         alloc(&ws.parents, sizeof(int32_t) * n);
+        alloc(&ws.params, sizeof(BfsParams));
+        GG_HIP(hipHostMalloc(reinterpret_cast<void**>(&ws.h_params), sizeof(BfsParams), hipHostMallocDefault));
     } catch (...) {
         free_workspace();
         throw;
@@ -600,8 +922,9 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     a.cols = g.cols;
     a.vis = ws.vis;
     a.prev = ws.prev;
-    a.levels = d_levels;
-    a.parents = d_parents;
+    a.fb[0] = ws.fb[0];
+    a.fb[1] = ws.fb[1];
+    a.iso = g.iso;
     for (int i = 0; i < 2; ++i) {
         a.q_rb[i] = ws.q_rb[i];
         a.q_eoff[i] = ws.q_eoff[i];
@@ -609,13 +932,16 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
         a.tile_start[i] = ws.tile_start[i];
     }
     a.ctl = ws.ctl;
+    a.p = ws.params;
     a.n = n;
     a.m = m;
     a.nwords = static_cast<uint32_t>((n + 31) / 32);
     a.nchunks = static_cast<uint32_t>(ws.nchunks);
     a.dense_edges = static_cast<uint32_t>(std::min<int64_t>(g.cfg.dense_edges, 0xffffffffll));
     a.tile_edges = static_cast<uint32_t>(g.cfg.block * g.cfg.edges_per_thread);
-    a.source = source;
+    a.bottom_up = (g.symmetric && g.cfg.direction_opt) ? 1u : 0u;
+    a.alpha = static_cast<uint32_t>(g.cfg.alpha);
+    a.beta = static_cast<uint32_t>(g.cfg.beta);
 
     int occ = 0;
     GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(ks->expand), g.cfg.block, 0));
@@ -624,23 +950,72 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     const int default_per_cu = std::max(1, std::min(occ, 16 / waves_per_block));
     const int per_cu = g.cfg.blocks_per_cu > 0 ? std::min(g.cfg.blocks_per_cu, occ) : default_per_cu;
     const uint32_t grid = static_cast<uint32_t>(per_cu * g.num_cus);
-    // compact(): chunks are independent; 4 workgroups of 256 per CU.
-    const uint32_t cgrid = static_cast<uint32_t>(std::min<int64_t>(ws.nchunks, int64_t(g.num_cus) * 4));
+    // compact(): chunks are independent; up to 2 workgroups of 1024 per CU.
+    const uint32_t cgrid = static_cast<uint32_t>(std::min<int64_t>(ws.nchunks, int64_t(g.num_cus) * 2));
     const uint32_t clear_grid = static_cast<uint32_t>(std::min<int64_t>((n + 255) / 256, int64_t(g.num_cus) * 8));
+    // bottom_up(): one wave per 64 vertices, up to 2 workgroups of 16 waves per CU.
+    const uint32_t bgrid = static_cast<uint32_t>(std::min<int64_t>((n + 1023) / 1024, int64_t(g.num_cus) * 2));
 
+    // bottom_up(L) is launched for the levels [bu_lo, bu_hi] learned from
+    // earlier searches of this graph; other levels run top-down.
+    auto bu_at = [&](int L) { return a.bottom_up && L >= bu_lo && L <= bu_hi; };
     auto launch_level = [&](int L) {
-        hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(g.cfg.block), 0, stream, a, L);
-        hipLaunchKernelGGL(bfs_compact_kernel, dim3(cgrid), dim3(kCompactBlock), 0, stream, a, L);
+        const int bu = bu_at(L) ? 1 : 0, bu1 = bu_at(L + 1) ? 1 : 0, bu2 = bu_at(L + 2) ? 1 : 0;
+        hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(g.cfg.block), 0, stream, a, L, bu, bu1);
+        if (bu) hipLaunchKernelGGL(bfs_bottom_up_kernel, dim3(bgrid), dim3(kBuBlock), 0, stream, a, L);
+        hipLaunchKernelGGL(bfs_compact_kernel, dim3(cgrid), dim3(kCompactBlock), 0, stream, a, L, bu1, bu2);
+    };
+    // The call's parameters go to device memory first (pinned source buffer:
+    // the copy reads it when the stream reaches it, also inside a graph).
+    ws.h_params->source = source;
+    ws.h_params->pad = 0;
+    ws.h_params->levels = d_levels;
+    ws.h_params->parents = d_parents;
+    auto launch_prologue = [&]() {
+        GG_HIP(hipMemcpyAsync(ws.params, ws.h_params, sizeof(BfsParams), hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(bfs_clear_kernel, dim3(clear_grid), dim3(256), 0, stream, a);
+        hipLaunchKernelGGL(bfs_seed_kernel, dim3(1), dim3(256), 0, stream, a);
     };
 
-    GG_HIP(hipEventRecord(ev_start, stream));
-    hipLaunchKernelGGL(bfs_clear_kernel, dim3(clear_grid), dim3(256), 0, stream, a);
-    hipLaunchKernelGGL(bfs_seed_kernel, dim3(1), dim3(256), 0, stream, a);
-    GG_HIP(hipGetLastError());
-    GG_HIP(hipEventRecord(ev_kernel, stream));
     int L = 0;
+    int last_slot = 0;  // h_ctl slot holding the final control block
+    auto enqueue = [&](int count, int slot) {
+        for (int i = 0; i < count; ++i) launch_level(L++);
+        GG_HIP(hipGetLastError());
+        GG_HIP(hipEventRecord(ev_stop, stream));
+        GG_HIP(hipMemcpyAsync(h_ctl + slot * kCtlWords, ws.ctl, sizeof(unsigned long long) * kCtlWords,
+                              hipMemcpyDeviceToHost, stream));
+        GG_HIP(hipEventRecord(ev_batch[slot], stream));
+        last_slot = slot;
+    };
+    auto done = [&](int slot) {
+        GG_HIP(hipEventSynchronize(ev_batch[slot]));
+        return h_ctl[slot * kCtlWords + CTL_DONE] != 0 || h_ctl[slot * kCtlWords + CTL_ERR] != 0;
+    };
+    // Deeper than the first batch: batches of doubling size (at most 64
+    // levels) are pipelined -- the done flag of batch b-1 is checked while
+    // batch b runs.  Extra levels are no-ops on an empty frontier.
+    auto continue_levels = [&]() {
+        if (done(last_slot)) return;
+        int batch = 8;
+        int slot = last_slot ^ 1;
+        enqueue(batch, slot);
+        for (;;) {
+            if (L > n + 64) fail(GB_EDEVICE, "internal error: more BFS levels than vertices");
+            batch = std::min(batch * 2, 64);
+            slot ^= 1;
+            enqueue(batch, slot);
+            if (done(slot ^ 1)) break;
+        }
+    };
+
+    bool timed_traversal = true;  // ev_kernel marks the end of the clears
     if (g.cfg.mode == GB_MODE_LEVEL_LOOP) {
         // One level at a time; the host reads the next frontier size.
+        GG_HIP(hipEventRecord(ev_start, stream));
+        launch_prologue();
+        GG_HIP(hipGetLastError());
+        GG_HIP(hipEventRecord(ev_kernel, stream));
         for (;; ++L) {
             if (L > n) fail(GB_EDEVICE, "internal error: more BFS levels than vertices");
             launch_level(L);
@@ -655,55 +1030,135 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
                 break;
             }
         }
+        GG_HIP(hipEventRecord(ev_stop, stream));
+        GG_HIP(hipMemcpyAsync(h_ctl, ws.ctl, sizeof(unsigned long long) * kCtlWords, hipMemcpyDeviceToHost,
+                              stream));
+        GG_HIP(hipEventRecord(ev_batch[0], stream));
+        GG_HIP(hipEventSynchronize(ev_batch[0]));
     } else {
         // Levels are queued ahead with no host synchronisation.  The first
-        // batch covers the previous BFS's depth + 1 (one empty level ends the
-        // search); if the search is deeper, batches of doubling size (at most
-        // 64 levels) are pipelined: the done flag of batch b-1 is checked
-        // while batch b runs.  Extra levels are no-ops on an empty frontier.
-        auto enqueue = [&](int count, int slot) {
-            for (int i = 0; i < count; ++i) launch_level(L++);
-            GG_HIP(hipGetLastError());
-            GG_HIP(hipMemcpyAsync(h_ctl + slot * kCtlWords, ws.ctl, sizeof(unsigned long long) * kCtlWords,
-                                  hipMemcpyDeviceToHost, stream));
-            GG_HIP(hipEventRecord(ev_batch[slot], stream));
-        };
-        auto done = [&](int slot) {
-            GG_HIP(hipEventSynchronize(ev_batch[slot]));
-            return h_ctl[slot * kCtlWords + CTL_DONE] != 0 || h_ctl[slot * kCtlWords + CTL_ERR] != 0;
-        };
-        enqueue(last_num_levels > 0 ? last_num_levels + 1 : 8, 0);
-        if (!done(0)) {
-            int batch = 8;
-            enqueue(batch, 1);
-            for (int b = 2;; ++b) {
-                if (L > n + 64) fail(GB_EDEVICE, "internal error: more BFS levels than vertices");
-                batch = std::min(batch * 2, 64);
-                enqueue(batch, b & 1);
-                if (done((b - 1) & 1)) break;
+        // batch covers the depth of recent searches + 1 (one empty level ends
+        // the search).  Each batch ends with a snapshot of the control block;
+        // the last batch's snapshot is the final state, so the common case
+        // (the first batch reaches the end) needs one synchronisation.
+        const int first = std::max(depth_hint, 2);
+        if (g.cfg.mode == GB_MODE_GRAPH) {
+            // The parameter upload, the clears and the first levels are one
+            // captured hipGraph, replayed while the depth, the bottom-up
+            // window and the kernel shape stay the same.
+            const int D = std::min(first, kMaxCapturedLevels);
+            const bool reuse = ws.graph_exec && ws.g_depth == D && ws.g_lo == bu_lo && ws.g_hi == bu_hi &&
+                               ws.g_block == g.cfg.block && ws.g_k == g.cfg.edges_per_thread &&
+                               ws.g_dense == int64_t(a.dense_edges) && ws.g_bu == int(a.bottom_up) &&
+                               ws.g_alpha == int(a.alpha) && ws.g_beta == int(a.beta);
+            if (!reuse) {
+                if (ws.graph_exec) (void)hipGraphExecDestroy(ws.graph_exec);
+                ws.graph_exec = nullptr;
+                hipGraph_t captured = nullptr;
+                GG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                try {
+                    launch_prologue();
+                    for (int i = 0; i < D; ++i) launch_level(i);
+                    GG_HIP(hipMemcpyAsync(h_ctl, ws.ctl, sizeof(unsigned long long) * kCtlWords,
+                                          hipMemcpyDeviceToHost, stream));
+                } catch (...) {
+                    hipGraph_t dummy = nullptr;
+                    (void)hipStreamEndCapture(stream, &dummy);
+                    if (dummy) (void)hipGraphDestroy(dummy);
+                    throw;
+                }
+                GG_HIP(hipStreamEndCapture(stream, &captured));
+                hipError_t e = hipGraphInstantiate(&ws.graph_exec, captured, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(captured);
+                GG_HIP(e);
+                ws.g_depth = D;
+                ws.g_lo = bu_lo;
+                ws.g_hi = bu_hi;
+                ws.g_block = g.cfg.block;
+                ws.g_k = g.cfg.edges_per_thread;
+                ws.g_dense = int64_t(a.dense_edges);
+                ws.g_bu = int(a.bottom_up);
+                ws.g_alpha = int(a.alpha);
+                ws.g_beta = int(a.beta);
             }
+            GG_HIP(hipEventRecord(ev_start, stream));
+            GG_HIP(hipGraphLaunch(ws.graph_exec, stream));
+            GG_HIP(hipEventRecord(ev_stop, stream));
+            GG_HIP(hipEventRecord(ev_batch[0], stream));
+            L = D;
+            last_slot = 0;
+            timed_traversal = false;
+        } else {
+            GG_HIP(hipEventRecord(ev_start, stream));
+            launch_prologue();
+            GG_HIP(hipGetLastError());
+            GG_HIP(hipEventRecord(ev_kernel, stream));
+            enqueue(first, 0);
         }
+        continue_levels();
+        GG_HIP(hipEventSynchronize(ev_batch[last_slot]));
     }
-    GG_HIP(hipEventRecord(ev_stop, stream));
-    unsigned long long fin[kCtlWords];
-    GG_HIP(hipMemcpyAsync(fin, ws.ctl, sizeof(fin), hipMemcpyDeviceToHost, stream));
-    GG_HIP(hipStreamSynchronize(stream));
+    const unsigned long long* fin = h_ctl + last_slot * kCtlWords;
     if (fin[CTL_ERR] & ERR_TILE) fail(GB_EDEVICE, "internal error: inconsistent frontier tile index");
     if (fin[CTL_ERR] & ERR_RANGE) fail(GB_EDEVICE, "internal error: frontier entry or column index out of range");
     if (!fin[CTL_DONE]) fail(GB_EDEVICE, "internal error: BFS did not reach an empty frontier");
     last_num_levels = static_cast<int>(fin[CTL_MAXLEV]) + 1;
+    last_bu_first = fin[CTL_BULAST] >= fin[CTL_BUFIRST] && fin[CTL_BUFIRST] != ~0ull ? int(fin[CTL_BUFIRST]) : -1;
+    last_bu_last = last_bu_first >= 0 ? int(fin[CTL_BULAST]) : -1;
+    learn_from_search(static_cast<int64_t>(fin[CTL_MAXE]) > static_cast<int64_t>(a.dense_edges));
     float ms = 0.f;
     GG_HIP(hipEventElapsedTime(&ms, ev_start, ev_stop));
     last_ms = ms;
     if (st) {
-        float kms = 0.f;
-        GG_HIP(hipEventElapsedTime(&kms, ev_kernel, ev_stop));
+        float kms = ms;
+        if (timed_traversal) GG_HIP(hipEventElapsedTime(&kms, ev_kernel, ev_stop));
         st->ms = ms;
         st->kernel_ms = kms;
         st->num_levels = last_num_levels;
         st->reserved = 0;
         st->n_reached = static_cast<int64_t>(fin[CTL_NREACH]);
         st->m_reached = static_cast<int64_t>(fin[CTL_MREACH]);
+    }
+}
+
+// Launch hints for the next search of this graph, from the one just done:
+//  * depth_hint: levels queued before the first done check -- the deepest
+//    recent search + 1, lowered after 16 shallower searches in a row;
+//  * bottom-up window: grown to cover the levels this search went bottom-up
+//    (the first learned window replaces the default); after two searches in
+//    a row with only small frontiers (deep, thin graphs) no bottom-up
+//    launches at all, until a search has a dense level again.
+// Both only shape launches; any level without a bottom-up launch runs top-down.
+void Runner::learn_from_search(bool had_dense_level) {
+    const int want = last_num_levels + 1;
+    if (want >= depth_hint) {
+        depth_hint = want;
+        short_streak = 0;
+    } else if (++short_streak >= 16) {
+        depth_hint = want;
+        short_streak = 0;
+    }
+    if (last_bu_first >= 0) {
+        thin_streak = 0;
+        if (bu_learned && bu_hi >= bu_lo) {
+            bu_lo = std::min(bu_lo, last_bu_first);
+            bu_hi = std::max(bu_hi, last_bu_last + 2);
+        } else {
+            bu_lo = std::max(1, last_bu_first - 1);
+            bu_hi = last_bu_last + 2;
+        }
+        bu_lo = std::max(1, std::min(bu_lo, last_bu_first - 1));  // one level of margin below
+        bu_learned = true;
+    } else if (had_dense_level) {
+        thin_streak = 0;
+        if (bu_hi < bu_lo) {  // re-arm the default window
+            bu_lo = 1;
+            bu_hi = kDefaultBuHigh;
+            bu_learned = false;
+        }
+    } else if (++thin_streak >= 2) {
+        bu_lo = 1;
+        bu_hi = 0;
     }
 }
 

@@ -3,6 +3,7 @@
 // returns the status; nothing throws across the ABI and nothing exits.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -158,6 +159,23 @@ int gb_graph_csr(const gb_graph* g, const int64_t** offsets, const int32_t** col
     });
 }
 
+int gb_graph_degree_buckets(gb_graph* g, int64_t* offsets, const int32_t** vertices) {
+    return guarded([&] {
+        need(g, "null graph");
+        static_assert(GB_DEGREE_BUCKETS == gg::kDegreeBuckets, "bucket count");
+        if (g->g.bucket_offsets.empty()) gg::compute_degree_buckets(g->g);
+        if (offsets) std::copy(g->g.bucket_offsets.begin(), g->g.bucket_offsets.end(), offsets);
+        if (vertices) *vertices = g->g.bucket_vertices.data();
+    });
+}
+
+int gb_graph_is_symmetric(const gb_graph* g, int* symmetric) {
+    return guarded([&] {
+        need(g && symmetric, "null argument");
+        *symmetric = gg::is_symmetric(g->g) ? 1 : 0;
+    });
+}
+
 int gb_graph_write_edgelist(const gb_graph* g, const char* path) {
     return guarded([&] {
         need(g && path, "null argument");
@@ -251,7 +269,7 @@ int gb_bfs_config(gb_dgraph* dg, int block, int edges_per_thread, int blocks_per
         need(dg, "null device graph");
         need(gg::bfs_config_supported(block, edges_per_thread), "unsupported block / edges_per_thread combination");
         need(blocks_per_cu >= 0, "blocks_per_cu must be >= 0");
-        need(mode == GB_MODE_ASYNC || mode == GB_MODE_LEVEL_LOOP, "unknown mode");
+        need(mode == GB_MODE_ASYNC || mode == GB_MODE_LEVEL_LOOP || mode == GB_MODE_GRAPH, "unknown mode");
         need(dense_edges >= 0, "dense_edges must be >= 0");
         gg::BfsConfig& c = dg->d->cfg;
         c.block = block;
@@ -259,6 +277,48 @@ int gb_bfs_config(gb_dgraph* dg, int block, int edges_per_thread, int blocks_per
         c.blocks_per_cu = blocks_per_cu;
         c.mode = mode;
         c.dense_edges = dense_edges > 0 ? dense_edges : dg->d->default_dense_edges();
+    });
+}
+
+int gb_bfs_direction_config(gb_dgraph* dg, int enable, int alpha, int beta) {
+    return guarded([&] {
+        need(dg, "null device graph");
+        need(alpha >= 0 && beta >= 0, "alpha and beta must be >= 0");
+        gg::BfsConfig& c = dg->d->cfg;
+        c.direction_opt = enable != 0;
+        if (alpha) c.alpha = alpha;
+        if (beta) c.beta = beta;
+        dg->d->runner->bu_lo = 1;
+        dg->d->runner->bu_hi = gg::Runner::kDefaultBuHigh;
+    });
+}
+
+int gb_bfs_get_direction_config(const gb_dgraph* dg, int* enable, int* alpha, int* beta, int* symmetric) {
+    return guarded([&] {
+        need(dg, "null device graph");
+        const gg::BfsConfig& c = dg->d->cfg;
+        if (enable) *enable = c.direction_opt ? 1 : 0;
+        if (alpha) *alpha = c.alpha;
+        if (beta) *beta = c.beta;
+        if (symmetric) *symmetric = dg->d->symmetric ? 1 : 0;
+    });
+}
+
+int gb_bfs_last_bottom_up(const gb_dgraph* dg, int* first_level, int* last_level) {
+    return guarded([&] {
+        need(dg, "null device graph");
+        if (first_level) *first_level = dg->d->runner->last_bu_first;
+        if (last_level) *last_level = dg->d->runner->last_bu_last;
+    });
+}
+
+int gb_bfs_launch_hints(const gb_dgraph* dg, int* depth, int* bu_lo, int* bu_hi) {
+    return guarded([&] {
+        need(dg, "null device graph");
+        const gg::Runner& r = *dg->d->runner;
+        if (depth) *depth = r.depth_hint;
+        if (bu_lo) *bu_lo = r.bu_lo;
+        if (bu_hi) *bu_hi = r.bu_hi;
     });
 }
 

@@ -45,14 +45,27 @@ struct BfsConfig {
     int block = 256;              // threads per workgroup of expand() (4 waves)
     int edges_per_thread = 4;     // K: a tile is block * K frontier edges
     int blocks_per_cu = 0;        // 0 = at most 16 waves per CU
-    int mode = GB_MODE_ASYNC;
+    int mode = GB_MODE_GRAPH;
     int64_t dense_edges = 0;      // levels with more frontier edges use bitmap compaction
+    bool direction_opt = true;    // bottom-up levels on symmetric graphs
+    int alpha = 14;               // go bottom-up when frontier edges * alpha > unexplored edges
+    int beta = 24;                // back top-down when frontier edges * beta < unexplored edges
+};
+
+// Per-call parameters of a single-source BFS, read by the kernels from
+// device memory so that one captured launch sequence serves every call.
+struct BfsParams {
+    int32_t source;
+    int32_t pad;
+    int32_t* levels;
+    int32_t* parents;  // may be null
 };
 
 // Device workspace of one BFS (reused across calls on the same handle).
 struct BfsWorkspace {
     uint32_t* vis = nullptr;                    // ceil(n/32) words: claimed vertices
     uint32_t* prev = nullptr;                   // ceil(n/32) words: vertices already queued
+    uint32_t* fb[2] = {nullptr, nullptr};       // ceil(n/32) words each: frontier bitmaps by level parity
     int64_t* q_rb[2] = {nullptr, nullptr};      // frontier: first CSR entry of the vertex
     uint32_t* q_eoff[2] = {nullptr, nullptr};   // frontier: exclusive prefix of degrees in the level
     int32_t* q_v[2] = {nullptr, nullptr};       // frontier: vertex id
@@ -60,10 +73,16 @@ struct BfsWorkspace {
     unsigned long long* ctl = nullptr;          // counters ring, stats, flags (see bfs.hip)
     int32_t* levels = nullptr;                  // levels staging for host-pointer calls
     int32_t* parents = nullptr;
+    BfsParams* params = nullptr;                // device
+    BfsParams* h_params = nullptr;              // pinned (read by the captured copy)
     int64_t tile_cap = 0;                       // entries per tile_start buffer
-    int64_t nchunks = 0;                        // compaction chunks of 4096 vertices
+    int64_t nchunks = 0;                        // compaction chunks
     int tile_edges = 0;                         // T the tile_start buffers were sized for
     size_t bytes = 0;
+    // Captured launch sequence (GB_MODE_GRAPH) and what it was captured for.
+    hipGraphExec_t graph_exec = nullptr;
+    int g_depth = 0, g_lo = 0, g_hi = 0, g_block = 0, g_k = 0, g_bu = 0, g_alpha = 0, g_beta = 0;
+    int64_t g_dense = -1;
 };
 
 // Tunables of the batched multi-source kernels.
@@ -123,9 +142,16 @@ struct Runner {
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_kernel = nullptr;
     hipEvent_t ev_batch[2] = {nullptr, nullptr};
-    static constexpr int kCtlWords = 16;
+    static constexpr int kCtlWords = 32;
+    static constexpr int kDefaultBuHigh = 15;
     unsigned long long* h_ctl = nullptr;  // pinned: 2 snapshots of the control block
     int last_num_levels = 0;              // depth of the previous BFS: first batch size
+    int bu_lo = 1, bu_hi = kDefaultBuHigh;  // levels that get a bottom-up launch
+    bool bu_learned = false;                // window learned from a search (else the default)
+    int thin_streak = 0;                    // searches in a row without dense levels
+    int last_bu_first = -1, last_bu_last = -1;  // levels the previous BFS searched bottom-up
+    int depth_hint = 0;                     // levels launched before the first done check
+    int short_streak = 0;                   // searches in a row shallower than depth_hint
     BfsWorkspace ws;
     MsWorkspace ms;
 
@@ -150,6 +176,7 @@ struct Runner {
     size_t bytes() const { return ws.bytes + ms.bytes + ms.levels_bytes; }
 
    private:
+    void learn_from_search(bool had_dense_level);
     void ensure_workspace();
     void free_workspace();
     void ensure_ms_workspace();
@@ -161,6 +188,8 @@ struct DeviceGraph {
     int64_t n = 0, m = 0;
     int64_t* offsets = nullptr;  // n + 1
     int32_t* cols = nullptr;     // m
+    uint32_t* iso = nullptr;     // ceil(n/32) words: degree-0 vertices (symmetric graphs only)
+    bool symmetric = false;      // every edge has its reverse: bottom-up search is valid
     size_t graph_bytes = 0;
     int num_cus = 0;
     BfsConfig cfg;

@@ -87,6 +87,55 @@ void sort_dedup_rows(HostGraph& g, bool drop_self_loops) {
     g.cols.swap(ncols);
 }
 
+void compute_degree_buckets(HostGraph& g) {
+    const int64_t n = g.n;
+    const int workers = host_threads();
+    // Per-worker counts over the contiguous vertex slices of parallel_for
+    // (the same slices in both passes), then a scatter that keeps ids
+    // ascending inside each bucket.
+    std::vector<std::vector<int64_t>> cnt(workers, std::vector<int64_t>(kDegreeBuckets, 0));
+    parallel_for(n, [&](int64_t b, int64_t e, int w) {
+        for (int64_t v = b; v < e; ++v) cnt[w][degree_bucket(g.degree(v))]++;
+    }, workers);
+    g.bucket_offsets.assign(kDegreeBuckets + 1, 0);
+    std::vector<std::vector<int64_t>> start(workers, std::vector<int64_t>(kDegreeBuckets, 0));
+    int64_t pos = 0;
+    for (int b = 0; b < kDegreeBuckets; ++b) {
+        g.bucket_offsets[b] = pos;
+        for (int w = 0; w < workers; ++w) {
+            start[w][b] = pos;
+            pos += cnt[w][b];
+        }
+    }
+    g.bucket_offsets[kDegreeBuckets] = pos;
+    g.bucket_vertices.resize(n);
+    parallel_for(n, [&](int64_t b, int64_t e, int w) {
+        std::vector<int64_t> cur = start[w];
+        for (int64_t v = b; v < e; ++v) g.bucket_vertices[cur[degree_bucket(g.degree(v))]++] = static_cast<int32_t>(v);
+    }, workers);
+}
+
+bool is_symmetric(const HostGraph& g) {
+    if (g.symmetric) return true;
+    if (g.flags & GB_KEEP_ORDER) return false;  // rows not sorted: not checked
+    std::atomic<bool> ok{true};
+    parallel_for(g.n, [&](int64_t b, int64_t e, int) {
+        for (int64_t v = b; v < e; ++v) {
+            if (!ok.load(std::memory_order_relaxed)) return;
+            for (int64_t i = g.offsets[v]; i < g.offsets[v + 1]; ++i) {
+                const int32_t u = g.cols[i];
+                const int32_t* rb = g.cols.data() + g.offsets[u];
+                const int32_t* re = g.cols.data() + g.offsets[u + 1];
+                if (!std::binary_search(rb, re, static_cast<int32_t>(v))) {
+                    ok.store(false, std::memory_order_relaxed);
+                    return;
+                }
+            }
+        }
+    });
+    return ok.load();
+}
+
 HostGraph build_csr(int64_t n, const int32_t* src, const int32_t* dst, int64_t m, uint32_t flags) {
     if (n < 0 || n > INT32_MAX) fail(GB_EINVAL, "vertex count must lie in [0, 2^31)");
     if (m < 0) fail(GB_EINVAL, "negative edge count");

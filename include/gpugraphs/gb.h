@@ -111,6 +111,18 @@ int  gb_graph_generate_grid(int64_t width, int64_t height, double extra_fraction
 int  gb_graph_info(const gb_graph* g, int64_t* n, int64_t* m);
 /* Borrowed views of row_offset / col_idx (valid until gb_graph_free). */
 int  gb_graph_csr(const gb_graph* g, const int64_t** offsets, const int32_t** cols);
+/* Degree-sorted vertex buckets, computed on the first call: bucket 0 holds the
+ * vertices of degree 0, bucket b >= 1 those with degree in [2^(b-1), 2^b).
+ * Fills offsets[GB_DEGREE_BUCKETS + 1]; bucket b is (*vertices)[offsets[b] ..
+ * offsets[b+1]), ids ascending inside a bucket; *vertices is borrowed (n
+ * entries, valid until gb_graph_free).  Replaces the degree tiers that the
+ * reference computes per frontier entry (gpuBFS_kernels.cu:451-478). */
+#define GB_DEGREE_BUCKETS 34
+int  gb_graph_degree_buckets(gb_graph* g, int64_t* offsets, const int32_t** vertices);
+/* *symmetric = 1 when every edge has its reverse (built with GB_SYMMETRIZE, by
+ * a generator, or found so on sorted rows); such graphs may be searched
+ * bottom-up. */
+int  gb_graph_is_symmetric(const gb_graph* g, int* symmetric);
 /* Writes the graph in the reference text format (one `u v` line per CSR entry). */
 int  gb_graph_write_edgelist(const gb_graph* g, const char* path);
 /* Binary CSR cache (little-endian header + offsets + cols). */
@@ -131,6 +143,8 @@ void gb_device_free(gb_dgraph* dg);
 /* Execution modes for gb_bfs_config. */
 #define GB_MODE_ASYNC       0   /* levels queued ahead in batches, no per-level host sync */
 #define GB_MODE_LEVEL_LOOP  1   /* host reads the frontier size after every level        */
+#define GB_MODE_GRAPH       2   /* as ASYNC, the first levels replayed from one captured
+                                   hipGraph (default)                                      */
 
 /* levels: n int32 (host), -1 = unreached.  parents: nullable, n int32,
  * parent[source] = source, -1 = unreached (any valid BFS tree). */
@@ -148,6 +162,21 @@ int  gb_bfs_config(gb_dgraph* dg, int block, int edges_per_thread, int blocks_pe
                    int mode, int64_t dense_edges);
 int  gb_bfs_get_config(const gb_dgraph* dg, int* block, int* edges_per_thread,
                        int* blocks_per_cu, int* mode, int64_t* dense_edges);
+/* Direction optimisation of gb_bfs (bottom-up levels, Beamer, Asanovic and
+ * Patterson, SC 2012), used on symmetric graphs only: enable (1/0); alpha:
+ * go bottom-up when frontier edges * alpha > unexplored edges (default 14);
+ * beta: back to top-down when frontier edges * beta < the edges still
+ * unexplored after the level (default 24).
+ * alpha or beta 0 keeps the current value.  The reference has no
+ * counterpart (top-down only, gpuBFS.cu:66-100). */
+int  gb_bfs_direction_config(gb_dgraph* dg, int enable, int alpha, int beta);
+int  gb_bfs_get_direction_config(const gb_dgraph* dg, int* enable, int* alpha, int* beta, int* symmetric);
+/* Levels the last gb_bfs on this handle searched bottom-up (-1, -1 if none). */
+int  gb_bfs_last_bottom_up(const gb_dgraph* dg, int* first_level, int* last_level);
+/* Launch hints learned from the searches so far: levels launched before the
+ * first done check, and the levels [bu_lo, bu_hi] that get a bottom-up
+ * launch (bu_hi < bu_lo: none). */
+int  gb_bfs_launch_hints(const gb_dgraph* dg, int* depth, int* bu_lo, int* bu_hi);
 
 /* ---- batched multi-source BFS (no reference counterpart: the reference
  *      runs one source per gpuBFS object, 1D Partition/main.cu:17-35).

@@ -59,7 +59,7 @@ DENSE = [1, 2**32 - 1, 0]
 
 
 @pytest.mark.parametrize("dense", DENSE)
-@pytest.mark.parametrize("mode", [gg.GB_MODE_ASYNC, gg.GB_MODE_LEVEL_LOOP])
+@pytest.mark.parametrize("mode", [gg.GB_MODE_GRAPH, gg.GB_MODE_ASYNC, gg.GB_MODE_LEVEL_LOOP])
 @pytest.mark.parametrize("block,k", CONFIGS)
 def test_rmat_all_kernel_shapes(gpu, mode, block, k, dense):
     G = gg.Graph.rmat(14, seed=41)
@@ -147,6 +147,148 @@ def test_concurrent_handles_on_large_graph(gpu):
     assert len(results) == 4 * 2 * len(srcs)
     for (i, rep, j), lv in results.items():
         np.testing.assert_array_equal(lv, want[j], err_msg=f"handle {i} rep {rep} source {srcs[j]}")
+
+
+# Direction optimisation: (alpha, beta) = defaults, bottom-up as early as
+# possible, back top-down early, bottom-up early and never back, and rarely
+# bottom-up.
+@pytest.mark.parametrize("alpha,beta", [(14, 24), (1000, 24), (14, 2), (1000, 100000), (2, 1000)])
+@pytest.mark.parametrize("scale,seed", [(14, 71), (17, 72)])
+def test_direction_optimising_matches_oracle(gpu, scale, seed, alpha, beta):
+    G = gg.Graph.rmat(scale, seed=seed)
+    dg = gg.DeviceGraph(G, gpu)
+    dg.direction_config(True, alpha, beta)
+    cfg = dg.get_direction_config()
+    assert cfg["symmetric"] and cfg["alpha"] == alpha and cfg["beta"] == beta
+    off, cols = G.offsets, G.cols
+    deg = np.diff(off)
+    rng = np.random.default_rng(seed + alpha)
+    srcs = [int(s) for s in rng.choice(np.nonzero(deg)[0], 6, replace=False)] + [int(np.argmax(deg))]
+    srcs.append(int(np.nonzero(deg == 0)[0][0]))  # isolated source
+    log(f"rmat{scale} alpha={alpha} beta={beta}")
+    bottom_up = []
+    for s in srcs:
+        lv, par, st = dg.bfs(s, parents=True)
+        want, _, ost = O.bfs(off, cols, s)
+        np.testing.assert_array_equal(lv, want, err_msg=f"source {s}")
+        assert O.check_parents(off, cols, s, lv, par) == 0, s
+        assert st["n_reached"] == ost["n_reached"] and st["m_reached"] == ost["m_reached"]
+        assert st["num_levels"] == ost["num_levels"]
+        bottom_up.append(dg.last_bottom_up())
+    if alpha >= 14:
+        assert any(f >= 1 for f, _ in bottom_up), bottom_up  # bottom-up levels did run
+
+
+def test_direction_optimising_on_off_and_modes(gpu):
+    G = gg.Graph.rmat(18, seed=73)
+    dg = gg.DeviceGraph(G, gpu)
+    deg = np.diff(G.offsets)
+    srcs = [int(s) for s in np.random.default_rng(73).choice(np.nonzero(deg)[0], 4, replace=False)]
+    want = {s: O.bfs(G.offsets, G.cols, s)[0] for s in srcs}
+    for enable in (True, False):
+        dg.direction_config(enable)
+        for mode in (gg.GB_MODE_GRAPH, gg.GB_MODE_ASYNC, gg.GB_MODE_LEVEL_LOOP):
+            dg.config(mode=mode)
+            for s in srcs:
+                lv, par, _ = dg.bfs(s, parents=True)
+                np.testing.assert_array_equal(lv, want[s], err_msg=f"enable={enable} mode={mode} source {s}")
+                assert O.check_parents(G.offsets, G.cols, s, lv, par) == 0
+                if not enable:
+                    assert dg.last_bottom_up() == (-1, -1)
+
+
+@pytest.mark.parametrize("block,k", CONFIGS)
+def test_direction_optimising_kernel_shapes_dense_every_level(gpu, block, k):
+    # dense_edges=1: every top-down level goes through compaction and leaves a
+    # frontier bitmap, so bottom-up can start at any level
+    G = gg.Graph.rmat(14, seed=74)
+    dg = gg.DeviceGraph(G, gpu)
+    dg.config(block=block, edges_per_thread=k, dense_edges=1)
+    deg = np.diff(G.offsets)
+    for s in np.random.default_rng(block * k).choice(np.nonzero(deg)[0], 3, replace=False):
+        lv, _, _ = dg.bfs(int(s))
+        np.testing.assert_array_equal(lv, O.bfs(G.offsets, G.cols, int(s))[0])
+
+
+def test_bottom_up_then_back_top_down(gpu):
+    # An R-MAT core joined by one edge to a 300 x 60 grid: the core's big
+    # levels go bottom-up, then the thin grid frontiers with most grid edges
+    # unexplored send the search back top-down (bitmap turned into a queue).
+    core = gg.Graph.rmat(14, seed=77)
+    grid = gg.Graph.grid(300, 60, 0.05, seed=78)
+    nc = core.n
+    cs = np.repeat(np.arange(nc), np.diff(core.offsets)).astype(np.int32)
+    gs = (np.repeat(np.arange(grid.n), np.diff(grid.offsets)) + nc).astype(np.int32)
+    hub = int(np.argmax(np.diff(core.offsets)))
+    src = np.concatenate([cs, gs, [hub, nc]]).astype(np.int32)
+    dst = np.concatenate([core.cols, grid.cols + nc, [nc, hub]]).astype(np.int32)
+    G = gg.Graph.from_edges(nc + grid.n, src, dst)
+    assert G.is_symmetric
+    dg = gg.DeviceGraph(G, gpu)
+    deg = np.diff(G.offsets)
+    srcs = [hub] + [int(s) for s in np.random.default_rng(77).choice(np.nonzero(deg[:nc])[0], 3, replace=False)]
+    switched = False
+    for s in srcs:
+        lv, par, st = dg.bfs(s, parents=True)
+        want, _, ost = O.bfs(G.offsets, G.cols, s)
+        np.testing.assert_array_equal(lv, want, err_msg=f"source {s}")
+        assert O.check_parents(G.offsets, G.cols, s, lv, par) == 0
+        assert st["n_reached"] == ost["n_reached"] and st["m_reached"] == ost["m_reached"]
+        first, last = dg.last_bottom_up()
+        if first >= 1 and last + 2 < st["num_levels"]:
+            switched = True  # bottom-up levels, then top-down to the end
+    assert switched
+
+
+def test_directed_graph_stays_top_down(gpu):
+    # a directed graph must not be searched bottom-up (in-edges != out-edges)
+    rng = np.random.default_rng(75)
+    n = 20000
+    src = rng.integers(0, n, 200000).astype(np.int32)
+    dst = rng.integers(0, n, 200000).astype(np.int32)
+    G = gg.Graph.from_edges(n, src, dst)
+    assert not G.is_symmetric
+    dg = gg.DeviceGraph(G, gpu)
+    assert not dg.get_direction_config()["symmetric"]
+    for s in (0, 17, 999):
+        lv, _, _ = dg.bfs(s)
+        np.testing.assert_array_equal(lv, O.bfs(G.offsets, G.cols, s)[0])
+        assert dg.last_bottom_up() == (-1, -1)
+
+
+def test_deep_grid_with_direction_optimisation(gpu):
+    # thin frontiers: no bottom-up level; the learned window turns the
+    # bottom-up launches off after the first search
+    G = gg.Graph.grid(400, 300, 0.05, seed=76)
+    dg = gg.DeviceGraph(G, gpu)
+    for s in (0, 150 * 400 + 200, G.n - 1):
+        lv, _, st = dg.bfs(s)
+        want, _, ost = O.bfs(G.offsets, G.cols, s)
+        np.testing.assert_array_equal(lv, want)
+        assert st["num_levels"] == ost["num_levels"]
+
+
+def test_graph_replay_serves_any_source_and_buffer(gpu):
+    # The captured launch sequence reads the source and the output pointers
+    # from device memory: replays with different sources, two level buffers,
+    # with and without parents, interleaved with direct-launch searches.
+    G = gg.Graph.rmat(15, seed=79)
+    dg = gg.DeviceGraph(G, gpu)
+    deg = np.diff(G.offsets)
+    srcs = [int(s) for s in np.random.default_rng(79).choice(np.nonzero(deg)[0], 12, replace=False)]
+    bufs = [(gg.DeviceArray(gpu, np.int32, G.n), gg.DeviceArray(gpu, np.int32, G.n)) for _ in range(2)]
+    for i, s in enumerate(srcs):
+        want = O.bfs(G.offsets, G.cols, s)[0]
+        if i % 4 == 3:
+            dg.config(mode=gg.GB_MODE_ASYNC)
+            np.testing.assert_array_equal(dg.bfs(s)[0], want)
+            dg.config(mode=gg.GB_MODE_GRAPH)
+            continue
+        lv, par = bufs[i % 2]
+        dg.bfs_device(s, lv, par if i % 3 == 0 else None)
+        np.testing.assert_array_equal(lv.numpy(), want, err_msg=f"replay {i} source {s}")
+        if i % 3 == 0:
+            assert O.check_parents(G.offsets, G.cols, s, want, par.numpy()) == 0
 
 
 def test_isolated_and_degree_zero_sources(gpu):

@@ -223,3 +223,44 @@ def test_grid_tree_is_the_minimum_spanning_tree_of_its_weights():
     rows = np.repeat(np.arange(n), np.diff(g.offsets))
     got = {(a, b) for a, b in zip(rows.tolist(), g.cols.tolist()) if a < b}
     assert got == want
+
+
+@pytest.mark.parametrize("make", [lambda: gg.Graph.rmat(12, seed=5), lambda: gg.Graph.grid(40, 30, 0.05, seed=2),
+                                  lambda: gg.Graph.erdos_renyi(500, 20, 900, seed=4)])
+def test_degree_buckets(make):
+    # bucket 0 = degree 0, bucket b = degree in [2^(b-1), 2^b); ids ascending
+    # within a bucket; the buckets partition the vertex set.
+    g = make()
+    off, verts = g.degree_buckets()
+    deg = np.diff(g.offsets)
+    assert off[0] == 0 and off[-1] == g.n and (np.diff(off) >= 0).all()
+    assert np.array_equal(np.sort(verts), np.arange(g.n))
+    for b in range(len(off) - 1):
+        vb = verts[off[b]:off[b + 1]]
+        assert (np.diff(vb) > 0).all()
+        d = deg[vb]
+        if b == 0:
+            assert (d == 0).all()
+        else:
+            assert ((d >= 2 ** (b - 1)) & (d < 2 ** b)).all()
+    # closed form of the bucket sizes from the degree histogram
+    want = np.bincount(np.where(deg > 0, np.floor(np.log2(np.maximum(deg, 1))).astype(int) + 1, 0),
+                       minlength=len(off) - 1)
+    assert np.array_equal(np.diff(off), want)
+
+
+def test_symmetry_detection(tmp_path):
+    assert gg.Graph.rmat(10, seed=1).is_symmetric
+    # directed path: not symmetric; the same edges both ways: symmetric (found
+    # on sorted rows without the symmetrize flag)
+    src, dst = np.arange(0, 9, dtype=np.int32), np.arange(1, 10, dtype=np.int32)
+    assert not gg.Graph.from_edges(10, src, dst).is_symmetric
+    both = gg.Graph.from_edges(10, np.concatenate([src, dst]), np.concatenate([dst, src]))
+    assert both.is_symmetric
+    # kept in file order: reported as not symmetric (rows are not checked)
+    kept = gg.Graph.from_edges(10, np.concatenate([src, dst]), np.concatenate([dst, src]), flags=gg.GB_KEEP_ORDER)
+    assert not kept.is_symmetric
+    # one missing reverse edge breaks it
+    s2 = np.concatenate([src, dst[:-1]])
+    d2 = np.concatenate([dst, src[:-1]])
+    assert not gg.Graph.from_edges(10, s2, d2).is_symmetric
