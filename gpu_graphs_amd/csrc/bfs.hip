@@ -646,12 +646,21 @@ __global__ __launch_bounds__(kCompactBlock) void bfs_compact_kernel(BfsArgs a, i
     const uint32_t F = static_cast<uint32_t>(cnt >> 32), E = static_cast<uint32_t>(cnt);
     if (F == 0) return;
     const int64_t explored = static_cast<int64_t>(a.ctl[CTL_EXP0 + (L & 1)]) + E;  // levels <= L
+    const unsigned long long dir = a.ctl[CTL_DIR0 + L % 3];
+    // Level L+1's vertex count is final here on every path: bottom-up and
+    // sparse top-down levels have filled ring[(L+1)%3], a dense level has
+    // counted its winners.  An empty next level ends the search now.
+    const bool dense_td = dir != DIR_BU && E > a.dense_edges;
+    const unsigned long long next_cnt =
+        dense_td ? a.ctl[CTL_DCNT0 + ((L + 1) & 1)] : a.ctl[CTL_RING0 + (L + 1) % 3];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         atomicAdd(&a.ctl[CTL_MREACH], static_cast<unsigned long long>(E));
         a.ctl[CTL_EXP0 + ((L + 1) & 1)] = static_cast<unsigned long long>(explored);
         a.ctl[CTL_DCNT0 + (L & 1)] = 0;  // read by compact(L-1); counts level L+2 next
+        if ((next_cnt >> 32) == 0) a.ctl[CTL_DONE] = 1;
     }
-    if (a.ctl[CTL_DIR0 + L % 3] == DIR_BU) {
+    if ((next_cnt >> 32) == 0) return;
+    if (dir == DIR_BU) {
         // Back top-down when level L+1's edges * beta are below the edges
         // still unexplored after it: a bottom-up level rescans those, while
         // a nearly finished search is cheaper to end bottom-up than to turn
@@ -1130,7 +1139,7 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
 //    launches at all, until a search has a dense level again.
 // Both only shape launches; any level without a bottom-up launch runs top-down.
 void Runner::learn_from_search(bool had_dense_level) {
-    const int want = last_num_levels + 1;
+    const int want = last_num_levels;  // compact() of the last level sets the done flag
     if (want >= depth_hint) {
         depth_hint = want;
         short_streak = 0;
