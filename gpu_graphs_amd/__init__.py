@@ -297,6 +297,20 @@ class DeviceGraph:
         return {"block": b.value, "edges_per_thread": k.value, "use_graph": bool(g.value),
                 "sparse_edges": s.value}
 
+    def batch_direction_config(self, enable=True, alpha=0):
+        check(lib().gb_batch_direction_config(self._h, int(enable), int(alpha)))
+
+    def get_batch_direction_config(self):
+        e, a = C.c_int(), C.c_int()
+        check(lib().gb_batch_get_direction_config(self._h, C.byref(e), C.byref(a)))
+        return {"enable": bool(e.value), "alpha": a.value}
+
+    def batch_last_bottom_up(self):
+        """Number of levels the last bfs_batch ran bottom-up."""
+        v = C.c_int()
+        check(lib().gb_batch_last_bottom_up(self._h, C.byref(v)))
+        return v.value
+
     def bfs_batch(self, sources, stats=True):
         """Multi-source BFS of up to 64 sources: returns (levels[k, n], stats or None)."""
         src = np.ascontiguousarray(sources, dtype=np.int32)

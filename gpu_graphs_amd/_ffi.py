@@ -145,6 +145,9 @@ SIGNATURES = {
     "gb_batch_config": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int64]),
     "gb_batch_get_config": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int64)]),
+    "gb_batch_direction_config": (C.c_int, [P, C.c_int, C.c_int]),
+    "gb_batch_get_direction_config": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "gb_batch_last_bottom_up": (C.c_int, [P, C.POINTER(C.c_int)]),
     "gb_server_create": (C.c_int, [C.POINTER(ServerCfg), PP]),
     "gb_server_add_graph": (C.c_int, [P, P, C.POINTER(C.c_int)]),
     "gb_server_submit": (C.c_int, [P, C.c_int, C.c_int32, C.POINTER(C.c_uint64)]),

@@ -375,6 +375,30 @@ int gb_batch_get_config(const gb_dgraph* dg, int* block, int* edges_per_thread, 
     });
 }
 
+int gb_batch_direction_config(gb_dgraph* dg, int enable, int alpha) {
+    return guarded([&] {
+        need(dg, "null device graph");
+        need(alpha >= 0, "alpha must be >= 0");
+        dg->d->ms_cfg.direction_opt = enable != 0;
+        if (alpha) dg->d->ms_cfg.alpha = alpha;
+    });
+}
+
+int gb_batch_get_direction_config(const gb_dgraph* dg, int* enable, int* alpha) {
+    return guarded([&] {
+        need(dg, "null device graph");
+        if (enable) *enable = dg->d->ms_cfg.direction_opt ? 1 : 0;
+        if (alpha) *alpha = dg->d->ms_cfg.alpha;
+    });
+}
+
+int gb_batch_last_bottom_up(const gb_dgraph* dg, int* levels) {
+    return guarded([&] {
+        need(dg && levels, "null argument");
+        *levels = dg->d->runner->ms.last_bu_levels;
+    });
+}
+
 int gb_exclusive_scan_i32(int device, const int32_t* d_in, int32_t* d_out, int64_t n) {
     return guarded([&] {
         if (n > 0) {

@@ -91,6 +91,8 @@ struct MsConfig {
     int edges_per_thread = 2;
     bool use_graph = true;      // replay one captured hipGraph per batch
     int64_t sparse_edges = 0;   // levels with at most this many frontier edges pass a touched list
+    bool direction_opt = true;  // bottom-up levels on symmetric graphs
+    int alpha = 2;              // bottom-up when frontier edges * alpha > edges not yet fully seen
 };
 
 // Per-batch parameters, read by the kernels from device memory so that one
@@ -106,6 +108,8 @@ struct MsParams {
 struct MsWorkspace {
     unsigned long long* seen = nullptr;   // n: sources that reached each vertex
     unsigned long long* next = nullptr;   // n: sources reaching it in the current level
+    unsigned long long* fmask = nullptr;  // n: sources that reached it in the last scanned level
+    unsigned long long* lv8 = nullptr;    // 8n: one level byte per (vertex, source)
     int64_t* q_rb = nullptr;              // queue of active vertices
     uint32_t* q_eoff = nullptr;
     int32_t* q_v = nullptr;
@@ -123,9 +127,10 @@ struct MsWorkspace {
     int64_t nchunks = 0;
     int tile_edges = 0;
     int last_depth = 0;
+    int last_bu_levels = 0;               // levels the last batch ran bottom-up
     size_t bytes = 0;
     hipGraphExec_t graph_exec = nullptr;  // captured batch sequence
-    int graph_depth = 0, graph_block = 0, graph_kpt = 0;
+    int graph_depth = 0, graph_block = 0, graph_kpt = 0, graph_bu = -1, graph_alpha = 0;
     int64_t graph_sparse = -1;
 };
 

@@ -16,14 +16,24 @@
 //   expand(L): edge-parallel tiles over the queue; for edge (u, v) the bits
 //       mask(u) & ~seen(v) are OR-ed into next(v) with a 64-bit atomicOr.
 // Dense levels post the atomicOr (its result is not needed, so the wave does
-// not stall) and the next update_compact scans all n vertices.  Sparse levels
+// not stall) and the next update_compact scans all n vertices.  On symmetric
+// graphs a big level runs bottom-up instead: every vertex with sources still
+// unseen ORs the frontier masks of its neighbours (fmask, stored by the
+// scanning update_compact) and stops as soon as all its unseen sources are
+// covered; next(v) is then a plain store.  A level goes bottom-up when its
+// frontier edges * alpha exceed the edges of the vertices not yet reached by
+// every source (most vertices' unseen sources all sit one level further).  Sparse levels
 // (few frontier edges) use the returned old value: the edge that turns next(v)
 // from 0 to non-zero appends v to a touched list, and the next update_compact
 // visits only the listed vertices -- a level then costs O(frontier), which is
 // what deep, thin graphs (road networks) need.
-// Level writes: the wave loops over the union of its lanes' new bits and
-// stores level[s][v] with the lanes that have bit s; on a scan of consecutive
-// vertices each store instruction covers one 256-byte run of row s.
+// Level writes are staged as one byte per (vertex, source): each vertex owns
+// a 64-byte line, cleared to 0xFF, in which update_compact sets the bytes of
+// its new sources with a few 64-bit read-modify-writes.  After the last level
+// ms_finish transposes the lines into the k rows of n int32 (0xFF -> -1) with
+// full 256-byte stores.  Levels from 254 up are stored into the rows
+// directly (the wave walks the union of its lanes' new bits) and marked 254
+// in the line, which the transpose leaves alone.
 // Sources, k and the output buffer are read from device memory (MsParams), so
 // the whole sequence (parameter upload, clears, levels) is captured once into
 // a hipGraph per kernel shape and replayed for every batch.
@@ -51,14 +61,18 @@ enum : int {
     MS_MAXLEV = 4,
     MS_TCNT0 = 5,
     MS_TVALID0 = 7,
+    MS_FULLDEG = 9,    // CSR entries of the vertices every source has reached
+    MS_BULEVELS = 10,  // levels run bottom-up
     MS_WORDS = 16
 };
 enum : unsigned long long { MS_ERR_TILE = 1, MS_ERR_RANGE = 2 };
 
-constexpr int kUcBlock = 256;
+constexpr int kUcBlock = 512;
 constexpr int kUcRounds = 16;
 constexpr int kUcChunk = kUcBlock * kUcRounds;
 constexpr int kMaxCapturedLevels = 32;  // deeper searches continue with direct launches
+constexpr uint32_t kMsBuLowDegree = 32;  // bottom-up: longer rows are scanned by the whole wave
+constexpr int kByteLevels = 254;         // levels kept in the byte lines; 254 = stored directly, 255 = none
 constexpr int kMaxLevelBatch = 128;     // levels enqueued between done checks
 
 struct MsArgs {
@@ -66,6 +80,8 @@ struct MsArgs {
     const int32_t* __restrict__ cols;
     unsigned long long* seen;
     unsigned long long* next;
+    unsigned long long* fmask;   // per vertex: sources that reached it in the last scanned level
+    unsigned long long* lv8;     // per vertex: 8 words = one level byte per source
     const MsParams* p;           // sources, k, output rows (device memory)
     int64_t* q_rb;
     uint32_t* q_eoff;
@@ -79,6 +95,8 @@ struct MsArgs {
     uint32_t nchunks;
     uint32_t tile_edges;
     uint32_t sparse_edges;
+    uint32_t bottom_up;          // bottom-up levels allowed (symmetric graph)
+    uint32_t alpha;
 };
 
 template <int BLOCK, int K>
@@ -89,7 +107,7 @@ struct MsTileLds {
     unsigned long long fm[T + 1];
 };
 
-// seen = next = 0; levels = -1 (k rows of n).
+// seen = next = 0; level lines = 0xFF (not reached).
 __global__ void ms_clear_kernel(MsArgs a) {
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
     const int64_t tid0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -97,14 +115,63 @@ __global__ void ms_clear_kernel(MsArgs a) {
         a.seen[i] = 0;
         a.next[i] = 0;
     }
-    int32_t* levels = a.p->levels;
-    const int64_t total = a.n * a.p->k;
-    int4* lv4 = reinterpret_cast<int4*>(levels);
-    // 16-byte stores only on a 16-byte aligned buffer (caller-owned memory).
-    const int64_t n4 = (reinterpret_cast<uintptr_t>(levels) & 15) ? 0 : total / 4;
-    const int4 minus1 = make_int4(-1, -1, -1, -1);
-    for (int64_t i = tid0; i < n4; i += stride) lv4[i] = minus1;
-    for (int64_t i = n4 * 4 + tid0; i < total; i += stride) levels[i] = -1;
+    ulonglong2* l2 = reinterpret_cast<ulonglong2*>(a.lv8);
+    const ulonglong2 ones = make_ulonglong2(~0ull, ~0ull);
+    for (int64_t i = tid0; i < a.n * 4; i += stride) l2[i] = ones;
+}
+
+// Byte i of the result is 0xFF when bit i of b is set (b < 256).
+__device__ __forceinline__ unsigned long long spread_bits_to_bytes(unsigned long long b) {
+    const unsigned long long m = (b * 0x0101010101010101ull) & 0x8040201008040201ull;
+    const unsigned long long nz = (((m & 0x7F7F7F7F7F7F7F7Full) + 0x7F7F7F7F7F7F7F7Full) | m) & 0x8080808080808080ull;
+    return (nz >> 7) * 0xFFull;
+}
+
+// Sets byte s of v's level line to `value` for every bit s of nv.
+__device__ __forceinline__ void set_level_bytes(unsigned long long* lv8, int64_t v, unsigned long long nv,
+                                                unsigned long long value) {
+    unsigned long long* line = lv8 + v * 8;
+    const unsigned long long vb = value * 0x0101010101010101ull;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const unsigned long long b = (nv >> (8 * j)) & 0xFFull;
+        if (b) {
+            const unsigned long long bm = spread_bits_to_bytes(b);
+            line[j] = (line[j] & ~bm) | (vb & bm);
+        }
+    }
+}
+
+// After the last level: level rows [k][n] from the byte lines, a wave per 64
+// vertices, one full 256-byte store per source (byte 255 -> -1; byte 254 was
+// stored directly and is left alone).
+__global__ __launch_bounds__(256) void ms_finish_kernel(MsArgs a) {
+    const int k = a.p->k;
+    int32_t* const levels = a.p->levels;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t ngroups = (static_cast<uint64_t>(a.n) + 63) / 64;
+    const uint64_t wstride = static_cast<uint64_t>(gridDim.x) * (blockDim.x / 64);
+    for (uint64_t g = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6); g < ngroups;
+         g += wstride) {
+        const int64_t v = static_cast<int64_t>(g * 64 + lane);
+        const bool valid = v < a.n;
+        unsigned long long w[8];
+        const ulonglong2* line = reinterpret_cast<const ulonglong2*>(a.lv8 + (valid ? v : 0) * 8);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const ulonglong2 x = valid ? line[j] : make_ulonglong2(~0ull, ~0ull);
+            w[2 * j] = x.x;
+            w[2 * j + 1] = x.y;
+        }
+#pragma unroll
+        for (int s = 0; s < 64; ++s) {
+            if (s < k) {
+                const uint32_t byte = static_cast<uint32_t>(w[s >> 3] >> (8 * (s & 7))) & 0xFFu;
+                if (valid && byte != static_cast<uint32_t>(kByteLevels))
+                    levels[int64_t(s) * a.n + v] = byte == 0xFFu ? -1 : static_cast<int32_t>(byte);
+            }
+        }
+    }
 }
 
 // Level 0: next(source i) |= bit i; each distinct source once on the touched
@@ -134,9 +201,10 @@ __device__ __forceinline__ void write_levels(int32_t* levels, int64_t n, int64_t
     }
 }
 
-// Chunk c of update_compact(L): thread t takes slots c*kUcChunk + r*256 + t
+// Chunk c of update_compact(L): thread t takes slots c*kUcChunk + r*kUcBlock + t
 // (r < 16) of either the vertex range (scan) or the touched list.
-__device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, bool list, uint32_t count) {
+__device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, bool list, uint32_t count,
+                            unsigned long long kmask, unsigned long long& fulldeg) {
     __shared__ unsigned long long s_wave[kUcBlock / 64];
     __shared__ unsigned long long s_excl;
     __shared__ uint32_t s_ok;
@@ -156,20 +224,26 @@ __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, bool list, uint3
         } else if (i < a.n) {
             v = i;
         }
-        unsigned long long nv = 0;
+        unsigned long long nv = 0, sv = 0;
         if (v >= 0) {
             const unsigned long long nx = a.next[v];
             if (nx) {
-                const unsigned long long sv = a.seen[v];
+                sv = a.seen[v];
                 nv = nx & ~sv;
                 a.seen[v] = sv | nv;
                 a.next[v] = 0;
             }
+            a.fmask[v] = nv;  // a scan covers every vertex: the next level may go bottom-up
         }
         newbits[r] = nv;
         verts[r] = static_cast<int32_t>(v);
-        // every lane of the wave takes part in the shuffles of write_levels
-        if (__ballot(nv != 0)) write_levels(levels, a.n, v, nv, L);
+        if (L < kByteLevels) {
+            if (nv) set_level_bytes(a.lv8, v, nv, static_cast<unsigned long long>(L));
+        } else {
+            // every lane of the wave takes part in the shuffles of write_levels
+            if (__ballot(nv != 0)) write_levels(levels, a.n, v, nv, L);
+            if (nv) set_level_bytes(a.lv8, v, nv, static_cast<unsigned long long>(kByteLevels));
+        }
         if (nv) {
             const uint32_t d = static_cast<uint32_t>(a.offsets[v + 1] - a.offsets[v]);
             if (d) {
@@ -177,6 +251,7 @@ __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, bool list, uint3
                 ++kept;
                 edges += d;
             }
+            if (((sv | nv) & kmask) == kmask) fulldeg += d;  // every source has reached v now
         }
     }
     bool anynew = false;
@@ -239,10 +314,92 @@ __global__ __launch_bounds__(kUcBlock) void ms_update_compact_kernel(MsArgs a, i
         return;
     }
     const uint32_t nch = list ? static_cast<uint32_t>((count + kUcChunk - 1) / kUcChunk) : a.nchunks;
+    const int k = a.p->k;
+    const unsigned long long kmask = k >= 64 ? ~0ull : ((1ull << k) - 1);
+    unsigned long long fulldeg = 0;
     for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
-        ms_uc_chunk(a, L, c, list, static_cast<uint32_t>(count));
+        ms_uc_chunk(a, L, c, list, static_cast<uint32_t>(count), kmask, fulldeg);
         __syncthreads();
     }
+    // One atomic per workgroup for the edges of the newly fully seen vertices.
+    __shared__ unsigned long long s_fd[kUcBlock / 64];
+    const unsigned long long w = wave_sum(fulldeg);
+    if ((threadIdx.x & 63) == 0) s_fd[threadIdx.x >> 6] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < kUcBlock / 64; ++i) t += s_fd[i];
+        if (t) atomicAdd(&a.ctl[MS_FULLDEG], t);
+    }
+}
+
+// Level L bottom-up: next(v) = the unseen sources of v found among the
+// frontier masks of its neighbours.  A wave owns 64 consecutive vertices;
+// rows of at most kMsBuLowDegree entries are scanned by their lane, longer
+// rows by the whole wave 64 entries at a time; a scan stops once all of the
+// vertex's unseen sources are covered.
+__device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t ngroups = (static_cast<uint64_t>(a.n) + 63) / 64;
+    const uint64_t wstride = static_cast<uint64_t>(gridDim.x) * (blockDim.x / 64);
+    bool bad = false;
+    for (uint64_t g = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6); g < ngroups;
+         g += wstride) {
+        const int64_t v = static_cast<int64_t>(g * 64 + lane);
+        unsigned long long un = 0;
+        if (v < a.n) un = kmask & ~a.seen[v];
+        if (!wave_ballot(un != 0)) continue;
+        int64_t rb = 0, re = 0;
+        if (un) {
+            rb = a.offsets[v];
+            re = a.offsets[v + 1];
+        }
+        const uint32_t deg = static_cast<uint32_t>(re - rb);
+        unsigned long long acc = 0;
+        if (un && deg <= kMsBuLowDegree) {
+            for (int64_t i = rb; i < re; i += 4) {
+                int32_t u[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) u[j] = (i + j < re) ? a.cols[i + j] : -1;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (u[j] < 0) continue;
+                    if (u[j] >= a.n) {
+                        bad = true;
+                        continue;
+                    }
+                    acc |= a.fmask[u[j]];
+                }
+                if ((acc & un) == un) break;
+            }
+        }
+        uint64_t hm = wave_ballot(un && deg > kMsBuLowDegree);
+        while (hm) {
+            const int hl = __ffsll(static_cast<long long>(hm)) - 1;
+            hm &= hm - 1;
+            const int64_t hrb = __shfl(rb, hl, 64);
+            const int64_t hre = __shfl(re, hl, 64);
+            const unsigned long long hun = __shfl(un, hl, 64);
+            unsigned long long hacc = 0;
+            for (int64_t b0 = hrb; b0 < hre; b0 += 64) {
+                const int64_t i = b0 + lane;
+                unsigned long long x = 0;
+                if (i < hre) {
+                    const int32_t u = a.cols[i];
+                    if (u < 0 || u >= a.n) bad = true;
+                    else x = a.fmask[u];
+                }
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o, 64);
+                hacc |= x;
+                if ((hacc & hun) == hun) break;  // uniform: hacc is the same in every lane
+            }
+            if (static_cast<int>(lane) == hl) acc = hacc;
+        }
+        const unsigned long long nb = acc & un;
+        if (nb) a.next[v] = nb;
+    }
+    if (bad) atomicOr(&a.ctl[MS_ERR], MS_ERR_RANGE);
 }
 
 // Level L, step 2: push the queue's masks along every edge.
@@ -263,7 +420,20 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(MsArgs a, int L) {
     }
     const bool sparse = E <= a.sparse_edges;
     const int npar = (L + 1) & 1;
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl[MS_TVALID0 + npar] = sparse ? 1 : 0;
+    // Bottom-up needs every vertex's frontier mask: only after a scanning
+    // update_compact(L).  MS_FULLDEG is final (update_compact(L) has ended).
+    const bool scanned = a.ctl[MS_TVALID0 + (L & 1)] == 0;
+    const int64_t open_edges = a.m - static_cast<int64_t>(a.ctl[MS_FULLDEG]);
+    const bool bottom_up = a.bottom_up && scanned && static_cast<int64_t>(E) * a.alpha > open_edges;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.ctl[MS_TVALID0 + npar] = (sparse && !bottom_up) ? 1 : 0;
+        if (bottom_up) atomicAdd(&a.ctl[MS_BULEVELS], 1ull);
+    }
+    if (bottom_up) {
+        const int k = a.p->k;
+        ms_bottom_up(a, k >= 64 ? ~0ull : ((1ull << k) - 1));
+        return;
+    }
     const uint32_t ntiles = static_cast<uint32_t>((uint64_t(E) + T - 1) / T);
     const uint32_t lane = threadIdx.x & 63u;
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -423,8 +593,8 @@ bool ms_config_supported(int block, int k) { return find_ms_kernels(block, k) !=
 
 void Runner::free_ms_workspace() {
     if (ms.graph_exec) (void)hipGraphExecDestroy(ms.graph_exec);
-    void* ptrs[] = {ms.seen,       ms.next,    ms.q_rb,   ms.q_eoff,  ms.q_v,     ms.q_fm,
-                    ms.tile_start, ms.touched, ms.ctl,    ms.params,  ms.levels,  ms.partial};
+    void* ptrs[] = {ms.seen,       ms.next,    ms.fmask,  ms.lv8,     ms.q_rb,    ms.q_eoff, ms.q_v,    ms.q_fm,
+                    ms.tile_start, ms.touched, ms.ctl,    ms.params,  ms.levels, ms.partial};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (ms.h_params) (void)hipHostFree(ms.h_params);
@@ -457,6 +627,8 @@ void Runner::ensure_ms_workspace() {
     try {
         alloc(&ms.seen, sizeof(unsigned long long) * n);
         alloc(&ms.next, sizeof(unsigned long long) * n);
+        alloc(&ms.fmask, sizeof(unsigned long long) * n);
+        alloc(&ms.lv8, sizeof(unsigned long long) * 8 * n);
         alloc(&ms.q_rb, sizeof(int64_t) * n);
         alloc(&ms.q_eoff, sizeof(uint32_t) * n);
         alloc(&ms.q_v, sizeof(int32_t) * n);
@@ -485,7 +657,7 @@ void Runner::prepare_batch() {
 }
 
 size_t batch_bytes_estimate(int64_t n_vertices, int64_t m_entries, int k, bool host_levels) {
-    size_t b = sizeof(unsigned long long) * n_vertices * 3 + (sizeof(int64_t) + 12) * n_vertices +
+    size_t b = sizeof(unsigned long long) * n_vertices * 12 + (sizeof(int64_t) + 12) * n_vertices +
                sizeof(uint32_t) * (m_entries / 256 + 2) + 1024 * 64 * 16 + 4096;
     if (host_levels) b += sizeof(int32_t) * n_vertices * k;
     return b;
@@ -509,6 +681,8 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     a.cols = g.cols;
     a.seen = ms.seen;
     a.next = ms.next;
+    a.fmask = ms.fmask;
+    a.lv8 = ms.lv8;
     a.p = ms.params;
     a.q_rb = ms.q_rb;
     a.q_eoff = ms.q_eoff;
@@ -522,6 +696,8 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     a.nchunks = static_cast<uint32_t>(ms.nchunks);
     a.tile_edges = static_cast<uint32_t>(g.ms_cfg.block * g.ms_cfg.edges_per_thread);
     a.sparse_edges = static_cast<uint32_t>(std::min<int64_t>(sparse_cfg, 0xffffffffll));
+    a.bottom_up = (g.symmetric && g.ms_cfg.direction_opt) ? 1u : 0u;
+    a.alpha = static_cast<uint32_t>(g.ms_cfg.alpha);
 
     int occ = 0;
     GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(ks->expand),
@@ -530,7 +706,8 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     const int waves_per_block = (g.ms_cfg.block + 63) / 64;
     const int per_cu = std::max(1, std::min(occ, 16 / waves_per_block));
     const uint32_t grid = static_cast<uint32_t>(per_cu * g.num_cus);
-    const uint32_t ugrid = static_cast<uint32_t>(std::min<int64_t>(ms.nchunks, int64_t(g.num_cus) * 4));
+    const uint32_t ugrid = static_cast<uint32_t>(std::min<int64_t>(ms.nchunks, int64_t(g.num_cus) * 2));
+    const uint32_t fgrid = static_cast<uint32_t>(std::min<int64_t>((n + 255) / 256, int64_t(g.num_cus) * 8));
     const uint32_t cgrid = static_cast<uint32_t>(std::min<int64_t>((n * 16 + 255) / 256 + 1, int64_t(g.num_cus) * 8));
 
     std::memset(ms.h_params, 0, sizeof(MsParams));
@@ -556,7 +733,8 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
         // Everything batch-specific is read from device memory, so the graph
         // is rebuilt only when the kernel shape or the depth hint changes.
         const bool reuse = ms.graph_exec && ms.graph_depth == depth_hint && ms.graph_block == g.ms_cfg.block &&
-                           ms.graph_kpt == g.ms_cfg.edges_per_thread && ms.graph_sparse == a.sparse_edges;
+                           ms.graph_kpt == g.ms_cfg.edges_per_thread && ms.graph_sparse == a.sparse_edges &&
+                           ms.graph_bu == int(a.bottom_up) && ms.graph_alpha == int(a.alpha);
         if (!reuse) {
             if (ms.graph_exec) (void)hipGraphExecDestroy(ms.graph_exec);
             ms.graph_exec = nullptr;
@@ -579,6 +757,8 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
             ms.graph_block = g.ms_cfg.block;
             ms.graph_kpt = g.ms_cfg.edges_per_thread;
             ms.graph_sparse = a.sparse_edges;
+            ms.graph_bu = int(a.bottom_up);
+            ms.graph_alpha = int(a.alpha);
             GG_HIP(hipEventRecord(ev_start, stream));
         }
         GG_HIP(hipGraphLaunch(ms.graph_exec, stream));
@@ -616,6 +796,9 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
             if (done((b - 1) & 1)) break;
         }
     }
+    // All levels have run (later ones are no-ops): level rows from the byte lines.
+    hipLaunchKernelGGL(ms_finish_kernel, dim3(fgrid), dim3(256), 0, stream, a);
+    GG_HIP(hipGetLastError());
     GG_HIP(hipEventRecord(ev_stop, stream));
     unsigned long long fin[kCtlWords];
     GG_HIP(hipMemcpyAsync(fin, ms.ctl, sizeof(fin), hipMemcpyDeviceToHost, stream));
@@ -625,6 +808,7 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
         fail(GB_EDEVICE, "internal error: frontier entry or column index out of range");
     if (!fin[MS_DONE]) fail(GB_EDEVICE, "internal error: batch did not reach an empty frontier");
     ms.last_depth = static_cast<int>(fin[MS_MAXLEV]) + 1;
+    ms.last_bu_levels = static_cast<int>(fin[MS_BULEVELS]);
     float ms_elapsed = 0.f;
     GG_HIP(hipEventElapsedTime(&ms_elapsed, ev_start, ev_stop));
     last_ms = ms_elapsed;

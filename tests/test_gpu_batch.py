@@ -153,3 +153,60 @@ def test_batch_errors(gpu):
     assert e.value.code == -5
     with pytest.raises(gg.GBError):
         dg.batch_config(block=100, edges_per_thread=1)
+
+
+@pytest.mark.parametrize("alpha", [1, 2, 8, 1000])
+@pytest.mark.parametrize("enable", [True, False])
+def test_bottom_up_batches(gpu, enable, alpha):
+    # Bottom-up levels pull frontier masks from neighbours; every row must
+    # equal the oracle's whatever the switch point.
+    G = gg.Graph.rmat(16, seed=81)
+    dg = gg.DeviceGraph(G, gpu)
+    dg.batch_direction_config(enable, alpha)
+    assert dg.get_batch_direction_config() == {"enable": enable, "alpha": alpha}
+    deg = np.diff(G.offsets)
+    rng = np.random.default_rng(alpha)
+    for k in (64, 7):
+        sources = rng.choice(np.nonzero(deg)[0], k, replace=False).astype(np.int32)
+        log(f"rmat16 bottom-up={enable} alpha={alpha} k={k}")
+        check_batch(G, dg, sources)
+        if not enable:
+            assert dg.batch_last_bottom_up() == 0
+        elif alpha >= 2 and k == 64:
+            assert dg.batch_last_bottom_up() >= 1
+
+
+def test_bottom_up_batch_on_mixed_graph_and_directed(gpu):
+    # R-MAT core plus a grid tail joined by one edge: bottom-up big levels,
+    # then thin levels; and a directed graph, where bottom-up must stay off.
+    core = gg.Graph.rmat(13, seed=82)
+    grid = gg.Graph.grid(120, 40, 0.05, seed=83)
+    nc = core.n
+    cs = np.repeat(np.arange(nc), np.diff(core.offsets)).astype(np.int32)
+    gs = (np.repeat(np.arange(grid.n), np.diff(grid.offsets)) + nc).astype(np.int32)
+    hub = int(np.argmax(np.diff(core.offsets)))
+    G = gg.Graph.from_edges(nc + grid.n, np.concatenate([cs, gs, [hub, nc]]),
+                            np.concatenate([core.cols, grid.cols + nc, [nc, hub]]))
+    dg = gg.DeviceGraph(G, gpu)
+    deg = np.diff(G.offsets)
+    sources = np.random.default_rng(82).choice(np.nonzero(deg)[0], 48, replace=False).astype(np.int32)
+    check_batch(G, dg, sources)
+    assert dg.batch_last_bottom_up() >= 1
+    rng = np.random.default_rng(84)
+    n = 5000
+    D = gg.Graph.from_edges(n, rng.integers(0, n, 40000), rng.integers(0, n, 40000))
+    dd = gg.DeviceGraph(D, gpu)
+    check_batch(D, dd, np.arange(0, 64, dtype=np.int32) * 7)
+    assert dd.batch_last_bottom_up() == 0
+
+
+def test_levels_past_the_byte_range(gpu):
+    # A 700-vertex path: levels go past 254, where the batch stores levels
+    # into the rows directly instead of the byte lines.
+    n = 700
+    src = np.arange(n - 1, dtype=np.int32)
+    G = gg.Graph.from_edges(n, np.concatenate([src, src + 1]), np.concatenate([src + 1, src]))
+    dg = gg.DeviceGraph(G, gpu)
+    sources = np.array([0, n - 1, 350, 253, 254, 255, 0], dtype=np.int32)
+    levels = check_batch(G, dg, sources)
+    assert levels[0, n - 1] == n - 1 and levels[2, 0] == 350
