@@ -122,6 +122,11 @@ def single_source(gg, args):
     dg.config(**cfg)
     if args.alpha or args.beta or args.no_bottom_up:
         dg.direction_config(not args.no_bottom_up, args.alpha or 0, args.beta or 0)
+    tuning = None
+    if not (args.no_tune or args.block or args.k or args.alpha or args.beta or args.no_bottom_up):
+        t0 = time.time()
+        tuning = dg.autotune(pick_sources(G.offsets, 8, SEED + 1), reps=2)
+        log(f"autotune ({time.time() - t0:.1f}s): {tuning}")
     d_levels = gg.DeviceArray(0, np.int32, n)
     log(f"uploaded; config {dg.get_config()}")
 
@@ -168,6 +173,7 @@ def single_source(gg, args):
             "scale": args.scale, "edge_factor": args.edge_factor, "abc": list(RMAT_ABC),
             "seed": SEED, "n": n, "m_directed": m, "sources": len(sources), "kernel": dg.get_config(),
             "direction": dg.get_direction_config(),
+            "tuning": tuning,
         },
         "device_gteps": round(edges / dev_s / 1e9, 4),
         "gteps_per_source": {"harmonic_mean": round(len(per_gteps) / sum(1 / g for g in per_gteps), 4),
@@ -192,23 +198,40 @@ def single_source(gg, args):
 
 
 def batched(gg, args):
-    """R-MAT s24: one 64-source batch vs 64 single-source BFS, same GPU."""
+    """R-MAT s24: one 64-source batch vs 64 single-source BFS, same GPU.  Both
+    paths run the configuration the autotuner picks for this graph (tuned on
+    other seeded sources than the timed ones)."""
     t0 = time.time()
     G = gg.Graph.rmat(24, 16, *RMAT_ABC, seed=SEED_BATCH, permute=True)
     n = G.n
     log(f"batched: rmat s24 n={n} m={G.m} in {time.time() - t0:.1f}s")
     sources = pick_sources(G.offsets, 64, SEED_BATCH)
     dg = gg.DeviceGraph(G, 0)
+    single_tuning = None
+    if not args.no_tune:
+        t0 = time.time()
+        single_tuning = dg.autotune(pick_sources(G.offsets, 8, SEED_BATCH + 2), reps=2)
+        log(f"single-source autotune on s24 ({time.time() - t0:.1f}s): {single_tuning}")
     d_rows = gg.DeviceArray(0, np.int32, 64 * n)
     d_lv = gg.DeviceArray(0, np.int32, n)
-    st = dg.bfs_batch_device(sources, d_rows, stats=True)  # warm-up: workspace + graph capture
-    sync(gg)
     reps = args.batch_reps
-    t = time.perf_counter()
-    for _ in range(reps):
-        dg.bfs_batch_device(sources, d_rows)
-    sync(gg)
-    batch_s = (time.perf_counter() - t) / reps
+
+    def time_batch():
+        st = dg.bfs_batch_device(sources, d_rows, stats=True)  # warm-up: workspace + graph capture
+        sync(gg)
+        t = time.perf_counter()
+        for _ in range(reps):
+            dg.bfs_batch_device(sources, d_rows)
+        sync(gg)
+        return st, (time.perf_counter() - t) / reps
+
+    st, batch_s = time_batch()
+    default_s, tuning = batch_s, None
+    if not args.no_tune:
+        t0 = time.time()
+        tuning = dg.batch_autotune(pick_sources(G.offsets, 64, SEED_BATCH + 1), reps=2)
+        log(f"batch autotune ({time.time() - t0:.1f}s): {tuning}")
+        st, batch_s = time_batch()
     for s in sources[:2]:
         dg.bfs_device(int(s), d_lv)
     sync(gg)
@@ -227,6 +250,10 @@ def batched(gg, args):
         "workload": "rmat-s24-ef16, 64 seeded sources in one multi-source BFS",
         "seed": SEED_BATCH, "n": n, "m_directed": G.m, "sources": len(sources),
         "kernel": dg.get_batch_config(),
+        "direction": dg.get_batch_direction_config(),
+        "tuning": tuning,
+        "single_source_tuning": single_tuning,
+        "batch_ms_default_config": round(default_s * 1e3, 3),
         "batch_ms": round(batch_s * 1e3, 3),
         "batched_qps": round(len(sources) / batch_s, 1),
         "batched_gteps": round(edges / batch_s / 1e9, 3),
@@ -360,15 +387,17 @@ def main():
     ap.add_argument("--alpha", type=int, default=None, help="bottom-up switch: frontier edges * alpha > unexplored")
     ap.add_argument("--beta", type=int, default=None, help="back top-down below n / beta frontier vertices")
     ap.add_argument("--no-bottom-up", action="store_true", help="top-down only")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default kernel configuration")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--batch-reps", type=int, default=5)
-    ap.add_argument("--serve-budget", type=int, default=16 << 30, help="device bytes for the batched server")
+    ap.add_argument("--serve-budget", type=int, default=64 << 30,
+                    help="device bytes for the batched server's workspaces and result buffers")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="arrival window of each swept rate")
     ap.add_argument("--serve-plain-queries", type=int, default=96)
-    ap.add_argument("--serve-batch-queries", type=int, default=768)
+    ap.add_argument("--serve-batch-queries", type=int, default=2048)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (byte-counter calibration) and exit")
     args = ap.parse_args()

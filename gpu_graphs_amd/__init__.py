@@ -311,6 +311,38 @@ class DeviceGraph:
         check(lib().gb_batch_last_bottom_up(self._h, C.byref(v)))
         return v.value
 
+    def autotune(self, sources, reps=3):
+        """Tunes bfs() over the given sources (1..64) on the device; the
+        fastest configuration stays on the handle.  Returns the result dict
+        (default_ms, best_ms, tried and the chosen values)."""
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        res = _ffi.TuneResult()
+        check(lib().gb_bfs_autotune(self._h, _ptr(src), src.size, int(reps), C.byref(res)))
+        return res.as_dict()
+
+    def batch_autotune(self, sources, reps=3):
+        """Tunes bfs_batch() on one batch of sources (1..64)."""
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        res = _ffi.TuneResult()
+        check(lib().gb_batch_autotune(self._h, _ptr(src), src.size, int(reps), C.byref(res)))
+        return res.as_dict()
+
+    @staticmethod
+    def _tune_struct(t):
+        res = _ffi.TuneResult()
+        for k, _ in res._fields_:
+            if k in t:
+                setattr(res, k, t[k])
+        return res
+
+    def apply_tuning(self, t):
+        """Applies a dict returned by autotune() (possibly from another process)."""
+        check(lib().gb_bfs_apply_tuning(self._h, C.byref(self._tune_struct(t))))
+
+    def apply_batch_tuning(self, t):
+        """Applies a dict returned by batch_autotune()."""
+        check(lib().gb_batch_apply_tuning(self._h, C.byref(self._tune_struct(t))))
+
     def bfs_batch(self, sources, stats=True):
         """Multi-source BFS of up to 64 sources: returns (levels[k, n], stats or None)."""
         src = np.ascontiguousarray(sources, dtype=np.int32)

@@ -95,6 +95,18 @@ class ServerStats(C.Structure):
         ("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64), ("rejected", C.c_int64),
         ("batches", C.c_int64), ("mean_batch", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
         ("max_ms", C.c_double), ("qps", C.c_double), ("memory_used", C.c_int64), ("memory_budget", C.c_int64),
+        ("workspace_drops", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TuneResult(C.Structure):
+    _fields_ = [
+        ("default_ms", C.c_double), ("best_ms", C.c_double), ("tried", C.c_int32), ("block", C.c_int32),
+        ("edges_per_thread", C.c_int32), ("alpha", C.c_int32), ("beta", C.c_int32), ("low_degree", C.c_int32),
+        ("frontier_edges", C.c_int64),
     ]
 
     def as_dict(self):
@@ -148,6 +160,10 @@ SIGNATURES = {
     "gb_batch_direction_config": (C.c_int, [P, C.c_int, C.c_int]),
     "gb_batch_get_direction_config": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "gb_batch_last_bottom_up": (C.c_int, [P, C.POINTER(C.c_int)]),
+    "gb_bfs_autotune": (C.c_int, [P, P, C.c_int, C.c_int, C.POINTER(TuneResult)]),
+    "gb_batch_autotune": (C.c_int, [P, P, C.c_int, C.c_int, C.POINTER(TuneResult)]),
+    "gb_bfs_apply_tuning": (C.c_int, [P, C.POINTER(TuneResult)]),
+    "gb_batch_apply_tuning": (C.c_int, [P, C.POINTER(TuneResult)]),
     "gb_server_create": (C.c_int, [C.POINTER(ServerCfg), PP]),
     "gb_server_add_graph": (C.c_int, [P, P, C.POINTER(C.c_int)]),
     "gb_server_submit": (C.c_int, [P, C.c_int, C.c_int32, C.POINTER(C.c_uint64)]),

@@ -42,7 +42,7 @@
 //    unvisited vertex looks for a neighbour in the frontier bitmap and stops
 //    at the first one.  A wave owns 64 consecutive vertices (two bitmap
 //    words, so it updates them with plain stores); rows of at most
-//    kBuLowDegree entries are scanned by their lane alone, longer rows by the
+//    `low_degree` entries are scanned by their lane alone, longer rows by the
 //    whole wave 64 entries at a time with a ballot -- the degree tiers of a
 //    thread or a wavefront per vertex.  Isolated vertices are marked visited
 //    up front.  Search goes back top-down when the frontier's edges * beta
@@ -99,7 +99,6 @@ constexpr int kCompactRounds = 16;                          // vertices per thre
 constexpr int kChunk = kCompactBlock * kCompactRounds;      // vertices per compaction chunk
 constexpr int kBuBlock = 1024;
 constexpr int kMaxCapturedLevels = 32;                      // deeper searches continue with direct launches
-constexpr uint32_t kBuLowDegree = 32;                       // longer rows are scanned by the whole wave
 
 struct BfsArgs {
     const int64_t* __restrict__ offsets;
@@ -122,6 +121,7 @@ struct BfsArgs {
     uint32_t tile_edges;        // T of the expand kernel (tile_start granularity)
     uint32_t bottom_up;         // direction optimisation enabled
     uint32_t alpha, beta;
+    uint32_t low_degree;        // bottom-up: longer rows are scanned by the whole wave
 };
 
 // Beamer's test for a frontier of E CSR entries after `explored` entries of
@@ -466,7 +466,7 @@ __global__ __launch_bounds__(kBuBlock) void bfs_bottom_up_kernel(BfsArgs a, int 
         bool found = false;
         int32_t par = -1;
         // Short rows: the lane alone, 4 entries per step.
-        if (unvisited && deg <= kBuLowDegree) {
+        if (unvisited && deg <= a.low_degree) {
             for (int64_t i = rb; i < re && !found; i += 4) {
                 int32_t u[4];
 #pragma unroll
@@ -486,7 +486,7 @@ __global__ __launch_bounds__(kBuBlock) void bfs_bottom_up_kernel(BfsArgs a, int 
             }
         }
         // Long rows: the whole wave, 64 entries per step, first hit wins.
-        uint64_t hm = wave_ballot(unvisited && deg > kBuLowDegree);
+        uint64_t hm = wave_ballot(unvisited && deg > a.low_degree);
         while (hm) {
             const int hl = __ffsll(static_cast<long long>(hm)) - 1;
             hm &= hm - 1;
@@ -951,6 +951,7 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     a.bottom_up = (g.symmetric && g.cfg.direction_opt) ? 1u : 0u;
     a.alpha = static_cast<uint32_t>(g.cfg.alpha);
     a.beta = static_cast<uint32_t>(g.cfg.beta);
+    a.low_degree = static_cast<uint32_t>(g.cfg.low_degree);
 
     int occ = 0;
     GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(ks->expand), g.cfg.block, 0));
@@ -1059,7 +1060,8 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
             const bool reuse = ws.graph_exec && ws.g_depth == D && ws.g_lo == bu_lo && ws.g_hi == bu_hi &&
                                ws.g_block == g.cfg.block && ws.g_k == g.cfg.edges_per_thread &&
                                ws.g_dense == int64_t(a.dense_edges) && ws.g_bu == int(a.bottom_up) &&
-                               ws.g_alpha == int(a.alpha) && ws.g_beta == int(a.beta);
+                               ws.g_alpha == int(a.alpha) && ws.g_beta == int(a.beta) &&
+                               ws.g_low == int(a.low_degree);
             if (!reuse) {
                 if (ws.graph_exec) (void)hipGraphExecDestroy(ws.graph_exec);
                 ws.graph_exec = nullptr;
@@ -1089,6 +1091,7 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
                 ws.g_bu = int(a.bottom_up);
                 ws.g_alpha = int(a.alpha);
                 ws.g_beta = int(a.beta);
+                ws.g_low = int(a.low_degree);
             }
             GG_HIP(hipEventRecord(ev_start, stream));
             GG_HIP(hipGraphLaunch(ws.graph_exec, stream));

@@ -288,8 +288,7 @@ int gb_bfs_direction_config(gb_dgraph* dg, int enable, int alpha, int beta) {
         c.direction_opt = enable != 0;
         if (alpha) c.alpha = alpha;
         if (beta) c.beta = beta;
-        dg->d->runner->bu_lo = 1;
-        dg->d->runner->bu_hi = gg::Runner::kDefaultBuHigh;
+        dg->d->runner->reset_bottom_up_window();
     });
 }
 
@@ -396,6 +395,50 @@ int gb_batch_last_bottom_up(const gb_dgraph* dg, int* levels) {
     return guarded([&] {
         need(dg && levels, "null argument");
         *levels = dg->d->runner->ms.last_bu_levels;
+    });
+}
+
+int gb_bfs_autotune(gb_dgraph* dg, const int32_t* sources, int count, int reps, gb_tune_result* out) {
+    return guarded([&] {
+        need(dg && sources, "null argument");
+        gg::autotune_bfs(*dg->d, sources, count, reps, out);
+    });
+}
+
+int gb_batch_autotune(gb_dgraph* dg, const int32_t* sources, int k, int reps, gb_tune_result* out) {
+    return guarded([&] {
+        need(dg && sources, "null argument");
+        gg::autotune_batch(*dg->d, sources, k, reps, out);
+    });
+}
+
+int gb_bfs_apply_tuning(gb_dgraph* dg, const gb_tune_result* t) {
+    return guarded([&] {
+        need(dg && t, "null argument");
+        need(gg::bfs_config_supported(t->block, t->edges_per_thread), "unsupported block / edges_per_thread combination");
+        need(t->alpha > 0 && t->beta >= 0 && t->low_degree > 0 && t->frontier_edges > 0, "thresholds must be positive");
+        gg::BfsConfig& c = dg->d->cfg;
+        c.block = t->block;
+        c.edges_per_thread = t->edges_per_thread;
+        c.alpha = t->alpha;
+        if (t->beta) c.beta = t->beta;
+        c.low_degree = t->low_degree;
+        c.dense_edges = t->frontier_edges;
+        dg->d->runner->reset_bottom_up_window();
+    });
+}
+
+int gb_batch_apply_tuning(gb_dgraph* dg, const gb_tune_result* t) {
+    return guarded([&] {
+        need(dg && t, "null argument");
+        need(gg::ms_config_supported(t->block, t->edges_per_thread), "unsupported block / edges_per_thread combination");
+        need(t->alpha > 0 && t->low_degree > 0 && t->frontier_edges > 0, "thresholds must be positive");
+        gg::MsConfig& c = dg->d->ms_cfg;
+        c.block = t->block;
+        c.edges_per_thread = t->edges_per_thread;
+        c.alpha = t->alpha;
+        c.low_degree = t->low_degree;
+        c.sparse_edges = t->frontier_edges;
     });
 }
 

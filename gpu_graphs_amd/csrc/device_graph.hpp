@@ -50,6 +50,7 @@ struct BfsConfig {
     bool direction_opt = true;    // bottom-up levels on symmetric graphs
     int alpha = 14;               // go bottom-up when frontier edges * alpha > unexplored edges
     int beta = 24;                // back top-down when frontier edges * beta < unexplored edges
+    int low_degree = 32;          // bottom-up: rows up to this long are scanned by one lane
 };
 
 // Per-call parameters of a single-source BFS, read by the kernels from
@@ -81,7 +82,7 @@ struct BfsWorkspace {
     size_t bytes = 0;
     // Captured launch sequence (GB_MODE_GRAPH) and what it was captured for.
     hipGraphExec_t graph_exec = nullptr;
-    int g_depth = 0, g_lo = 0, g_hi = 0, g_block = 0, g_k = 0, g_bu = 0, g_alpha = 0, g_beta = 0;
+    int g_depth = 0, g_lo = 0, g_hi = 0, g_block = 0, g_k = 0, g_bu = 0, g_alpha = 0, g_beta = 0, g_low = 0;
     int64_t g_dense = -1;
 };
 
@@ -93,6 +94,7 @@ struct MsConfig {
     int64_t sparse_edges = 0;   // levels with at most this many frontier edges pass a touched list
     bool direction_opt = true;  // bottom-up levels on symmetric graphs
     int alpha = 2;              // bottom-up when frontier edges * alpha > edges not yet fully seen
+    int low_degree = 32;        // bottom-up: rows up to this long are scanned by one lane
 };
 
 // Per-batch parameters, read by the kernels from device memory so that one
@@ -130,7 +132,7 @@ struct MsWorkspace {
     int last_bu_levels = 0;               // levels the last batch ran bottom-up
     size_t bytes = 0;
     hipGraphExec_t graph_exec = nullptr;  // captured batch sequence
-    int graph_depth = 0, graph_block = 0, graph_kpt = 0, graph_bu = -1, graph_alpha = 0;
+    int graph_depth = 0, graph_block = 0, graph_kpt = 0, graph_bu = -1, graph_alpha = 0, graph_low = 0;
     int64_t graph_sparse = -1;
 };
 
@@ -179,6 +181,14 @@ struct Runner {
     void prepare_batch();
 
     size_t bytes() const { return ws.bytes + ms.bytes + ms.levels_bytes; }
+    // Forgets the bottom-up launch window learned so far (after the direction
+    // thresholds change).
+    void reset_bottom_up_window() {
+        bu_lo = 1;
+        bu_hi = kDefaultBuHigh;
+        bu_learned = false;
+        thin_streak = 0;
+    }
 
    private:
     void learn_from_search(bool had_dense_level);
@@ -229,6 +239,11 @@ bool ms_config_supported(int block, int k);
 // HBM a batch workspace needs beyond the graph (plus the staging buffer when
 // results go to the host).
 size_t batch_bytes_estimate(int64_t n, int64_t m, int k, bool host_levels);
+
+// Coordinate-descent tuners (autotune.cpp): leave the fastest configuration
+// on the handle.
+void autotune_bfs(DeviceGraph& g, const int32_t* sources, int count, int reps, gb_tune_result* out);
+void autotune_batch(DeviceGraph& g, const int32_t* sources, int k, int reps, gb_tune_result* out);
 
 // Device-wide exclusive scan of int32 (scan.hip).
 void exclusive_scan_i32(int device, const int32_t* d_in, int32_t* d_out, int64_t n);

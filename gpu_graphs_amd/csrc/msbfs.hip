@@ -8,25 +8,25 @@
 // Per vertex: seen (sources that reached it), next (sources reaching it in
 // the level being expanded).  A level L is two launches:
 //   update_compact(L): for every vertex with next != 0: new = next & ~seen,
-//       seen |= new, level[s][v] = L for every bit s of new, and (degree > 0)
-//       an entry (vertex, first CSR entry, edge prefix, mask = new) in the
-//       queue of active vertices; a block scan orders the entries of a
-//       4096-vertex chunk and one packed atomicAdd per chunk reserves their
+//       seen |= new, the level of every source in new recorded, and (degree
+//       > 0) an entry (vertex, first CSR entry, edge prefix, mask = new) in
+//       the queue of active vertices; a block scan orders the entries of an
+//       8192-vertex chunk and one packed atomicAdd per chunk reserves their
 //       slots and edge range, as in the single-source compaction.
 //   expand(L): edge-parallel tiles over the queue; for edge (u, v) the bits
 //       mask(u) & ~seen(v) are OR-ed into next(v) with a 64-bit atomicOr.
 // Dense levels post the atomicOr (its result is not needed, so the wave does
-// not stall) and the next update_compact scans all n vertices.  On symmetric
-// graphs a big level runs bottom-up instead: every vertex with sources still
-// unseen ORs the frontier masks of its neighbours (fmask, stored by the
-// scanning update_compact) and stops as soon as all its unseen sources are
-// covered; next(v) is then a plain store.  A level goes bottom-up when its
-// frontier edges * alpha exceed the edges of the vertices not yet reached by
-// every source (most vertices' unseen sources all sit one level further).  Sparse levels
-// (few frontier edges) use the returned old value: the edge that turns next(v)
-// from 0 to non-zero appends v to a touched list, and the next update_compact
-// visits only the listed vertices -- a level then costs O(frontier), which is
-// what deep, thin graphs (road networks) need.
+// not stall) and the next update_compact scans all n vertices.  Sparse levels
+// (few frontier edges) use the returned old value: the edge that turns
+// next(v) from 0 to non-zero appends v to a touched list, and the next
+// update_compact visits only the listed vertices -- a level then costs
+// O(frontier), which is what deep, thin graphs (road networks) need.
+// On symmetric graphs a big level runs bottom-up instead: every vertex with
+// sources still unseen ORs the frontier masks of its neighbours (fmask,
+// stored by the scanning update_compact) and stops as soon as all its unseen
+// sources are covered; next(v) is then a plain store.  A level goes
+// bottom-up when its frontier edges * alpha exceed the edges of the vertices
+// not yet reached by every source.
 // Level writes are staged as one byte per (vertex, source): each vertex owns
 // a 64-byte line, cleared to 0xFF, in which update_compact sets the bytes of
 // its new sources with a few 64-bit read-modify-writes.  After the last level
@@ -71,7 +71,6 @@ constexpr int kUcBlock = 512;
 constexpr int kUcRounds = 16;
 constexpr int kUcChunk = kUcBlock * kUcRounds;
 constexpr int kMaxCapturedLevels = 32;  // deeper searches continue with direct launches
-constexpr uint32_t kMsBuLowDegree = 32;  // bottom-up: longer rows are scanned by the whole wave
 constexpr int kByteLevels = 254;         // levels kept in the byte lines; 254 = stored directly, 255 = none
 constexpr int kMaxLevelBatch = 128;     // levels enqueued between done checks
 
@@ -97,6 +96,7 @@ struct MsArgs {
     uint32_t sparse_edges;
     uint32_t bottom_up;          // bottom-up levels allowed (symmetric graph)
     uint32_t alpha;
+    uint32_t low_degree;         // bottom-up: longer rows are scanned by the whole wave
 };
 
 template <int BLOCK, int K>
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(kUcBlock) void ms_update_compact_kernel(MsArgs a, i
 
 // Level L bottom-up: next(v) = the unseen sources of v found among the
 // frontier masks of its neighbours.  A wave owns 64 consecutive vertices;
-// rows of at most kMsBuLowDegree entries are scanned by their lane, longer
+// rows of at most `low_degree` entries are scanned by their lane, longer
 // rows by the whole wave 64 entries at a time; a scan stops once all of the
 // vertex's unseen sources are covered.
 __device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask) {
@@ -356,7 +356,7 @@ __device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask) {
         }
         const uint32_t deg = static_cast<uint32_t>(re - rb);
         unsigned long long acc = 0;
-        if (un && deg <= kMsBuLowDegree) {
+        if (un && deg <= a.low_degree) {
             for (int64_t i = rb; i < re; i += 4) {
                 int32_t u[4];
 #pragma unroll
@@ -373,7 +373,7 @@ __device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask) {
                 if ((acc & un) == un) break;
             }
         }
-        uint64_t hm = wave_ballot(un && deg > kMsBuLowDegree);
+        uint64_t hm = wave_ballot(un && deg > a.low_degree);
         while (hm) {
             const int hl = __ffsll(static_cast<long long>(hm)) - 1;
             hm &= hm - 1;
@@ -698,6 +698,7 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     a.sparse_edges = static_cast<uint32_t>(std::min<int64_t>(sparse_cfg, 0xffffffffll));
     a.bottom_up = (g.symmetric && g.ms_cfg.direction_opt) ? 1u : 0u;
     a.alpha = static_cast<uint32_t>(g.ms_cfg.alpha);
+    a.low_degree = static_cast<uint32_t>(g.ms_cfg.low_degree);
 
     int occ = 0;
     GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(ks->expand),
@@ -734,7 +735,8 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
         // is rebuilt only when the kernel shape or the depth hint changes.
         const bool reuse = ms.graph_exec && ms.graph_depth == depth_hint && ms.graph_block == g.ms_cfg.block &&
                            ms.graph_kpt == g.ms_cfg.edges_per_thread && ms.graph_sparse == a.sparse_edges &&
-                           ms.graph_bu == int(a.bottom_up) && ms.graph_alpha == int(a.alpha);
+                           ms.graph_bu == int(a.bottom_up) && ms.graph_alpha == int(a.alpha) &&
+                           ms.graph_low == int(a.low_degree);
         if (!reuse) {
             if (ms.graph_exec) (void)hipGraphExecDestroy(ms.graph_exec);
             ms.graph_exec = nullptr;
@@ -759,6 +761,7 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
             ms.graph_sparse = a.sparse_edges;
             ms.graph_bu = int(a.bottom_up);
             ms.graph_alpha = int(a.alpha);
+            ms.graph_low = int(a.low_degree);
             GG_HIP(hipEventRecord(ev_start, stream));
         }
         GG_HIP(hipGraphLaunch(ms.graph_exec, stream));

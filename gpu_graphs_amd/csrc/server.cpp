@@ -192,6 +192,22 @@ bool Server::try_dispatch(std::unique_lock<std::mutex>&) {
                 ++it;
             }
         }
+        // Then the workspaces idle lanes hold for other graphs (an idle
+        // lane's thread is waiting for a job and touches none of them).
+        for (int i = 0; i < cfg_.lanes && used_ + need_for(k) > cfg_.memory_budget; ++i) {
+            if (lanes_[i].busy) continue;
+            for (auto it = lanes_[i].has_workspace.begin();
+                 it != lanes_[i].has_workspace.end() && used_ + need_for(k) > cfg_.memory_budget;) {
+                if (it->first == gid) {
+                    ++it;
+                    continue;
+                }
+                exec_->drop_workspace(i, it->first);
+                used_ -= static_cast<int64_t>(exec_->workspace_bytes(it->first, cfg_.plain));
+                ++workspace_drops_;
+                it = lanes_[i].has_workspace.erase(it);
+            }
+        }
     }
     while (k > 1 && used_ + need_for(k) > cfg_.memory_budget) k = pow2(k) / 2;  // shrink to what fits
     const int64_t need = need_for(k);
@@ -435,13 +451,14 @@ ServerStats Server::stats() {
         s.qps = span > 0 ? double(completed_) / (span / 1e3) : 0.0;
     }
     s.memory_used = used_;
+    s.workspace_drops = workspace_drops_;
     s.memory_budget = cfg_.memory_budget == std::numeric_limits<int64_t>::max() ? 0 : cfg_.memory_budget;
     return s;
 }
 
 void Server::reset_stats() {
     std::lock_guard<std::mutex> lk(mu_);
-    submitted_ = completed_ = failed_ = rejected_ = batches_ = batched_queries_ = 0;
+    submitted_ = completed_ = failed_ = rejected_ = batches_ = batched_queries_ = workspace_drops_ = 0;
     latencies_ms_.clear();
     have_first_ = false;
 }

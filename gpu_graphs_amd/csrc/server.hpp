@@ -48,6 +48,9 @@ class Executor {
     virtual int64_t vertices(int gid) = 0;
     // Workspace a lane needs to serve graph gid (charged once per lane/graph).
     virtual size_t workspace_bytes(int gid, bool plain) = 0;
+    // Frees the workspace `lane` holds for graph gid.  Called only while the
+    // lane is idle; must not throw.
+    virtual void drop_workspace(int lane, int gid) noexcept = 0;
     // Result buffers of `bytes`; alloc throws gg::Error (GB_ENOMEM) on failure.
     virtual void* alloc_result(size_t bytes) = 0;
     virtual void free_result(void* p) noexcept = 0;  // must not throw (destructors, scheduler)
@@ -69,6 +72,7 @@ struct ServerStats {
     int64_t submitted = 0, completed = 0, failed = 0, rejected = 0, batches = 0;
     double mean_batch = 0, p50_ms = 0, p99_ms = 0, max_ms = 0, qps = 0;
     int64_t memory_used = 0, memory_budget = 0;
+    int64_t workspace_drops = 0;
 };
 
 class Server {
@@ -150,6 +154,7 @@ class Server {
     uint64_t next_ticket_ = 1;
     // statistics
     int64_t submitted_ = 0, completed_ = 0, failed_ = 0, rejected_ = 0, batches_ = 0, batched_queries_ = 0;
+    int64_t workspace_drops_ = 0;
     std::vector<double> latencies_ms_;
     Clock::time_point first_arrival_{}, last_done_{};
     bool have_first_ = false;

@@ -51,6 +51,13 @@ class HipExecutor : public Executor {
         return plain ? single : single + batch_bytes_estimate(g->n, g->m, 64, false);
     }
 
+    void drop_workspace(int lane, int gid) noexcept override {
+        int prev = -1;
+        const bool restore = hipGetDevice(&prev) == hipSuccess && prev != device_;
+        runners_[lane].erase(gid);  // ~Runner sets the device and frees its stream and buffers
+        if (restore) (void)hipSetDevice(prev);
+    }
+
     void* alloc_result(size_t bytes) override {
         DeviceGuard guard(device_);
         void* p = nullptr;
@@ -273,6 +280,7 @@ int gb_server_get_stats(gb_server* s, gb_server_stats* out) {
         out->qps = st.qps;
         out->memory_used = st.memory_used;
         out->memory_budget = st.memory_budget;
+        out->workspace_drops = st.workspace_drops;
     });
 }
 

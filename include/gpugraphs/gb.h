@@ -213,6 +213,35 @@ int  gb_batch_get_direction_config(const gb_dgraph* dg, int* enable, int* alpha)
 /* Number of levels the last gb_bfs_batch on this handle ran bottom-up. */
 int  gb_batch_last_bottom_up(const gb_dgraph* dg, int* levels);
 
+/* ---- autotuner (no reference counterpart: the reference's kernel shape is
+ *      the compile-time BLOCK_SIZE 1024, 1D Partition/include/
+ *      gpuBFS_kernels.cuh, and it has no thresholds).  Coordinate descent
+ *      over the kernel shape, the direction thresholds, the bottom-up
+ *      degree tier and the frontier-size threshold, each candidate timed on
+ *      the device (best of `reps` passes after a warm-up); the fastest
+ *      configuration is left on the handle.  Results are exact for every
+ *      candidate; only the time changes. ------------------------------- */
+typedef struct gb_tune_result {
+    double  default_ms;        /* configuration on entry: ms per call       */
+    double  best_ms;           /* chosen configuration: ms per call         */
+    int32_t tried;             /* configurations timed                      */
+    int32_t block;             /* chosen workgroup size                     */
+    int32_t edges_per_thread;  /* chosen edges per thread per tile          */
+    int32_t alpha;             /* chosen top-down -> bottom-up threshold    */
+    int32_t beta;              /* chosen bottom-up -> top-down threshold (single-source only, else 0) */
+    int32_t low_degree;        /* bottom-up: rows up to this long go to one lane */
+    int64_t frontier_edges;    /* gb_bfs: dense_edges; gb_bfs_batch: sparse_edges */
+} gb_tune_result;
+
+/* Tunes gb_bfs over `count` (1..64) sources: each pass runs them all. */
+int  gb_bfs_autotune(gb_dgraph* dg, const int32_t* sources, int count, int reps, gb_tune_result* out);
+/* Tunes gb_bfs_batch on one batch of k (1..64) sources. */
+int  gb_batch_autotune(gb_dgraph* dg, const int32_t* sources, int k, int reps, gb_tune_result* out);
+/* Applies a tuning result (from this or an earlier process) to a handle.
+ * default_ms, best_ms and tried are ignored; beta 0 keeps the current one. */
+int  gb_bfs_apply_tuning(gb_dgraph* dg, const gb_tune_result* t);
+int  gb_batch_apply_tuning(gb_dgraph* dg, const gb_tune_result* t);
+
 /* ---- continuous-batching query server (no reference counterpart: the
  *      reference answers one (graph, source) per process run, 1D Partition/
  *      main.cu:10-82).  Queries on any added graph are batched (up to
@@ -243,6 +272,7 @@ typedef struct gb_server_stats {
     double  p50_ms, p99_ms, max_ms;   /* arrival -> done of completed queries */
     double  qps;                      /* completed / (last done - first arrival) */
     int64_t memory_used, memory_budget;
+    int64_t workspace_drops;          /* lane workspaces freed to admit other graphs' batches */
 } gb_server_stats;
 
 int  gb_server_create(const gb_server_cfg* cfg, gb_server** out);

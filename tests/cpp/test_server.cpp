@@ -43,6 +43,8 @@ struct FakeExec : Executor {
 
     int64_t vertices(int gid) override { return n.at(gid); }
     size_t workspace_bytes(int, bool) override { return 1000; }
+    std::atomic<int> drops{0};
+    void drop_workspace(int, int) noexcept override { ++drops; }
     void* alloc_result(size_t bytes) override {
         if (allocs++ == fail_alloc_at) fail(GB_ENOMEM, "injected allocation failure");
         live_alloc += int64_t(bytes);
@@ -329,6 +331,29 @@ static void test_errors() {
     CHECK(bad_cfg);
 }
 
+static void test_idle_workspaces_make_room_for_other_graphs() {
+    // Two graphs, one lane, room for one workspace and one row of either
+    // graph: alternating queries switch the lane's workspace every time.
+    ServerConfig cfg;
+    cfg.lanes = 1;
+    cfg.max_batch = 1;
+    const int64_t n = 1000;
+    cfg.memory_budget = n * 4 + 1000 + 100;
+    Fixture f(cfg, {{0, n}, {1, n}}, 1);
+    for (int i = 0; i < 6; ++i) {
+        const int gid = i % 2;
+        uint64_t t = f.srv->submit(gid, 5 + i);
+        std::vector<int32_t> row(n);
+        CHECK(f.srv->wait(t, 5000, row.data(), nullptr) == GB_OK);
+        CHECK(row_ok(f.ex, gid, 5 + i, row));
+        f.srv->release(t);
+    }
+    const ServerStats st = f.srv->stats();
+    CHECK(st.completed == 6 && st.failed == 0);
+    CHECK(st.workspace_drops == 5 && f.ex->drops.load() == 5);
+    CHECK(st.memory_used <= cfg.memory_budget);
+}
+
 int main() {
     struct {
         const char* name;
@@ -342,7 +367,8 @@ int main() {
                  {"injected_failures", test_injected_failures},
                  {"backpressure_and_cancel", test_backpressure_and_cancel},
                  {"shutdown_with_pending", test_shutdown_with_pending},
-                 {"errors", test_errors}};
+                 {"errors", test_errors},
+                 {"idle_workspaces_make_room", test_idle_workspaces_make_room_for_other_graphs}};
     for (auto& t : tests) {
         const int before = g_failures;
         t.fn();
