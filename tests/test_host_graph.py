@@ -98,6 +98,91 @@ def test_loader_errors(tmp_path):
     assert g.num_nodes == 3 and g.num_edges == 0
 
 
+def write_big_edgelist(path, n=60000, m=250000, seed=3, one_based=False, crlf=False, span_every=0,
+                       comment_every=0, bad_line=None, bad_text="1 x"):
+    """A multi-MiB edge list (parsed by several workers).  Returns the line
+    numbers of the pairs.  span_every splits every k-th pair over two lines,
+    as `ifstream >>` allows; bad_line replaces that line with bad_text."""
+    rng = np.random.default_rng(seed)
+    src = rng.integers(0, n, m) + int(one_based)
+    dst = rng.integers(0, n, m) + int(one_based)
+    lines = [str(n), str(m)]
+    for i in range(m):
+        if comment_every and i % comment_every == 0:
+            lines.append("# comment %d" % i)
+        if span_every and i % span_every == 0:
+            lines += [str(src[i]), str(dst[i])]
+        else:
+            lines.append("%d %d" % (src[i], dst[i]))
+    if bad_line is not None:
+        lines[bad_line - 1] = bad_text
+    eol = "\r\n" if crlf else "\n"
+    with open(path, "w", newline="") as f:
+        f.write(eol.join(lines) + eol)
+    assert os.path.getsize(path) > (1 << 20)
+
+
+@pytest.mark.parametrize("variant", ["plain", "crlf_spanning_pairs", "one_based"])
+def test_parallel_parser_matches_oracle_loader(tmp_path, monkeypatch, variant):
+    p = str(tmp_path / "big.txt")
+    one_based = variant == "one_based"
+    write_big_edgelist(p, one_based=one_based, crlf=variant == "crlf_spanning_pairs",
+                       span_every=7 if variant == "crlf_spanning_pairs" else 0)
+    off, cols = O.load_edgelist(p, one_based=one_based)
+    for threads in ("16", "7", "1"):
+        monkeypatch.setenv("GB_THREADS", threads)
+        g = gg.csr(p, one_based=one_based, keep_order=True)
+        np.testing.assert_array_equal(g.row_offset, off)
+        np.testing.assert_array_equal(g.col_idx, cols)
+
+
+def test_parallel_parser_comments_and_auto_n(tmp_path, monkeypatch):
+    p = str(tmp_path / "big.txt")
+    write_big_edgelist(p, comment_every=500, span_every=11)
+    got = {}
+    for threads in ("16", "1"):
+        monkeypatch.setenv("GB_THREADS", threads)
+        g = gg.csr(p, auto_n=True)
+        got[threads] = (g.num_nodes, g.row_offset.copy(), g.col_idx.copy())
+    assert got["16"][0] == got["1"][0]
+    np.testing.assert_array_equal(got["16"][1], got["1"][1])
+    np.testing.assert_array_equal(got["16"][2], got["1"][2])
+    assert_csr_invariants(gg.csr(p).graph)
+
+
+@pytest.mark.parametrize("bad_text,code,needle", [
+    ("1 x", -2, "malformed token on line 200001"),
+    ("5 60000", -5, ":200001: edge (5, 60000) outside the 60000-vertex id range"),
+    ("17", -2, "malformed token on line 200002"),  # odd token count: the pairing breaks there
+])
+def test_parallel_parser_errors_name_the_line(tmp_path, monkeypatch, bad_text, code, needle):
+    p = str(tmp_path / "bad.txt")
+    write_big_edgelist(p, bad_line=200001, bad_text=bad_text)
+    if bad_text == "17":  # a lone id: the next line's pair is read across, ending on a dangling id
+        with open(p) as f:
+            text = f.read().splitlines()
+        text[200001] = "3 y"
+        with open(p, "w") as f:
+            f.write("\n".join(text) + "\n")
+    for threads in ("16", "1"):
+        monkeypatch.setenv("GB_THREADS", threads)
+        with pytest.raises(gg.GBError) as e:
+            gg.csr(p)
+        assert e.value.code == code and needle in str(e.value), str(e.value)
+
+
+def test_parallel_parser_dangling_last_id(tmp_path, monkeypatch):
+    p = str(tmp_path / "odd.txt")
+    write_big_edgelist(p)
+    with open(p, "a") as f:
+        f.write("42\n")
+    for threads in ("16", "1"):
+        monkeypatch.setenv("GB_THREADS", threads)
+        with pytest.raises(gg.GBError) as e:
+            gg.csr(p)
+        assert e.value.code == -2 and "dangling vertex id on line 250003" in str(e.value)
+
+
 def test_symmetrize_and_flags(tmp_path):
     p = tmp_path / "g.txt"
     p.write_text("4\n5\n0 1\n1 2\n2 2\n0 1\n3 0\n")
