@@ -104,14 +104,12 @@ void autotune_bfs(DeviceGraph& g, const int32_t* sources, int count, int reps, g
                              window();
                          },
                          [&]() { return int64_t(c.alpha); }});
-        knobs.push_back({{4, 24, 1000},
+        knobs.push_back({{8, 24, 64, 256},
                          [&, window](int64_t v) {
                              c.beta = int(v);
                              window();
                          },
                          [&]() { return int64_t(c.beta); }});
-        knobs.push_back({{16, 32, 64}, [&](int64_t v) { c.low_degree = int(v); },
-                         [&]() { return int64_t(c.low_degree); }});
     }
     const int64_t dense0 = g.default_dense_edges();
     knobs.push_back({{dense0 / 4, dense0, dense0 * 4}, [&](int64_t v) { c.dense_edges = v; },
@@ -125,7 +123,7 @@ void autotune_bfs(DeviceGraph& g, const int32_t* sources, int count, int reps, g
     res.edges_per_thread = c.edges_per_thread;
     res.alpha = c.alpha;
     res.beta = c.beta;
-    res.low_degree = c.low_degree;
+    res.low_degree = 0;
     res.frontier_edges = c.dense_edges;
     if (out) *out = res;
 }

@@ -40,14 +40,14 @@
 //  * Bottom-up (Beamer, Asanovic, Patterson, SC 2012), on symmetric graphs:
 //    when a level's frontier edges exceed the unexplored edges / alpha, each
 //    unvisited vertex looks for a neighbour in the frontier bitmap and stops
-//    at the first one.  A wave owns 64 consecutive vertices (two bitmap
-//    words, so it updates them with plain stores); rows of at most
-//    `low_degree` entries are scanned by their lane alone, longer rows by the
-//    whole wave 64 entries at a time with a ballot -- the degree tiers of a
-//    thread or a wavefront per vertex.  Isolated vertices are marked visited
-//    up front.  Search goes back top-down when the frontier's edges * beta
-//    fall below the edges still unexplored (a bottom-up level rescans
-//    those); compact() then turns the frontier bitmap into a queue.
+//    at the first one.  A wave owns groups of 64 consecutive vertices (two
+//    bitmap words each, so it updates them with plain stores).  Each lane
+//    probes the first entries of its own row; rows still open after those go
+//    to 8-lane sub-waves that scan 8 entries per step -- the degree tiers of
+//    a thread or a group of lanes per vertex.  Isolated vertices are marked
+//    visited up front.  Search goes bottom-up only while the frontier grows,
+//    and back top-down once it has fewer than n / beta vertices and shrinks
+//    (Beamer's rules); compact() then turns the frontier bitmap into a queue.
 //  * Directions are chosen on the device from counters that no kernel of
 //    the level modifies, so expand(L), bottom_up(L) and compact(L) agree.
 //    The host launches bottom_up(L) only for levels where searches of this
@@ -64,6 +64,7 @@
 #include <cstring>
 #include <vector>
 
+#include "common.hpp"
 #include "device_graph.hpp"
 #include "frontier.hpp"
 #include "wave.hpp"
@@ -97,7 +98,10 @@ enum : unsigned long long { DIR_TD = 1, DIR_BU = 2 };
 constexpr int kCompactBlock = 1024;
 constexpr int kCompactRounds = 16;                          // vertices per thread
 constexpr int kChunk = kCompactBlock * kCompactRounds;      // vertices per compaction chunk
-constexpr int kBuBlock = 1024;
+constexpr int kBuBlock = 256;                               // 4 waves
+constexpr int kBuGroups = 4;                                // 64-vertex groups per wave pass
+constexpr int kBuFirst = 4;                                 // row entries each lane probes before the list (= int4 head)
+static_assert(kBuFirst == 4, "the head array holds one int4 per vertex");
 constexpr int kMaxCapturedLevels = 32;                      // deeper searches continue with direct launches
 
 struct BfsArgs {
@@ -107,6 +111,7 @@ struct BfsArgs {
     uint32_t* prev;   // vertices already placed in (or handled by) a frontier queue
     uint32_t* fb[2];  // frontier bitmaps, by level parity
     const uint32_t* iso;  // vertices of degree 0 (symmetric graphs), or null
+    const int4* head;     // first kBuFirst neighbours of each vertex, -1 padded (symmetric graphs)
     int64_t* q_rb[2];
     uint32_t* q_eoff[2];
     int32_t* q_v[2];
@@ -121,14 +126,24 @@ struct BfsArgs {
     uint32_t tile_edges;        // T of the expand kernel (tile_start granularity)
     uint32_t bottom_up;         // direction optimisation enabled
     uint32_t alpha, beta;
-    uint32_t low_degree;        // bottom-up: longer rows are scanned by the whole wave
 };
 
 // Beamer's test for a frontier of E CSR entries after `explored` entries of
 // earlier frontiers: frontier edges * alpha against the unexplored edges.
-__device__ __forceinline__ bool go_bottom_up(const BfsArgs& a, int64_t explored, uint32_t E) {
+// Top-down -> bottom-up (Beamer et al.): the frontier's edges E, growing
+// from the previous level's E_prev, exceed the unexplored edges / alpha.
+// (Unexplored edges include those of vertices the search never reaches, so
+// without the growth test a small frontier late in the search would go
+// bottom-up.)
+__device__ __forceinline__ bool go_bottom_up(const BfsArgs& a, int64_t explored, uint32_t E, uint32_t E_prev) {
     const int64_t unexplored = a.m - explored - static_cast<int64_t>(E);
-    return static_cast<int64_t>(E) * a.alpha > unexplored;
+    return E > E_prev && static_cast<int64_t>(E) * a.alpha > unexplored;
+}
+
+// Bottom-up -> top-down (Beamer et al.): the next frontier has fewer than
+// n / beta vertices and is shrinking.
+__device__ __forceinline__ bool go_top_down(const BfsArgs& a, uint32_t F, uint32_t F_next) {
+    return F_next < F && static_cast<int64_t>(F_next) * a.beta < a.n;
 }
 
 // Direction of level L.  Reads only words that no kernel of level L writes.
@@ -138,7 +153,11 @@ __device__ __forceinline__ unsigned long long choose_dir(const BfsArgs& a, int L
     if (!(rep & REP_BITMAP)) return DIR_TD;
     if (a.ctl[CTL_DIR0 + (L - 1) % 3] == DIR_BU) return (rep & REP_QUEUE) ? DIR_TD : DIR_BU;
     // After a top-down level: frontier edges against the unexplored edges.
-    return go_bottom_up(a, static_cast<int64_t>(a.ctl[CTL_EXP0 + (L & 1)]), E) ? DIR_BU : DIR_TD;
+    // EXP[L & 1] = edges of levels <= L-1, EXP[(L + 1) & 1] = of levels <= L-2;
+    // compact(L) is the next kernel to write either.
+    const unsigned long long through_prev = a.ctl[CTL_EXP0 + (L & 1)];
+    const uint32_t E_prev = static_cast<uint32_t>(through_prev - a.ctl[CTL_EXP0 + ((L + 1) & 1)]);
+    return go_bottom_up(a, static_cast<int64_t>(through_prev), E, E_prev) ? DIR_BU : DIR_TD;
 }
 
 template <int BLOCK, int K>
@@ -424,7 +443,32 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(BfsArgs a, int L, int
 // Level L bottom-up: every unvisited vertex looks for a neighbour in the
 // frontier bitmap of level L; the vertices found form the bitmap of level
 // L+1, and their (count, CSR entries) go to ring[(L+1) % 3].
+//
+// The kernel is bound by memory latency, not bandwidth (a search level
+// touches few bytes per vertex), so it is organised for loads in flight:
+//   * a wave takes kBuGroups groups of 64 vertices per pass and issues the
+//     visited-word, offset, first-entries and frontier-probe loads of all of
+//     them together (four dependent steps per pass, not per group);
+//   * every unvisited lane probes the first kBuFirst neighbours of its
+//     vertex, kept side by side per vertex (`head`, built at upload), so a
+//     wave reads them for its 64 vertices in one coalesced load; the first
+//     hit usually comes within them;
+//   * rows not settled by that go to a per-wave list in LDS, which 8-lane
+//     sub-waves drain: a sub-wave scans 8 consecutive entries of its row per
+//     step, stops at the first hit, and takes the next row from the list as
+//     soon as it is done, so long and short rows share the wave's steps.
+// The wave owns the bitmap words of its groups and writes them with plain
+// stores; the output bitmap arrives zeroed (compact(L-1)), so only words
+// with new vertices are written.
+struct BuWaveLds {
+    int64_t rb[kBuGroups * 64];    // list: next entry of the row
+    uint32_t len[kBuGroups * 64];  // list: entries left
+    uint16_t vid[kBuGroups * 64];  // list: vertex (group * 64 + lane) within the pass
+    int32_t par[kBuGroups * 64];   // by vertex within the pass: parent found by the list, or -1
+};
+
 __global__ __launch_bounds__(kBuBlock) void bfs_bottom_up_kernel(BfsArgs a, int L) {
+    __shared__ BuWaveLds s_wave[kBuBlock / 64];
     __shared__ unsigned long long s_acc[kBuBlock / 64];
     if (a.ctl[CTL_DONE] | a.ctl[CTL_ERR]) return;
     const unsigned long long cnt = a.ctl[CTL_RING0 + L % 3];
@@ -435,103 +479,145 @@ __global__ __launch_bounds__(kBuBlock) void bfs_bottom_up_kernel(BfsArgs a, int 
         atomicMin(&a.ctl[CTL_BUFIRST], static_cast<unsigned long long>(L));
         atomicMax(&a.ctl[CTL_BULAST], static_cast<unsigned long long>(L));
     }
+    BuWaveLds& sw = s_wave[threadIdx.x >> 6];
     const uint32_t* fin = a.fb[L & 1];
     uint32_t* fout = a.fb[(L + 1) & 1];
     int32_t* const levels = a.p->levels;
     int32_t* const parents = a.p->parents;
     const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t ngroups = (static_cast<uint64_t>(a.n) + 63) / 64;
-    const uint64_t wstride = static_cast<uint64_t>(gridDim.x) * (kBuBlock / 64);
+    const uint32_t sg = lane >> 3, sl = lane & 7u;  // sub-wave and lane within it
+    const uint64_t n = static_cast<uint64_t>(a.n);
+    const uint64_t ngroups = (n + 63) / 64;
+    const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * (kBuBlock / 64);
     uint32_t my_cnt = 0;
     uint64_t my_edges = 0;
     bool bad = false;
-    for (uint64_t g = static_cast<uint64_t>(blockIdx.x) * (kBuBlock / 64) + (threadIdx.x >> 6); g < ngroups;
-         g += wstride) {
-        const int64_t v = static_cast<int64_t>(g * 64 + lane);
-        const uint32_t w = static_cast<uint32_t>(g * 2 + (lane >> 5));
-        const bool in_range = v < a.n;
-        const uint32_t vw = in_range ? a.vis[w] : ~0u;
-        const bool unvisited = in_range && !((vw >> (lane & 31u)) & 1u);
-        const uint64_t um = wave_ballot(unvisited);
-        if (!um) {
-            if ((lane & 31u) == 0 && w < a.nwords) fout[w] = 0u;
-            continue;
+    auto in_frontier = [&](int32_t u) -> bool {
+        if (static_cast<uint32_t>(u) >= n) {
+            bad = true;
+            return false;
         }
-        int64_t rb = 0, re = 0;
-        if (unvisited) {
-            rb = a.offsets[v];
-            re = a.offsets[v + 1];
+        return (fin[static_cast<uint32_t>(u) >> 5] >> (u & 31)) & 1u;
+    };
+    for (uint64_t g0 = (static_cast<uint64_t>(blockIdx.x) * (kBuBlock / 64) + (threadIdx.x >> 6)) * kBuGroups;
+         g0 < ngroups; g0 += nwaves * kBuGroups) {
+        uint32_t vw[kBuGroups];
+        bool unv[kBuGroups];
+        int64_t rb[kBuGroups], re[kBuGroups];
+        int32_t par[kBuGroups];
+        // Step 1: visited words.
+#pragma unroll
+        for (int j = 0; j < kBuGroups; ++j) {
+            const uint64_t v = (g0 + j) * 64 + lane;
+            vw[j] = v < n ? a.vis[(g0 + j) * 2 + (lane >> 5)] : ~0u;
+            unv[j] = !((vw[j] >> (lane & 31u)) & 1u);
         }
-        const uint32_t deg = static_cast<uint32_t>(re - rb);
-        bool found = false;
-        int32_t par = -1;
-        // Short rows: the lane alone, 4 entries per step.
-        if (unvisited && deg <= a.low_degree) {
-            for (int64_t i = rb; i < re && !found; i += 4) {
-                int32_t u[4];
+        // Step 2: first neighbours (side by side for consecutive vertices)
+        // and row bounds of the unvisited vertices.
+        int4 hd[kBuGroups];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) u[j] = (i + j < re) ? a.cols[i + j] : -1;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    if (u[j] < 0 || found) continue;
-                    if (static_cast<uint32_t>(u[j]) >= static_cast<uint64_t>(a.n)) {
-                        bad = true;
-                        continue;
-                    }
-                    if ((fin[static_cast<uint32_t>(u[j]) >> 5] >> (u[j] & 31)) & 1u) {
-                        found = true;
-                        par = u[j];
-                    }
-                }
+        for (int j = 0; j < kBuGroups; ++j) {
+            rb[j] = re[j] = 0;
+            hd[j] = make_int4(-1, -1, -1, -1);
+            if (unv[j]) {
+                const uint64_t v = (g0 + j) * 64 + lane;
+                hd[j] = a.head[v];
+                rb[j] = a.offsets[v];
+                re[j] = a.offsets[v + 1];
             }
         }
-        // Long rows: the whole wave, 64 entries per step, first hit wins.
-        uint64_t hm = wave_ballot(unvisited && deg > a.low_degree);
-        while (hm) {
-            const int hl = __ffsll(static_cast<long long>(hm)) - 1;
-            hm &= hm - 1;
-            const int64_t hrb = __shfl(rb, hl, 64);
-            const int64_t hre = __shfl(re, hl, 64);
-            int32_t hpar = -1;
-            for (int64_t b0 = hrb; b0 < hre; b0 += 64) {
-                const int64_t i = b0 + lane;
-                int32_t u = -1;
+        // Step 3: probes of the first neighbours.
+        uint32_t cont = 0;  // bit j: the row goes to the list
+#pragma unroll
+        for (int j = 0; j < kBuGroups; ++j) {
+            const int32_t u[kBuFirst] = {hd[j].x, hd[j].y, hd[j].z, hd[j].w};
+            par[j] = -1;
+#pragma unroll
+            for (int k = 0; k < kBuFirst; ++k)
+                if (par[j] < 0 && u[k] >= 0 && in_frontier(u[k])) par[j] = u[k];
+            if (unv[j] && par[j] < 0 && re[j] - rb[j] > kBuFirst) cont |= 1u << j;
+        }
+        // Step 4: the rows still open, drained by the 8 sub-waves.
+        uint32_t nitems = 0;
+#pragma unroll
+        for (int j = 0; j < kBuGroups; ++j) {
+            const bool c = (cont >> j) & 1u;
+            const uint64_t m = wave_ballot(c);
+            if (c) {
+                const uint32_t slot = nitems + rank_below(m);
+                sw.rb[slot] = rb[j] + kBuFirst;
+                sw.len[slot] = static_cast<uint32_t>(re[j] - rb[j] - kBuFirst);
+                sw.vid[slot] = static_cast<uint16_t>(j * 64 + lane);
+                sw.par[j * 64 + lane] = -1;
+            }
+            nitems += static_cast<uint32_t>(__popcll(m));
+        }
+        if (nitems) {
+            wave_lds_sync();
+            int32_t item = sg < nitems ? static_cast<int32_t>(sg) : -1;
+            int64_t pos = 0, end = 0;
+            if (item >= 0) {
+                pos = sw.rb[item];
+                end = pos + sw.len[item];
+            }
+            uint32_t next = 8;
+            while (wave_ballot(item >= 0)) {
+                const int64_t i = pos + sl;
+                int32_t cu = -1;
                 bool hit = false;
-                if (i < hre) {
-                    u = a.cols[i];
-                    if (static_cast<uint32_t>(u) >= static_cast<uint64_t>(a.n)) {
-                        bad = true;
-                    } else {
-                        hit = (fin[static_cast<uint32_t>(u) >> 5] >> (u & 31)) & 1u;
-                    }
+                if (item >= 0 && i < end) {
+                    cu = a.cols[i];
+                    hit = in_frontier(cu);
                 }
                 const uint64_t hb = wave_ballot(hit);
-                if (hb) {
-                    hpar = __shfl(u, __ffsll(static_cast<long long>(hb)) - 1, 64);
-                    break;
+                const uint32_t mine = static_cast<uint32_t>(hb >> (sg * 8)) & 0xffu;
+                const int src = mine ? static_cast<int>(sg * 8 + __ffs(mine) - 1) : static_cast<int>(lane);
+                const int32_t pu = __shfl(cu, src, 64);
+                bool done = false;
+                if (item >= 0) {
+                    if (mine) {
+                        if (sl == 0) sw.par[sw.vid[item]] = pu;
+                        done = true;
+                    } else {
+                        pos += 8;
+                        done = pos >= end;
+                    }
                 }
+                const uint64_t need = wave_ballot(done && sl == 0);
+                if (done) {
+                    const uint32_t t = next + static_cast<uint32_t>(__popcll(need & ((1ull << (sg * 8)) - 1)));
+                    item = t < nitems ? static_cast<int32_t>(t) : -1;
+                    if (item >= 0) {
+                        pos = sw.rb[item];
+                        end = pos + sw.len[item];
+                    }
+                }
+                next += static_cast<uint32_t>(__popcll(need));
             }
-            if (static_cast<int>(lane) == hl && hpar >= 0) {
-                found = true;
-                par = hpar;
+            wave_lds_sync();
+        }
+        // Step 5: levels, parents, bitmap words.
+#pragma unroll
+        for (int j = 0; j < kBuGroups; ++j) {
+            if ((cont >> j) & 1u) par[j] = sw.par[j * 64 + lane];
+            const bool found = unv[j] && par[j] >= 0;
+            const uint64_t nb = wave_ballot(found);
+            const uint64_t v = (g0 + j) * 64 + lane;
+            if (found) {
+                levels[v] = L + 1;
+                if (parents) parents[v] = par[j];
+                ++my_cnt;
+                my_edges += static_cast<uint64_t>(re[j] - rb[j]);
             }
-        }
-        const uint64_t nb = wave_ballot(found);
-        if (found) {
-            levels[v] = L + 1;
-            if (parents) parents[v] = par;
-            ++my_cnt;
-            my_edges += deg;
-        }
-        // The wave owns words 2g and 2g+1: plain stores.
-        if ((lane & 31u) == 0 && w < a.nwords) {
+            const uint64_t w = (g0 + j) * 2 + (lane >> 5);
             const uint32_t bits = static_cast<uint32_t>(nb >> (lane & 32u));
-            fout[w] = bits;
-            if (bits) {
-                a.vis[w] = vw | bits;
-                a.prev[w] = vw | bits;
+            if ((lane & 31u) == 0 && bits && w < a.nwords) {  // fout was zeroed by compact(L-1)
+                fout[w] = bits;
+                a.vis[w] = vw[j] | bits;
+                a.prev[w] = vw[j] | bits;
             }
         }
+        if (nitems) wave_lds_sync();  // the list of the next pass reuses the LDS
     }
     if (bad) atomicOr(&a.ctl[CTL_ERR], ERR_RANGE);
     const unsigned long long mine = (static_cast<unsigned long long>(wave_sum(my_cnt)) << 32) | wave_sum(my_edges);
@@ -631,13 +717,16 @@ __device__ void compact_chunk(const BfsArgs& a, int L, uint32_t c, bool from_bit
 }
 
 // `bu_next`: bottom_up(L+1) will be launched; `bu_next2`: bottom_up(L+2)
-// will be, so fb[L & 1] (read by bottom_up(L) if at all, next written as
-// level L+2's bitmap by a sparse expand(L+1)) is zeroed here.  Level L+1's
+// will be.  With either, fb[L & 1] (read by bottom_up(L) if at all, next
+// written as level L+2's bitmap by bottom_up(L+1) or a sparse expand(L+1),
+// which only set bits) is zeroed here.  Level L+1's
 // counter ring[(L+1)%3] was zeroed by expand(L-1); a dense top-down level L
 // left it alone and the chunks' reservations build it up.
 __global__ __launch_bounds__(kCompactBlock) void bfs_compact_kernel(BfsArgs a, int L, int bu_next, int bu_next2) {
     if (a.ctl[CTL_DONE] | a.ctl[CTL_ERR]) return;
-    if (bu_next2) {
+    if (bu_next || bu_next2) {
+        // fb[L & 1] is the output bitmap of level L+1: bottom_up(L+1) and
+        // sparse top-down winners only set bits in it.
         uint32_t* fz = a.fb[L & 1];
         for (uint32_t i = blockIdx.x * kCompactBlock + threadIdx.x; i < a.nwords; i += gridDim.x * kCompactBlock)
             fz[i] = 0u;
@@ -661,14 +750,11 @@ __global__ __launch_bounds__(kCompactBlock) void bfs_compact_kernel(BfsArgs a, i
     }
     if ((next_cnt >> 32) == 0) return;
     if (dir == DIR_BU) {
-        // Back top-down when level L+1's edges * beta are below the edges
-        // still unexplored after it: a bottom-up level rescans those, while
-        // a nearly finished search is cheaper to end bottom-up than to turn
-        // the bitmap into a queue.
+        // Back top-down when level L+1 is small and shrinking (a bottom-up
+        // level costs a pass over every vertex), or gets no bottom-up launch.
         const unsigned long long nx = a.ctl[CTL_RING0 + (L + 1) % 3];
-        const uint32_t F1 = static_cast<uint32_t>(nx >> 32), E1 = static_cast<uint32_t>(nx);
-        const int64_t unexplored_after = a.m - explored - static_cast<int64_t>(E1);
-        const bool to_top_down = !bu_next || static_cast<int64_t>(E1) * a.beta < unexplored_after;
+        const uint32_t F1 = static_cast<uint32_t>(nx >> 32);
+        const bool to_top_down = !bu_next || go_top_down(a, F, F1);
         if (F1 == 0 || !to_top_down) return;  // empty, or level L+1 stays bottom-up
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.ctl[CTL_REP0 + (L + 1) % 3], REP_QUEUE);
         for (uint32_t c = blockIdx.x; c < a.nchunks; c += gridDim.x) {
@@ -683,7 +769,7 @@ __global__ __launch_bounds__(kCompactBlock) void bfs_compact_kernel(BfsArgs a, i
     // needed; otherwise the queue too.
     const unsigned long long nx = a.ctl[CTL_DCNT0 + ((L + 1) & 1)];
     const uint32_t F1 = static_cast<uint32_t>(nx >> 32), E1 = static_cast<uint32_t>(nx);
-    if (bu_next && a.bottom_up && F1 > 0 && go_bottom_up(a, explored, E1)) {
+    if (bu_next && a.bottom_up && F1 > 0 && go_bottom_up(a, explored, E1, E)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             a.ctl[CTL_RING0 + (L + 1) % 3] = nx;
             a.ctl[CTL_REP0 + (L + 1) % 3] = REP_BITMAP;
@@ -803,6 +889,21 @@ DeviceGraph::DeviceGraph(const HostGraph& g, int dev) : device(dev), n(g.n), m(g
             GG_HIP(hipMalloc(&iso, sizeof(uint32_t) * nwords));
             graph_bytes += sizeof(uint32_t) * nwords;
             GG_HIP(hipMemcpy(iso, h.data(), sizeof(uint32_t) * nwords, hipMemcpyHostToDevice));
+            // The first neighbours of every vertex side by side: a bottom-up
+            // level reads them for 64 consecutive vertices in one coalesced
+            // load instead of 64 scattered row starts.
+            std::vector<int4> hd(static_cast<size_t>(n));
+            parallel_for(n, [&](int64_t b, int64_t e, int) {
+                for (int64_t v = b; v < e; ++v) {
+                    const int64_t rb = g.offsets[v], d = g.offsets[v + 1] - rb;
+                    int32_t f[4];
+                    for (int k = 0; k < 4; ++k) f[k] = k < d ? g.cols[rb + k] : -1;
+                    hd[v] = make_int4(f[0], f[1], f[2], f[3]);
+                }
+            });
+            GG_HIP(hipMalloc(&head, sizeof(int4) * n));
+            graph_bytes += sizeof(int4) * n;
+            GG_HIP(hipMemcpy(head, hd.data(), sizeof(int4) * n, hipMemcpyHostToDevice));
         }
         runner = std::make_unique<Runner>(*this);
     } catch (...) {
@@ -817,9 +918,11 @@ DeviceGraph::~DeviceGraph() {
     if (offsets) (void)hipFree(offsets);
     if (cols) (void)hipFree(cols);
     if (iso) (void)hipFree(iso);
+    if (head) (void)hipFree(head);
     offsets = nullptr;
     cols = nullptr;
     iso = nullptr;
+    head = nullptr;
 }
 
 Runner::Runner(DeviceGraph& graph) : g(graph), device(graph.device), n(graph.n), m(graph.m) {
@@ -934,6 +1037,7 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     a.fb[0] = ws.fb[0];
     a.fb[1] = ws.fb[1];
     a.iso = g.iso;
+    a.head = g.head;
     for (int i = 0; i < 2; ++i) {
         a.q_rb[i] = ws.q_rb[i];
         a.q_eoff[i] = ws.q_eoff[i];
@@ -951,7 +1055,6 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     a.bottom_up = (g.symmetric && g.cfg.direction_opt) ? 1u : 0u;
     a.alpha = static_cast<uint32_t>(g.cfg.alpha);
     a.beta = static_cast<uint32_t>(g.cfg.beta);
-    a.low_degree = static_cast<uint32_t>(g.cfg.low_degree);
 
     int occ = 0;
     GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(ks->expand), g.cfg.block, 0));
@@ -963,8 +1066,14 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     // compact(): chunks are independent; up to 2 workgroups of 1024 per CU.
     const uint32_t cgrid = static_cast<uint32_t>(std::min<int64_t>(ws.nchunks, int64_t(g.num_cus) * 2));
     const uint32_t clear_grid = static_cast<uint32_t>(std::min<int64_t>((n + 255) / 256, int64_t(g.num_cus) * 8));
-    // bottom_up(): one wave per 64 vertices, up to 2 workgroups of 16 waves per CU.
-    const uint32_t bgrid = static_cast<uint32_t>(std::min<int64_t>((n + 1023) / 1024, int64_t(g.num_cus) * 2));
+    // bottom_up(): a wave takes kBuGroups groups of 64 vertices per pass; as
+    // many workgroups as are resident at once (the passes are grid-strided).
+    int bu_occ = 0;
+    GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bu_occ, reinterpret_cast<const void*>(bfs_bottom_up_kernel),
+                                                        kBuBlock, 0));
+    const int64_t bu_waves = (n + 64 * kBuGroups - 1) / (64 * kBuGroups);
+    const uint32_t bgrid = static_cast<uint32_t>(std::max<int64_t>(
+        1, std::min<int64_t>((bu_waves + kBuBlock / 64 - 1) / (kBuBlock / 64), int64_t(g.num_cus) * bu_occ)));
 
     // bottom_up(L) is launched for the levels [bu_lo, bu_hi] learned from
     // earlier searches of this graph; other levels run top-down.
@@ -1060,8 +1169,7 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
             const bool reuse = ws.graph_exec && ws.g_depth == D && ws.g_lo == bu_lo && ws.g_hi == bu_hi &&
                                ws.g_block == g.cfg.block && ws.g_k == g.cfg.edges_per_thread &&
                                ws.g_dense == int64_t(a.dense_edges) && ws.g_bu == int(a.bottom_up) &&
-                               ws.g_alpha == int(a.alpha) && ws.g_beta == int(a.beta) &&
-                               ws.g_low == int(a.low_degree);
+                               ws.g_alpha == int(a.alpha) && ws.g_beta == int(a.beta);
             if (!reuse) {
                 if (ws.graph_exec) (void)hipGraphExecDestroy(ws.graph_exec);
                 ws.graph_exec = nullptr;
@@ -1091,7 +1199,6 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
                 ws.g_bu = int(a.bottom_up);
                 ws.g_alpha = int(a.alpha);
                 ws.g_beta = int(a.beta);
-                ws.g_low = int(a.low_degree);
             }
             GG_HIP(hipEventRecord(ev_start, stream));
             GG_HIP(hipGraphLaunch(ws.graph_exec, stream));

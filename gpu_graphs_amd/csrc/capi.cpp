@@ -416,13 +416,12 @@ int gb_bfs_apply_tuning(gb_dgraph* dg, const gb_tune_result* t) {
     return guarded([&] {
         need(dg && t, "null argument");
         need(gg::bfs_config_supported(t->block, t->edges_per_thread), "unsupported block / edges_per_thread combination");
-        need(t->alpha > 0 && t->beta >= 0 && t->low_degree > 0 && t->frontier_edges > 0, "thresholds must be positive");
+        need(t->alpha > 0 && t->beta >= 0 && t->frontier_edges > 0, "thresholds must be positive");
         gg::BfsConfig& c = dg->d->cfg;
         c.block = t->block;
         c.edges_per_thread = t->edges_per_thread;
         c.alpha = t->alpha;
         if (t->beta) c.beta = t->beta;
-        c.low_degree = t->low_degree;
         c.dense_edges = t->frontier_edges;
         dg->d->runner->reset_bottom_up_window();
     });

@@ -48,9 +48,8 @@ struct BfsConfig {
     int mode = GB_MODE_GRAPH;
     int64_t dense_edges = 0;      // levels with more frontier edges use bitmap compaction
     bool direction_opt = true;    // bottom-up levels on symmetric graphs
-    int alpha = 14;               // go bottom-up when frontier edges * alpha > unexplored edges
-    int beta = 24;                // back top-down when frontier edges * beta < unexplored edges
-    int low_degree = 32;          // bottom-up: rows up to this long are scanned by one lane
+    int alpha = 14;               // bottom-up when the frontier grows and its edges * alpha > unexplored edges
+    int beta = 24;                // back top-down when the frontier shrinks below n / beta vertices
 };
 
 // Per-call parameters of a single-source BFS, read by the kernels from
@@ -82,7 +81,7 @@ struct BfsWorkspace {
     size_t bytes = 0;
     // Captured launch sequence (GB_MODE_GRAPH) and what it was captured for.
     hipGraphExec_t graph_exec = nullptr;
-    int g_depth = 0, g_lo = 0, g_hi = 0, g_block = 0, g_k = 0, g_bu = 0, g_alpha = 0, g_beta = 0, g_low = 0;
+    int g_depth = 0, g_lo = 0, g_hi = 0, g_block = 0, g_k = 0, g_bu = 0, g_alpha = 0, g_beta = 0;
     int64_t g_dense = -1;
 };
 
@@ -204,6 +203,7 @@ struct DeviceGraph {
     int64_t* offsets = nullptr;  // n + 1
     int32_t* cols = nullptr;     // m
     uint32_t* iso = nullptr;     // ceil(n/32) words: degree-0 vertices (symmetric graphs only)
+    int4* head = nullptr;        // n x 4: first neighbours of each vertex, -1 padded (symmetric graphs only)
     bool symmetric = false;      // every edge has its reverse: bottom-up search is valid
     size_t graph_bytes = 0;
     int num_cus = 0;

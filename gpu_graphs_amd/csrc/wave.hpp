@@ -25,6 +25,15 @@ __device__ __forceinline__ uint32_t rank_below(uint64_t mask) {
 
 __device__ __forceinline__ uint64_t wave_ballot(bool p) { return __ballot(p); }
 
+// Orders this wave's LDS accesses: stores of every lane before the barrier
+// are visible to loads of every lane after it (lanes of a wave run in
+// lockstep; the fence stops the compiler moving LDS accesses across).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <class T>
 __device__ __forceinline__ T wave_inclusive_scan(T x) {
     const uint32_t lane = lane_id();

@@ -164,9 +164,9 @@ int  gb_bfs_get_config(const gb_dgraph* dg, int* block, int* edges_per_thread,
                        int* blocks_per_cu, int* mode, int64_t* dense_edges);
 /* Direction optimisation of gb_bfs (bottom-up levels, Beamer, Asanovic and
  * Patterson, SC 2012), used on symmetric graphs only: enable (1/0); alpha:
- * go bottom-up when frontier edges * alpha > unexplored edges (default 14);
- * beta: back to top-down when frontier edges * beta < the edges still
- * unexplored after the level (default 24).
+ * go bottom-up when the frontier grows and its edges * alpha > unexplored
+ * edges (default 14); beta: back to top-down when the next frontier shrinks
+ * and has fewer than n / beta vertices (default 24).
  * alpha or beta 0 keeps the current value.  The reference has no
  * counterpart (top-down only, gpuBFS.cu:66-100). */
 int  gb_bfs_direction_config(gb_dgraph* dg, int enable, int alpha, int beta);
@@ -229,7 +229,7 @@ typedef struct gb_tune_result {
     int32_t edges_per_thread;  /* chosen edges per thread per tile          */
     int32_t alpha;             /* chosen top-down -> bottom-up threshold    */
     int32_t beta;              /* chosen bottom-up -> top-down threshold (single-source only, else 0) */
-    int32_t low_degree;        /* bottom-up: rows up to this long go to one lane */
+    int32_t low_degree;        /* batch bottom-up: rows up to this long go to one lane (0 for gb_bfs) */
     int64_t frontier_edges;    /* gb_bfs: dense_edges; gb_bfs_batch: sparse_edges */
 } gb_tune_result;
 
@@ -238,7 +238,8 @@ int  gb_bfs_autotune(gb_dgraph* dg, const int32_t* sources, int count, int reps,
 /* Tunes gb_bfs_batch on one batch of k (1..64) sources. */
 int  gb_batch_autotune(gb_dgraph* dg, const int32_t* sources, int k, int reps, gb_tune_result* out);
 /* Applies a tuning result (from this or an earlier process) to a handle.
- * default_ms, best_ms and tried are ignored; beta 0 keeps the current one. */
+ * default_ms, best_ms and tried are ignored; beta 0 keeps the current one;
+ * gb_bfs_apply_tuning ignores low_degree. */
 int  gb_bfs_apply_tuning(gb_dgraph* dg, const gb_tune_result* t);
 int  gb_batch_apply_tuning(gb_dgraph* dg, const gb_tune_result* t);
 

@@ -212,8 +212,9 @@ def test_direction_optimising_kernel_shapes_dense_every_level(gpu, block, k):
 
 def test_bottom_up_then_back_top_down(gpu):
     # An R-MAT core joined by one edge to a 300 x 60 grid: the core's big
-    # levels go bottom-up, then the thin grid frontiers with most grid edges
-    # unexplored send the search back top-down (bitmap turned into a queue).
+    # levels go bottom-up, then the thin, shrinking grid frontiers (far below
+    # n / beta vertices) send the search back top-down (bitmap turned into a
+    # queue).
     core = gg.Graph.rmat(14, seed=77)
     grid = gg.Graph.grid(300, 60, 0.05, seed=78)
     nc = core.n
