@@ -14,9 +14,10 @@
 // kernel launches ~1.5 us, a software grid barrier 17-57 us.  Hence no grid
 // barrier and no per-tile atomics on a shared counter in large levels:
 //
-//  * A level L runs as `expand(L)`, optionally `bottom_up(L)`, and
-//    `compact(L)`, issued back to back with no host synchronisation; the
-//    host only checks a done flag once per batch of levels.
+//  * A level L runs as `expand(L)` (top-down or bottom-up, chosen on the
+//    device) and `compact(L)`, issued back to back with no host
+//    synchronisation; the host only checks a done flag once per batch of
+//    levels.
 //  * Top-down: the frontier of a level is a queue of entries (vertex, first
 //    CSR entry, exclusive prefix of degrees).  expand(L) cuts the level's
 //    edges into tiles of T = BLOCK*K edges; tile_start[t] names the entry
@@ -32,7 +33,7 @@
 //    a packed (count << 32 | edges) counter gives both their queue slots and
 //    their edge offsets.
 //  * Dense levels skip the enqueue.  compact(L) rebuilds the queue from the
-//    bitmaps: new = visited & ~prev over 4096-vertex chunks -- per-thread
+//    bitmaps: new = visited & ~prev over 16384-vertex chunks -- per-thread
 //    serial sums, a wave64 scan and an LDS-staged block scan give each new
 //    vertex its offset inside the chunk, and the chunk reserves its queue
 //    slots and edge range with one packed 64-bit atomicAdd, like a sparse
@@ -49,10 +50,11 @@
 //    and back top-down once it has fewer than n / beta vertices and shrinks
 //    (Beamer's rules); compact() then turns the frontier bitmap into a queue.
 //  * Directions are chosen on the device from counters that no kernel of
-//    the level modifies, so expand(L), bottom_up(L) and compact(L) agree.
-//    The host launches bottom_up(L) only for levels where searches of this
-//    graph have gone bottom-up (a window learned per handle); elsewhere the
-//    level runs top-down, which is always correct.
+//    the level modifies, so every workgroup of expand(L) and compact(L)
+//    agrees.  The host allows bottom-up (and keeps the frontier bitmaps it
+//    needs) only for levels where searches of this graph have gone
+//    bottom-up (a window learned per handle); elsewhere the level runs
+//    top-down, which is always correct.
 //  * No workgroup waits on another, so the kernels need no particular
 //    residency and can share the device with other streams' work.
 //  * Every kernel stops at once when the error word is set, and the tiles
@@ -98,7 +100,6 @@ enum : unsigned long long { DIR_TD = 1, DIR_BU = 2 };
 constexpr int kCompactBlock = 1024;
 constexpr int kCompactRounds = 16;                          // vertices per thread
 constexpr int kChunk = kCompactBlock * kCompactRounds;      // vertices per compaction chunk
-constexpr int kBuBlock = 256;                               // 4 waves
 constexpr int kBuGroups = 4;                                // 64-vertex groups per wave pass
 constexpr int kBuFirst = 4;                                 // row entries each lane probes before the list (= int4 head)
 static_assert(kBuFirst == 4, "the head array holds one int4 per vertex");
@@ -367,84 +368,12 @@ __device__ void expand_tile(const BfsArgs& a, int32_t* levels, int32_t* parents,
     __syncthreads();
 }
 
-// Level L, step 1: pick the direction and, top-down, expand the queue of
-// level L.  `bu_launched`: bottom_up(L) follows in the stream; `bu_next`:
-// bottom_up(L+1) will be launched, so a sparse level also records its
-// winners in the frontier bitmap fb[(L+1) & 1] (zeroed by compact(L-1)).
-template <int BLOCK, int K>
-__global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(BfsArgs a, int L, int bu_launched, int bu_next) {
-    __shared__ TileLds<BLOCK, K> s;
-    constexpr uint32_t T = BLOCK * K;
-    // Levels queued past the end of the search are no-ops: once done, the
-    // ring slots are no longer reset and would hold stale counts.  After an
-    // error the queue is not trusted: nothing runs.
-    if (a.ctl[CTL_DONE] | a.ctl[CTL_ERR]) return;
-    const unsigned long long cnt = a.ctl[CTL_RING0 + L % 3];  // final: written by earlier launches
-    const uint32_t F = static_cast<uint32_t>(cnt >> 32), E = static_cast<uint32_t>(cnt);
-    if (F == 0) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl[CTL_DONE] = 1;
-        return;
-    }
-    if (F > a.n || E > a.m) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.ctl[CTL_ERR], ERR_RANGE);
-        return;
-    }
-    const unsigned long long dir = choose_dir(a, L, E, bu_launched != 0);
-    const bool dense = E > a.dense_edges;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        // ring[(L+2)%3] held level L-1's counter; it next counts level L+2.
-        a.ctl[CTL_RING0 + (L + 2) % 3] = 0;
-        atomicAdd(&a.ctl[CTL_NREACH], static_cast<unsigned long long>(F));
-        atomicMax(&a.ctl[CTL_MAXLEV], static_cast<unsigned long long>(L));
-        atomicMax(&a.ctl[CTL_MAXE], static_cast<unsigned long long>(E));
-        a.ctl[CTL_DIR0 + L % 3] = dir;
-        a.ctl[CTL_REP0 + (L + 1) % 3] =
-            dir == DIR_BU ? REP_BITMAP : (dense ? 0ull : (bu_next ? REP_QUEUE | REP_BITMAP : REP_QUEUE));
-        a.ctl[CTL_QRES] = 0;
-    }
-    if (dir == DIR_BU) return;  // bottom_up(L) runs this level
-    int32_t* const levels = a.p->levels;
-    int32_t* const parents = a.p->parents;
-    const uint32_t ntiles = static_cast<uint32_t>((uint64_t(E) + T - 1) / T);
-    uint32_t dn_cnt = 0, dn_zero = 0;
-    uint64_t dn_edges = 0;
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x)
-        expand_tile<BLOCK, K>(a, levels, parents, s, L, t, ntiles, F, E, dense, bu_next != 0, dn_cnt, dn_edges,
-                              dn_zero);
-    if (dense) {
-        // One atomic per workgroup: the next level's (count, edges) and the
-        // degree-0 winners (reached, never queued).
-        const unsigned long long mine =
-            (static_cast<unsigned long long>(wave_sum(dn_cnt)) << 32) | wave_sum(dn_edges);
-        const uint32_t z = wave_sum(dn_zero);
-        __shared__ unsigned long long s_tot[BLOCK / 64];
-        __shared__ uint32_t s_z[BLOCK / 64];
-        if ((threadIdx.x & 63) == 0) {
-            s_tot[threadIdx.x >> 6] = mine;
-            s_z[threadIdx.x >> 6] = z;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned long long tot = 0;
-            uint32_t zt = 0;
-            for (int i = 0; i < BLOCK / 64; ++i) {
-                tot += s_tot[i];
-                zt += s_z[i];
-            }
-            if (tot) atomicAdd(&a.ctl[CTL_DCNT0 + ((L + 1) & 1)], tot);
-            if (zt) {
-                atomicAdd(&a.ctl[CTL_NREACH], static_cast<unsigned long long>(zt));
-                atomicMax(&a.ctl[CTL_MAXLEV], static_cast<unsigned long long>(L + 1));
-            }
-        }
-    }
-}
-
-// Level L bottom-up: every unvisited vertex looks for a neighbour in the
-// frontier bitmap of level L; the vertices found form the bitmap of level
-// L+1, and their (count, CSR entries) go to ring[(L+1) % 3].
+// Level L bottom-up (run by the level kernel when level L goes bottom-up):
+// every unvisited vertex looks for a neighbour in the frontier bitmap of
+// level L; the vertices found form the bitmap of level L+1, and their
+// (count, CSR entries) go to ring[(L+1) % 3].
 //
-// The kernel is bound by memory latency, not bandwidth (a search level
+// The step is bound by memory latency, not bandwidth (a search level
 // touches few bytes per vertex), so it is organised for loads in flight:
 //   * a wave takes kBuGroups groups of 64 vertices per pass and issues the
 //     visited-word, offset, first-entries and frontier-probe loads of all of
@@ -467,18 +396,9 @@ struct BuWaveLds {
     int32_t par[kBuGroups * 64];   // by vertex within the pass: parent found by the list, or -1
 };
 
-__global__ __launch_bounds__(kBuBlock) void bfs_bottom_up_kernel(BfsArgs a, int L) {
-    __shared__ BuWaveLds s_wave[kBuBlock / 64];
-    __shared__ unsigned long long s_acc[kBuBlock / 64];
-    if (a.ctl[CTL_DONE] | a.ctl[CTL_ERR]) return;
-    const unsigned long long cnt = a.ctl[CTL_RING0 + L % 3];
-    const uint32_t F = static_cast<uint32_t>(cnt >> 32), E = static_cast<uint32_t>(cnt);
-    if (F == 0 || F > a.n || E > a.m) return;  // expand(L) has set the done or error word
-    if (choose_dir(a, L, E, true) != DIR_BU) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        atomicMin(&a.ctl[CTL_BUFIRST], static_cast<unsigned long long>(L));
-        atomicMax(&a.ctl[CTL_BULAST], static_cast<unsigned long long>(L));
-    }
+template <int WAVES>
+__device__ void bottom_up_level(const BfsArgs& a, int L, BuWaveLds* s_wave) {
+    __shared__ unsigned long long s_acc[WAVES];
     BuWaveLds& sw = s_wave[threadIdx.x >> 6];
     const uint32_t* fin = a.fb[L & 1];
     uint32_t* fout = a.fb[(L + 1) & 1];
@@ -488,7 +408,7 @@ __global__ __launch_bounds__(kBuBlock) void bfs_bottom_up_kernel(BfsArgs a, int 
     const uint32_t sg = lane >> 3, sl = lane & 7u;  // sub-wave and lane within it
     const uint64_t n = static_cast<uint64_t>(a.n);
     const uint64_t ngroups = (n + 63) / 64;
-    const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * (kBuBlock / 64);
+    const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * WAVES;
     uint32_t my_cnt = 0;
     uint64_t my_edges = 0;
     bool bad = false;
@@ -499,7 +419,7 @@ __global__ __launch_bounds__(kBuBlock) void bfs_bottom_up_kernel(BfsArgs a, int 
         }
         return (fin[static_cast<uint32_t>(u) >> 5] >> (u & 31)) & 1u;
     };
-    for (uint64_t g0 = (static_cast<uint64_t>(blockIdx.x) * (kBuBlock / 64) + (threadIdx.x >> 6)) * kBuGroups;
+    for (uint64_t g0 = (static_cast<uint64_t>(blockIdx.x) * WAVES + (threadIdx.x >> 6)) * kBuGroups;
          g0 < ngroups; g0 += nwaves * kBuGroups) {
         uint32_t vw[kBuGroups];
         bool unv[kBuGroups];
@@ -625,8 +545,96 @@ __global__ __launch_bounds__(kBuBlock) void bfs_bottom_up_kernel(BfsArgs a, int 
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long tot = 0;
-        for (int i = 0; i < kBuBlock / 64; ++i) tot += s_acc[i];
+        for (int i = 0; i < WAVES; ++i) tot += s_acc[i];
         if (tot) atomicAdd(&a.ctl[CTL_RING0 + (L + 1) % 3], tot);
+    }
+}
+
+// Level L, step 1: pick the direction, then expand level L top-down (the
+// queue) or bottom-up (the bitmap).  `bu_launched`: level L may go
+// bottom-up (it lies in the learned window, and the bitmaps it needs are
+// kept); `bu_next`: level L+1 may, so a sparse level also records its
+// winners in the frontier bitmap fb[(L+1) & 1] (zeroed by compact(L-1)).
+template <int BLOCK, int K>
+union LevelLds {
+    TileLds<BLOCK, K> td;
+    BuWaveLds bu[BLOCK / 64];
+};
+
+template <int BLOCK, int K>
+__global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(BfsArgs a, int L, int bu_launched, int bu_next) {
+    __shared__ LevelLds<BLOCK, K> lds;
+    TileLds<BLOCK, K>& s = lds.td;
+    constexpr uint32_t T = BLOCK * K;
+    // Levels queued past the end of the search are no-ops: once done, the
+    // ring slots are no longer reset and would hold stale counts.  After an
+    // error the queue is not trusted: nothing runs.
+    if (a.ctl[CTL_DONE] | a.ctl[CTL_ERR]) return;
+    const unsigned long long cnt = a.ctl[CTL_RING0 + L % 3];  // final: written by earlier launches
+    const uint32_t F = static_cast<uint32_t>(cnt >> 32), E = static_cast<uint32_t>(cnt);
+    if (F == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl[CTL_DONE] = 1;
+        return;
+    }
+    if (F > a.n || E > a.m) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.ctl[CTL_ERR], ERR_RANGE);
+        return;
+    }
+    const unsigned long long dir = choose_dir(a, L, E, bu_launched != 0);
+    const bool dense = E > a.dense_edges;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // ring[(L+2)%3] held level L-1's counter; it next counts level L+2.
+        a.ctl[CTL_RING0 + (L + 2) % 3] = 0;
+        atomicAdd(&a.ctl[CTL_NREACH], static_cast<unsigned long long>(F));
+        atomicMax(&a.ctl[CTL_MAXLEV], static_cast<unsigned long long>(L));
+        atomicMax(&a.ctl[CTL_MAXE], static_cast<unsigned long long>(E));
+        a.ctl[CTL_DIR0 + L % 3] = dir;
+        a.ctl[CTL_REP0 + (L + 1) % 3] =
+            dir == DIR_BU ? REP_BITMAP : (dense ? 0ull : (bu_next ? REP_QUEUE | REP_BITMAP : REP_QUEUE));
+        a.ctl[CTL_QRES] = 0;
+    }
+    if (dir == DIR_BU) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            atomicMin(&a.ctl[CTL_BUFIRST], static_cast<unsigned long long>(L));
+            atomicMax(&a.ctl[CTL_BULAST], static_cast<unsigned long long>(L));
+        }
+        bottom_up_level<BLOCK / 64>(a, L, lds.bu);
+        return;
+    }
+    int32_t* const levels = a.p->levels;
+    int32_t* const parents = a.p->parents;
+    const uint32_t ntiles = static_cast<uint32_t>((uint64_t(E) + T - 1) / T);
+    uint32_t dn_cnt = 0, dn_zero = 0;
+    uint64_t dn_edges = 0;
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x)
+        expand_tile<BLOCK, K>(a, levels, parents, s, L, t, ntiles, F, E, dense, bu_next != 0, dn_cnt, dn_edges,
+                              dn_zero);
+    if (dense) {
+        // One atomic per workgroup: the next level's (count, edges) and the
+        // degree-0 winners (reached, never queued).
+        const unsigned long long mine =
+            (static_cast<unsigned long long>(wave_sum(dn_cnt)) << 32) | wave_sum(dn_edges);
+        const uint32_t z = wave_sum(dn_zero);
+        __shared__ unsigned long long s_tot[BLOCK / 64];
+        __shared__ uint32_t s_z[BLOCK / 64];
+        if ((threadIdx.x & 63) == 0) {
+            s_tot[threadIdx.x >> 6] = mine;
+            s_z[threadIdx.x >> 6] = z;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long tot = 0;
+            uint32_t zt = 0;
+            for (int i = 0; i < BLOCK / 64; ++i) {
+                tot += s_tot[i];
+                zt += s_z[i];
+            }
+            if (tot) atomicAdd(&a.ctl[CTL_DCNT0 + ((L + 1) & 1)], tot);
+            if (zt) {
+                atomicAdd(&a.ctl[CTL_NREACH], static_cast<unsigned long long>(zt));
+                atomicMax(&a.ctl[CTL_MAXLEV], static_cast<unsigned long long>(L + 1));
+            }
+        }
     }
 }
 
@@ -716,17 +724,17 @@ __device__ void compact_chunk(const BfsArgs& a, int L, uint32_t c, bool from_bit
     }
 }
 
-// `bu_next`: bottom_up(L+1) will be launched; `bu_next2`: bottom_up(L+2)
-// will be.  With either, fb[L & 1] (read by bottom_up(L) if at all, next
-// written as level L+2's bitmap by bottom_up(L+1) or a sparse expand(L+1),
-// which only set bits) is zeroed here.  Level L+1's
+// `bu_next`: level L+1 may go bottom-up; `bu_next2`: level L+2 may.  With
+// either, fb[L & 1] (read by a bottom-up level L if at all, next written as
+// level L+2's bitmap by a bottom-up or sparse top-down expand(L+1), which
+// only set bits) is zeroed here.  Level L+1's
 // counter ring[(L+1)%3] was zeroed by expand(L-1); a dense top-down level L
 // left it alone and the chunks' reservations build it up.
 __global__ __launch_bounds__(kCompactBlock) void bfs_compact_kernel(BfsArgs a, int L, int bu_next, int bu_next2) {
     if (a.ctl[CTL_DONE] | a.ctl[CTL_ERR]) return;
     if (bu_next || bu_next2) {
-        // fb[L & 1] is the output bitmap of level L+1: bottom_up(L+1) and
-        // sparse top-down winners only set bits in it.
+        // fb[L & 1] is the output bitmap of level L+1: bottom-up and sparse
+        // top-down winners only set bits in it.
         uint32_t* fz = a.fb[L & 1];
         for (uint32_t i = blockIdx.x * kCompactBlock + threadIdx.x; i < a.nwords; i += gridDim.x * kCompactBlock)
             fz[i] = 0u;
@@ -1066,22 +1074,12 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     // compact(): chunks are independent; up to 2 workgroups of 1024 per CU.
     const uint32_t cgrid = static_cast<uint32_t>(std::min<int64_t>(ws.nchunks, int64_t(g.num_cus) * 2));
     const uint32_t clear_grid = static_cast<uint32_t>(std::min<int64_t>((n + 255) / 256, int64_t(g.num_cus) * 8));
-    // bottom_up(): a wave takes kBuGroups groups of 64 vertices per pass; as
-    // many workgroups as are resident at once (the passes are grid-strided).
-    int bu_occ = 0;
-    GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bu_occ, reinterpret_cast<const void*>(bfs_bottom_up_kernel),
-                                                        kBuBlock, 0));
-    const int64_t bu_waves = (n + 64 * kBuGroups - 1) / (64 * kBuGroups);
-    const uint32_t bgrid = static_cast<uint32_t>(std::max<int64_t>(
-        1, std::min<int64_t>((bu_waves + kBuBlock / 64 - 1) / (kBuBlock / 64), int64_t(g.num_cus) * bu_occ)));
-
-    // bottom_up(L) is launched for the levels [bu_lo, bu_hi] learned from
-    // earlier searches of this graph; other levels run top-down.
+    // Levels [bu_lo, bu_hi], learned from earlier searches of this graph, may
+    // go bottom-up; other levels run top-down.
     auto bu_at = [&](int L) { return a.bottom_up && L >= bu_lo && L <= bu_hi; };
     auto launch_level = [&](int L) {
         const int bu = bu_at(L) ? 1 : 0, bu1 = bu_at(L + 1) ? 1 : 0, bu2 = bu_at(L + 2) ? 1 : 0;
         hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(g.cfg.block), 0, stream, a, L, bu, bu1);
-        if (bu) hipLaunchKernelGGL(bfs_bottom_up_kernel, dim3(bgrid), dim3(kBuBlock), 0, stream, a, L);
         hipLaunchKernelGGL(bfs_compact_kernel, dim3(cgrid), dim3(kCompactBlock), 0, stream, a, L, bu1, bu2);
     };
     // The call's parameters go to device memory first (pinned source buffer:
