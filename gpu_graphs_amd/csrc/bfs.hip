@@ -913,6 +913,7 @@ DeviceGraph::DeviceGraph(const HostGraph& g, int dev) : device(dev), n(g.n), m(g
             graph_bytes += sizeof(int4) * n;
             GG_HIP(hipMemcpy(head, hd.data(), sizeof(int4) * n, hipMemcpyHostToDevice));
         }
+        build_ms_graph(*this, g);
         runner = std::make_unique<Runner>(*this);
     } catch (...) {
         this->~DeviceGraph();
@@ -927,6 +928,13 @@ DeviceGraph::~DeviceGraph() {
     if (cols) (void)hipFree(cols);
     if (iso) (void)hipFree(iso);
     if (head) (void)hipFree(head);
+    for (void* p : {static_cast<void*>(ms_offsets), static_cast<void*>(ms_cols), static_cast<void*>(ms_perm),
+                    static_cast<void*>(ms_iperm), static_cast<void*>(ms_heavy_tiles)})
+        if (p) (void)hipFree(p);
+    ms_offsets = nullptr;
+    ms_cols = nullptr;
+    ms_perm = ms_iperm = nullptr;
+    ms_heavy_tiles = nullptr;
     offsets = nullptr;
     cols = nullptr;
     iso = nullptr;

@@ -204,6 +204,16 @@ struct DeviceGraph {
     int32_t* cols = nullptr;     // m
     uint32_t* iso = nullptr;     // ceil(n/32) words: degree-0 vertices (symmetric graphs only)
     int4* head = nullptr;        // n x 4: first neighbours of each vertex, -1 padded (symmetric graphs only)
+    // Degree-ordered copy for the batched search: vertex ids by descending
+    // log2-degree bucket, so the per-vertex masks of the hubs that most rows
+    // point to share cache lines.
+    int64_t* ms_offsets = nullptr;  // n + 1
+    int32_t* ms_cols = nullptr;     // m, in new ids (rows keep their entry order)
+    int32_t* ms_perm = nullptr;     // n: original id -> new id
+    int32_t* ms_iperm = nullptr;    // n: new id -> original id
+    int64_t ms_heavy = 0;           // vertices [0, ms_heavy) have rows of >= kMsHeavyRow entries, by descending degree
+    int32_t ms_heavy_rounds = 0;    // tiles of kMsHeavyRow entries in the longest row
+    uint32_t* ms_heavy_tiles = nullptr;  // ms_heavy_rounds + 1: first tile of each round (tile r of every row that has one)
     bool symmetric = false;      // every edge has its reverse: bottom-up search is valid
     size_t graph_bytes = 0;
     int num_cus = 0;
@@ -235,6 +245,12 @@ struct DeviceGraph {
 };
 
 bool bfs_config_supported(int block, int k);
+// Rows this long or longer are "heavy" in the batched bottom-up step: their
+// entries are scanned in tiles of this size spread over all workgroups.
+constexpr int kMsHeavyRow = 1024;
+// Builds the degree-ordered copy of `g` (msbfs.hip); called by the
+// DeviceGraph constructor with the device current.
+void build_ms_graph(DeviceGraph& dg, const HostGraph& g);
 bool ms_config_supported(int block, int k);
 // HBM a batch workspace needs beyond the graph (plus the staging buffer when
 // results go to the host).

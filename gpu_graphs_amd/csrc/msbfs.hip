@@ -63,6 +63,7 @@ enum : int {
     MS_TVALID0 = 7,
     MS_FULLDEG = 9,    // CSR entries of the vertices every source has reached
     MS_BULEVELS = 10,  // levels run bottom-up
+    MS_ACT0 = 11,      // act[L & 1]: sources with a non-empty level-L frontier
     MS_WORDS = 16
 };
 enum : unsigned long long { MS_ERR_TILE = 1, MS_ERR_RANGE = 2 };
@@ -82,6 +83,11 @@ struct MsArgs {
     unsigned long long* fmask;   // per vertex: sources that reached it in the last scanned level
     unsigned long long* lv8;     // per vertex: 8 words = one level byte per source
     const MsParams* p;           // sources, k, output rows (device memory)
+    const int32_t* perm;         // original id -> id of the degree-ordered copy
+    const int32_t* iperm;        // id of the degree-ordered copy -> original id
+    const uint32_t* heavy_tiles; // rounds + 1: first tile of each round of the heavy rows
+    int64_t heavy;               // vertices [0, heavy) have rows of >= kMsHeavyRow entries
+    int32_t heavy_rounds;
     int64_t* q_rb;
     uint32_t* q_eoff;
     int32_t* q_v;
@@ -153,10 +159,11 @@ __global__ __launch_bounds__(256) void ms_finish_kernel(MsArgs a) {
     const uint64_t wstride = static_cast<uint64_t>(gridDim.x) * (blockDim.x / 64);
     for (uint64_t g = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6); g < ngroups;
          g += wstride) {
-        const int64_t v = static_cast<int64_t>(g * 64 + lane);
+        const int64_t v = static_cast<int64_t>(g * 64 + lane);  // original id: the output position
         const bool valid = v < a.n;
         unsigned long long w[8];
-        const ulonglong2* line = reinterpret_cast<const ulonglong2*>(a.lv8 + (valid ? v : 0) * 8);
+        const int64_t vn = valid ? a.perm[v] : 0;
+        const ulonglong2* line = reinterpret_cast<const ulonglong2*>(a.lv8 + vn * 8);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const ulonglong2 x = valid ? line[j] : make_ulonglong2(~0ull, ~0ull);
@@ -181,7 +188,7 @@ __global__ void ms_seed_kernel(MsArgs a) {
         for (int i = 0; i < MS_WORDS; ++i) a.ctl[i] = 0;
     __syncthreads();
     if (threadIdx.x < static_cast<uint32_t>(a.p->k)) {
-        const int32_t s = a.p->sources[threadIdx.x];
+        const int32_t s = a.perm[a.p->sources[threadIdx.x]];
         const unsigned long long old = atomicOr(&a.next[s], 1ull << threadIdx.x);
         if (old == 0) a.touched[atomicAdd(&a.ctl[MS_TCNT0], 1ull)] = s;
     }
@@ -204,7 +211,7 @@ __device__ __forceinline__ void write_levels(int32_t* levels, int64_t n, int64_t
 // Chunk c of update_compact(L): thread t takes slots c*kUcChunk + r*kUcBlock + t
 // (r < 16) of either the vertex range (scan) or the touched list.
 __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, bool list, uint32_t count,
-                            unsigned long long kmask, unsigned long long& fulldeg) {
+                            unsigned long long kmask, unsigned long long& fulldeg, unsigned long long& act) {
     __shared__ unsigned long long s_wave[kUcBlock / 64];
     __shared__ unsigned long long s_excl;
     __shared__ uint32_t s_ok;
@@ -236,12 +243,13 @@ __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, bool list, uint3
             a.fmask[v] = nv;  // a scan covers every vertex: the next level may go bottom-up
         }
         newbits[r] = nv;
+        act |= nv;
         verts[r] = static_cast<int32_t>(v);
         if (L < kByteLevels) {
             if (nv) set_level_bytes(a.lv8, v, nv, static_cast<unsigned long long>(L));
         } else {
             // every lane of the wave takes part in the shuffles of write_levels
-            if (__ballot(nv != 0)) write_levels(levels, a.n, v, nv, L);
+            if (__ballot(nv != 0)) write_levels(levels, a.n, nv ? a.iperm[v] : 0, nv, L);
             if (nv) set_level_bytes(a.lv8, v, nv, static_cast<unsigned long long>(kByteLevels));
         }
         if (nv) {
@@ -308,6 +316,7 @@ __global__ __launch_bounds__(kUcBlock) void ms_update_compact_kernel(MsArgs a, i
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.ctl[MS_CNT0 + ((L + 1) & 1)] = 0;
         a.ctl[MS_TCNT0 + ((L + 1) & 1)] = 0;
+        a.ctl[MS_ACT0 + ((L + 1) & 1)] = 0;  // last read by expand(L-1)
     }
     if (count > static_cast<unsigned long long>(a.n)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.ctl[MS_ERR], MS_ERR_RANGE);
@@ -316,38 +325,100 @@ __global__ __launch_bounds__(kUcBlock) void ms_update_compact_kernel(MsArgs a, i
     const uint32_t nch = list ? static_cast<uint32_t>((count + kUcChunk - 1) / kUcChunk) : a.nchunks;
     const int k = a.p->k;
     const unsigned long long kmask = k >= 64 ? ~0ull : ((1ull << k) - 1);
-    unsigned long long fulldeg = 0;
+    unsigned long long fulldeg = 0, act = 0;
     for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
-        ms_uc_chunk(a, L, c, list, static_cast<uint32_t>(count), kmask, fulldeg);
+        ms_uc_chunk(a, L, c, list, static_cast<uint32_t>(count), kmask, fulldeg, act);
         __syncthreads();
     }
-    // One atomic per workgroup for the edges of the newly fully seen vertices.
-    __shared__ unsigned long long s_fd[kUcBlock / 64];
+    // One atomic per workgroup for the edges of the newly fully seen
+    // vertices, one for the sources that reached new vertices.
+    __shared__ unsigned long long s_fd[kUcBlock / 64], s_act[kUcBlock / 64];
     const unsigned long long w = wave_sum(fulldeg);
-    if ((threadIdx.x & 63) == 0) s_fd[threadIdx.x >> 6] = w;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) act |= __shfl_xor(act, o, 64);
+    if ((threadIdx.x & 63) == 0) {
+        s_fd[threadIdx.x >> 6] = w;
+        s_act[threadIdx.x >> 6] = act;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long t = 0;
-        for (int i = 0; i < kUcBlock / 64; ++i) t += s_fd[i];
+        unsigned long long t = 0, b = 0;
+        for (int i = 0; i < kUcBlock / 64; ++i) {
+            t += s_fd[i];
+            b |= s_act[i];
+        }
         if (t) atomicAdd(&a.ctl[MS_FULLDEG], t);
+        if (b) atomicOr(&a.ctl[MS_ACT0 + (L & 1)], b);
     }
 }
 
 // Level L bottom-up: next(v) = the unseen sources of v found among the
-// frontier masks of its neighbours.  A wave owns 64 consecutive vertices;
-// rows of at most `low_degree` entries are scanned by their lane, longer
-// rows by the whole wave 64 entries at a time; a scan stops once all of the
-// vertex's unseen sources are covered.
+// frontier masks of its neighbours; `kmask` = the sources still searching.
+// Heavy rows go in tiles over all workgroups (below).  Of the other rows, a
+// wave owns 64 consecutive vertices; rows of at most `low_degree` entries
+// are scanned by their lane, longer rows by the whole wave 64 entries at a
+// time; a scan stops once all of the vertex's unseen sources are covered.
 __device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask) {
+    __shared__ unsigned long long s_or[32];
+    __shared__ int64_t s_v, s_rb, s_re;
+    __shared__ unsigned long long s_un;
     const uint32_t lane = threadIdx.x & 63u;
+    bool bad = false;
+    // Heavy rows (a prefix of the degree-ordered ids): tiles of kMsHeavyRow
+    // entries spread over every workgroup, so the longest rows do not hold
+    // up a few waves.  Tiles are numbered round by round (tile r of every row
+    // that has one), so a row's first tile is scanned early and its later
+    // tiles are skipped once the row's unseen sources are covered; each tile
+    // ORs what it finds into next.
+    const uint32_t ntiles = a.heavy_rounds > 0 ? a.heavy_tiles[a.heavy_rounds] : 0u;
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        if (threadIdx.x == 0) {
+            int32_t lo = 0, hi = a.heavy_rounds - 1;  // round of tile t: last round starting at or before t
+            while (lo < hi) {
+                const int32_t mid = (lo + hi + 1) / 2;
+                if (a.heavy_tiles[mid] <= t) lo = mid;
+                else hi = mid - 1;
+            }
+            const int64_t v = static_cast<int64_t>(t - a.heavy_tiles[lo]);  // rows of a round are a prefix
+            const unsigned long long un = kmask & ~a.seen[v];
+            const unsigned long long got =
+                __hip_atomic_load(&a.next[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int64_t rb = a.offsets[v] + int64_t(lo) * kMsHeavyRow;
+            s_v = v;
+            s_un = (un & ~got) ? un : 0ull;
+            s_rb = rb;
+            s_re = std::min<int64_t>(a.offsets[v + 1], rb + kMsHeavyRow);
+        }
+        __syncthreads();
+        const unsigned long long un = s_un;
+        if (un) {
+            unsigned long long acc = 0;
+            for (int64_t i = s_rb + threadIdx.x; i < s_re; i += blockDim.x) {
+                const int32_t u = a.cols[i];
+                if (u < 0 || u >= a.n) bad = true;
+                else acc |= a.fmask[u];
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) acc |= __shfl_xor(acc, o, 64);
+            if (lane == 0) s_or[threadIdx.x >> 6] = acc;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                unsigned long long b = 0;
+                for (uint32_t w = 0; w < blockDim.x / 64; ++w) b |= s_or[w];
+                if (b & un) atomicOr(&a.next[s_v], b & un);
+            }
+        }
+        __syncthreads();  // the shared row of the next tile
+    }
+    // Light rows: a wave owns 64 consecutive vertices.
     const uint64_t ngroups = (static_cast<uint64_t>(a.n) + 63) / 64;
     const uint64_t wstride = static_cast<uint64_t>(gridDim.x) * (blockDim.x / 64);
-    bool bad = false;
-    for (uint64_t g = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6); g < ngroups;
-         g += wstride) {
+    const uint64_t g_first = static_cast<uint64_t>(a.heavy) / 64;
+    for (uint64_t g = g_first + static_cast<uint64_t>(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
+         g < ngroups; g += wstride) {
         const int64_t v = static_cast<int64_t>(g * 64 + lane);
         unsigned long long un = 0;
-        if (v < a.n) un = kmask & ~a.seen[v];
+        if (v >= a.heavy && v < a.n) un = kmask & ~a.seen[v];
         if (!wave_ballot(un != 0)) continue;
         int64_t rb = 0, re = 0;
         if (un) {
@@ -430,8 +501,12 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(MsArgs a, int L) {
         if (bottom_up) atomicAdd(&a.ctl[MS_BULEVELS], 1ull);
     }
     if (bottom_up) {
+        // Only sources with a non-empty level-L frontier can reach anything
+        // at level L+1: a source that has finished (say, in a small
+        // component) would otherwise keep every row it never reaches open.
         const int k = a.p->k;
-        ms_bottom_up(a, k >= 64 ? ~0ull : ((1ull << k) - 1));
+        const unsigned long long kmask = k >= 64 ? ~0ull : ((1ull << k) - 1);
+        ms_bottom_up(a, kmask & a.ctl[MS_ACT0 + (L & 1)]);
         return;
     }
     const uint32_t ntiles = static_cast<uint32_t>((uint64_t(E) + T - 1) / T);
@@ -591,6 +666,82 @@ constexpr int kStatsBlocks = 1024;
 
 bool ms_config_supported(int block, int k) { return find_ms_kernels(block, k) != nullptr; }
 
+namespace {
+
+// Row i of the degree-ordered copy = row iperm[i] of the graph, columns
+// mapped through perm (entry order kept).  One wave per row.
+__global__ void ms_relabel_kernel(const int64_t* __restrict__ offsets, const int32_t* __restrict__ cols,
+                                  const int32_t* __restrict__ perm, const int32_t* __restrict__ iperm,
+                                  const int64_t* __restrict__ new_offsets, int32_t* __restrict__ new_cols, int64_t n) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const int64_t nwaves = int64_t(gridDim.x) * (blockDim.x / 64);
+    for (int64_t i = int64_t(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6); i < n; i += nwaves) {
+        const int64_t o = iperm[i];
+        const int64_t rb = offsets[o], re = offsets[o + 1], nb = new_offsets[i];
+        for (int64_t j = rb + lane; j < re; j += 64) new_cols[nb + (j - rb)] = perm[cols[j]];
+    }
+}
+
+}  // namespace
+
+void build_ms_graph(DeviceGraph& dg, const HostGraph& g) {
+    const int64_t n = g.n, m = g.m();
+    // Counting sort of the vertices by descending degree bucket, ids
+    // ascending inside a bucket.
+    std::vector<int64_t> start(kDegreeBuckets + 1, 0);
+    std::vector<uint8_t> bucket(static_cast<size_t>(n));
+    for (int64_t v = 0; v < n; ++v) {
+        bucket[v] = static_cast<uint8_t>(degree_bucket(g.offsets[v + 1] - g.offsets[v]));
+        ++start[kDegreeBuckets - 1 - bucket[v] + 1];
+    }
+    for (int b = 0; b < kDegreeBuckets; ++b) start[b + 1] += start[b];
+    std::vector<int32_t> perm(static_cast<size_t>(n)), iperm(static_cast<size_t>(n));
+    for (int64_t v = 0; v < n; ++v) {
+        const int64_t pos = start[kDegreeBuckets - 1 - bucket[v]]++;
+        iperm[pos] = static_cast<int32_t>(v);
+        perm[v] = static_cast<int32_t>(pos);
+    }
+    // Heavy rows lead (descending degree buckets); order them by exact
+    // descending degree, so that the rows with a tile in round r (tile r of
+    // every row that long) are a prefix.  Tiles are numbered round by round:
+    // the first tiles of all heavy rows come first, and a row covered by
+    // them skips its later tiles.
+    auto degree_of = [&](int32_t v) { return g.offsets[v + 1] - g.offsets[v]; };
+    int64_t nh = 0;
+    while (nh < n && degree_of(iperm[nh]) >= kMsHeavyRow) ++nh;
+    std::stable_sort(iperm.begin(), iperm.begin() + nh,
+                     [&](int32_t x, int32_t y) { return degree_of(x) > degree_of(y); });
+    for (int64_t i = 0; i < nh; ++i) perm[iperm[i]] = static_cast<int32_t>(i);
+    std::vector<int64_t> noff(static_cast<size_t>(n) + 1, 0);
+    for (int64_t i = 0; i < n; ++i) noff[i + 1] = noff[i] + degree_of(iperm[i]);
+    const int32_t rounds = nh ? static_cast<int32_t>((degree_of(iperm[0]) + kMsHeavyRow - 1) / kMsHeavyRow) : 0;
+    std::vector<uint32_t> tiles(static_cast<size_t>(rounds) + 1, 0);
+    for (int32_t r = 0, rows = static_cast<int32_t>(nh); r < rounds; ++r) {
+        while (rows > 0 && degree_of(iperm[rows - 1]) <= int64_t(r) * kMsHeavyRow) --rows;
+        tiles[r + 1] = tiles[r] + static_cast<uint32_t>(rows);
+    }
+    dg.ms_heavy = nh;
+    dg.ms_heavy_rounds = rounds;
+    GG_HIP(hipMalloc(&dg.ms_heavy_tiles, sizeof(uint32_t) * (rounds + 1)));
+    dg.graph_bytes += sizeof(uint32_t) * (rounds + 1);
+    GG_HIP(hipMemcpy(dg.ms_heavy_tiles, tiles.data(), sizeof(uint32_t) * (rounds + 1), hipMemcpyHostToDevice));
+    GG_HIP(hipMalloc(&dg.ms_perm, sizeof(int32_t) * n));
+    GG_HIP(hipMalloc(&dg.ms_iperm, sizeof(int32_t) * n));
+    GG_HIP(hipMalloc(&dg.ms_offsets, sizeof(int64_t) * (n + 1)));
+    GG_HIP(hipMalloc(&dg.ms_cols, sizeof(int32_t) * std::max<int64_t>(m, 1)));
+    dg.graph_bytes += sizeof(int32_t) * 2 * n + sizeof(int64_t) * (n + 1) + sizeof(int32_t) * std::max<int64_t>(m, 1);
+    GG_HIP(hipMemcpy(dg.ms_perm, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    GG_HIP(hipMemcpy(dg.ms_iperm, iperm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    GG_HIP(hipMemcpy(dg.ms_offsets, noff.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
+    if (m) {
+        const uint32_t grid = static_cast<uint32_t>(std::min<int64_t>((n + 3) / 4, int64_t(dg.num_cus) * 8));
+        hipLaunchKernelGGL(ms_relabel_kernel, dim3(grid), dim3(256), 0, nullptr, dg.offsets, dg.cols, dg.ms_perm,
+                           dg.ms_iperm, dg.ms_offsets, dg.ms_cols, n);
+        GG_HIP(hipGetLastError());
+        GG_HIP(hipStreamSynchronize(nullptr));
+    }
+}
+
 void Runner::free_ms_workspace() {
     if (ms.graph_exec) (void)hipGraphExecDestroy(ms.graph_exec);
     void* ptrs[] = {ms.seen,       ms.next,    ms.fmask,  ms.lv8,     ms.q_rb,    ms.q_eoff, ms.q_v,    ms.q_fm,
@@ -677,8 +828,13 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
 
     const int64_t sparse_cfg = g.ms_cfg.sparse_edges > 0 ? g.ms_cfg.sparse_edges : g.default_ms_sparse_edges();
     MsArgs a;
-    a.offsets = g.offsets;
-    a.cols = g.cols;
+    a.offsets = g.ms_offsets;
+    a.cols = g.ms_cols;
+    a.perm = g.ms_perm;
+    a.iperm = g.ms_iperm;
+    a.heavy_tiles = g.ms_heavy_tiles;
+    a.heavy = g.ms_heavy;
+    a.heavy_rounds = g.ms_heavy_rounds;
     a.seen = ms.seen;
     a.next = ms.next;
     a.fmask = ms.fmask;

@@ -176,6 +176,37 @@ def test_bottom_up_batches(gpu, enable, alpha):
             assert dg.batch_last_bottom_up() >= 1
 
 
+def test_bottom_up_heavy_rows_and_small_components(gpu):
+    # Rows of thousands of entries are scanned in tiles spread over all
+    # workgroups (several tiles per row, one row spanning the tile boundary
+    # of the next), and sources in small components finish early while the
+    # others still search.
+    rng = np.random.default_rng(85)
+    base = gg.Graph.rmat(14, seed=85)
+    n0 = base.n
+    src = [np.repeat(np.arange(n0), np.diff(base.offsets)).astype(np.int64)]
+    dst = [base.cols.astype(np.int64)]
+    for h, d in ((0, 5000), (1, 2048), (2, 1024), (3, 1023), (4, 3333)):
+        nb = rng.choice(n0, d, replace=False)
+        src += [np.full(d, h), nb]
+        dst += [nb, np.full(d, h)]
+    # ten 3-vertex paths, each its own component
+    for c in range(10):
+        a, b, c3 = n0 + 3 * c, n0 + 3 * c + 1, n0 + 3 * c + 2
+        src += [np.array([a, b, b, c3])]
+        dst += [np.array([b, a, c3, b])]
+    G = gg.Graph.from_edges(n0 + 30, np.concatenate(src).astype(np.int32), np.concatenate(dst).astype(np.int32))
+    assert G.is_symmetric and int(np.diff(G.offsets).max()) >= 5000
+    dg = gg.DeviceGraph(G, gpu)
+    dg.batch_direction_config(True, 8)
+    deg = np.diff(G.offsets)
+    giant = rng.choice(np.nonzero(deg[:n0])[0], 54, replace=False)
+    sources = np.concatenate([giant, n0 + 3 * np.arange(10)]).astype(np.int32)
+    log("heavy rows + small components")
+    check_batch(G, dg, sources)
+    assert dg.batch_last_bottom_up() >= 1
+
+
 def test_bottom_up_batch_on_mixed_graph_and_directed(gpu):
     # R-MAT core plus a grid tail joined by one edge: bottom-up big levels,
     # then thin levels; and a directed graph, where bottom-up must stay off.
