@@ -208,15 +208,18 @@ __device__ __forceinline__ void write_levels(int32_t* levels, int64_t n, int64_t
     }
 }
 
-// Chunk c of update_compact(L): thread t takes slots c*kUcChunk + r*kUcBlock + t
-// (r < 16) of either the vertex range (scan) or the touched list.
-__device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, bool list, uint32_t count,
+// Chunk c of update_compact(L): thread t takes slots c*span + r*kUcBlock + t
+// (r < 16, below (c+1)*span) of either the vertex range (scan) or the
+// touched list.  span = kUcChunk, or kUcBlock for a short list, which then
+// spreads over all workgroups instead of a few.
+__device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, int64_t span, bool list, uint32_t count,
                             unsigned long long kmask, unsigned long long& fulldeg, unsigned long long& act) {
     __shared__ unsigned long long s_wave[kUcBlock / 64];
     __shared__ unsigned long long s_excl;
     __shared__ uint32_t s_ok;
     const uint32_t tid = threadIdx.x;
-    const int64_t ibase = int64_t(c) * kUcChunk;
+    const int64_t ibase = int64_t(c) * span;
+    const int64_t iend = ibase + span;
     int32_t* levels = a.p->levels;
     uint32_t active = 0;  // bit r: the vertex of slot r goes to the queue
     uint32_t kept = 0, edges = 0;
@@ -227,7 +230,7 @@ __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, bool list, uint3
         const int64_t i = ibase + r * kUcBlock + tid;
         int64_t v = -1;
         if (list) {
-            if (i < count) v = a.touched[i];
+            if (i < count && i < iend) v = a.touched[i];
         } else if (i < a.n) {
             v = i;
         }
@@ -322,12 +325,13 @@ __global__ __launch_bounds__(kUcBlock) void ms_update_compact_kernel(MsArgs a, i
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.ctl[MS_ERR], MS_ERR_RANGE);
         return;
     }
-    const uint32_t nch = list ? static_cast<uint32_t>((count + kUcChunk - 1) / kUcChunk) : a.nchunks;
+    const int64_t span = list && count <= static_cast<unsigned long long>(kUcChunk) * gridDim.x ? kUcBlock : kUcChunk;
+    const uint32_t nch = list ? static_cast<uint32_t>((count + span - 1) / span) : a.nchunks;
     const int k = a.p->k;
     const unsigned long long kmask = k >= 64 ? ~0ull : ((1ull << k) - 1);
     unsigned long long fulldeg = 0, act = 0;
     for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
-        ms_uc_chunk(a, L, c, list, static_cast<uint32_t>(count), kmask, fulldeg, act);
+        ms_uc_chunk(a, L, c, span, list, static_cast<uint32_t>(count), kmask, fulldeg, act);
         __syncthreads();
     }
     // One atomic per workgroup for the edges of the newly fully seen
@@ -359,56 +363,62 @@ __global__ __launch_bounds__(kUcBlock) void ms_update_compact_kernel(MsArgs a, i
 // are scanned by their lane, longer rows by the whole wave 64 entries at a
 // time; a scan stops once all of the vertex's unseen sources are covered.
 __device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask) {
-    __shared__ unsigned long long s_or[32];
-    __shared__ int64_t s_v, s_rb, s_re;
-    __shared__ unsigned long long s_un;
     const uint32_t lane = threadIdx.x & 63u;
     bool bad = false;
     // Heavy rows (a prefix of the degree-ordered ids): tiles of kMsHeavyRow
-    // entries spread over every workgroup, so the longest rows do not hold
-    // up a few waves.  Tiles are numbered round by round (tile r of every row
+    // entries spread over every wave, so the longest rows do not hold up a
+    // few waves.  Tiles are numbered round by round (tile r of every row
     // that has one), so a row's first tile is scanned early and its later
-    // tiles are skipped once the row's unseen sources are covered; each tile
-    // ORs what it finds into next.
+    // tiles are skipped once the row's unseen sources are covered.  A wave
+    // checks 64 tiles at once (a lane each), then scans the open ones
+    // together, 16 entries per lane, and ORs what it finds into next.
     const uint32_t ntiles = a.heavy_rounds > 0 ? a.heavy_tiles[a.heavy_rounds] : 0u;
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        if (threadIdx.x == 0) {
+    const uint32_t nwaves = gridDim.x * (blockDim.x / 64);
+    for (uint32_t t0 = (blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6)) * 64u; t0 < ntiles;
+         t0 += nwaves * 64u) {
+        const uint32_t t = t0 + lane;
+        int64_t v = 0, rb = 0, re = 0;
+        unsigned long long un = 0;
+        if (t < ntiles) {
             int32_t lo = 0, hi = a.heavy_rounds - 1;  // round of tile t: last round starting at or before t
             while (lo < hi) {
                 const int32_t mid = (lo + hi + 1) / 2;
                 if (a.heavy_tiles[mid] <= t) lo = mid;
                 else hi = mid - 1;
             }
-            const int64_t v = static_cast<int64_t>(t - a.heavy_tiles[lo]);  // rows of a round are a prefix
-            const unsigned long long un = kmask & ~a.seen[v];
+            v = static_cast<int64_t>(t - a.heavy_tiles[lo]);  // the rows of a round are a prefix
             const unsigned long long got =
                 __hip_atomic_load(&a.next[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const int64_t rb = a.offsets[v] + int64_t(lo) * kMsHeavyRow;
-            s_v = v;
-            s_un = (un & ~got) ? un : 0ull;
-            s_rb = rb;
-            s_re = std::min<int64_t>(a.offsets[v + 1], rb + kMsHeavyRow);
+            un = kmask & ~a.seen[v];
+            if (!(un & ~got)) un = 0;
+            rb = a.offsets[v] + int64_t(lo) * kMsHeavyRow;
+            re = std::min<int64_t>(a.offsets[v + 1], rb + kMsHeavyRow);
         }
-        __syncthreads();
-        const unsigned long long un = s_un;
-        if (un) {
+        uint64_t om = wave_ballot(un != 0);
+        while (om) {
+            const int hl = __ffsll(static_cast<long long>(om)) - 1;
+            om &= om - 1;
+            const int64_t hv = __shfl(v, hl, 64);
+            const int64_t hrb = __shfl(rb, hl, 64);
+            const int64_t hre = __shfl(re, hl, 64);
+            const unsigned long long hun = __shfl(un, hl, 64);
+            int32_t u[kMsHeavyRow / 64];
+#pragma unroll
+            for (int j = 0; j < kMsHeavyRow / 64; ++j) {
+                const int64_t i = hrb + j * 64 + lane;
+                u[j] = i < hre ? a.cols[i] : -1;
+            }
             unsigned long long acc = 0;
-            for (int64_t i = s_rb + threadIdx.x; i < s_re; i += blockDim.x) {
-                const int32_t u = a.cols[i];
-                if (u < 0 || u >= a.n) bad = true;
-                else acc |= a.fmask[u];
+#pragma unroll
+            for (int j = 0; j < kMsHeavyRow / 64; ++j) {
+                if (u[j] < 0) continue;
+                if (u[j] >= a.n) bad = true;
+                else acc |= a.fmask[u[j]];
             }
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) acc |= __shfl_xor(acc, o, 64);
-            if (lane == 0) s_or[threadIdx.x >> 6] = acc;
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                unsigned long long b = 0;
-                for (uint32_t w = 0; w < blockDim.x / 64; ++w) b |= s_or[w];
-                if (b & un) atomicOr(&a.next[s_v], b & un);
-            }
+            if (lane == 0 && (acc & hun)) atomicOr(&a.next[hv], acc & hun);
         }
-        __syncthreads();  // the shared row of the next tile
     }
     // Light rows: a wave owns 64 consecutive vertices.
     const uint64_t ngroups = (static_cast<uint64_t>(a.n) + 63) / 64;
