@@ -88,6 +88,8 @@ struct MsArgs {
     const uint32_t* heavy_tiles; // rounds + 1: first tile of each round of the heavy rows
     int64_t heavy;               // vertices [0, heavy) have rows of >= kMsHeavyRow entries
     int32_t heavy_rounds;
+    int64_t nz;                  // vertices [nz, n) have no edges: reached only as sources, at level 0
+    uint32_t nz_chunks;          // update chunks covering [0, nz)
     int64_t* q_rb;
     uint32_t* q_eoff;
     int32_t* q_v;
@@ -231,7 +233,7 @@ __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, int64_t span, bo
         int64_t v = -1;
         if (list) {
             if (i < count && i < iend) v = a.touched[i];
-        } else if (i < a.n) {
+        } else if (i < a.nz) {  // a scan comes after level 0: vertices without edges get nothing
             v = i;
         }
         unsigned long long nv = 0, sv = 0;
@@ -326,7 +328,7 @@ __global__ __launch_bounds__(kUcBlock) void ms_update_compact_kernel(MsArgs a, i
         return;
     }
     const int64_t span = list && count <= static_cast<unsigned long long>(kUcChunk) * gridDim.x ? kUcBlock : kUcChunk;
-    const uint32_t nch = list ? static_cast<uint32_t>((count + span - 1) / span) : a.nchunks;
+    const uint32_t nch = list ? static_cast<uint32_t>((count + span - 1) / span) : a.nz_chunks;
     const int k = a.p->k;
     const unsigned long long kmask = k >= 64 ? ~0ull : ((1ull << k) - 1);
     unsigned long long fulldeg = 0, act = 0;
@@ -420,15 +422,16 @@ __device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask) {
             if (lane == 0 && (acc & hun)) atomicOr(&a.next[hv], acc & hun);
         }
     }
-    // Light rows: a wave owns 64 consecutive vertices.
-    const uint64_t ngroups = (static_cast<uint64_t>(a.n) + 63) / 64;
+    // Light rows: a wave owns 64 consecutive vertices (vertices without edges,
+    // the tail of the order, find nothing and are left out).
+    const uint64_t ngroups = (static_cast<uint64_t>(a.nz) + 63) / 64;
     const uint64_t wstride = static_cast<uint64_t>(gridDim.x) * (blockDim.x / 64);
     const uint64_t g_first = static_cast<uint64_t>(a.heavy) / 64;
     for (uint64_t g = g_first + static_cast<uint64_t>(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
          g < ngroups; g += wstride) {
         const int64_t v = static_cast<int64_t>(g * 64 + lane);
         unsigned long long un = 0;
-        if (v >= a.heavy && v < a.n) un = kmask & ~a.seen[v];
+        if (v >= a.heavy && v < a.nz) un = kmask & ~a.seen[v];
         if (!wave_ballot(un != 0)) continue;
         int64_t rb = 0, re = 0;
         if (un) {
@@ -732,6 +735,9 @@ void build_ms_graph(DeviceGraph& dg, const HostGraph& g) {
     }
     dg.ms_heavy = nh;
     dg.ms_heavy_rounds = rounds;
+    int64_t nz = n;
+    while (nz > 0 && degree_of(iperm[nz - 1]) == 0) --nz;
+    dg.ms_nonzero = nz;
     GG_HIP(hipMalloc(&dg.ms_heavy_tiles, sizeof(uint32_t) * (rounds + 1)));
     dg.graph_bytes += sizeof(uint32_t) * (rounds + 1);
     GG_HIP(hipMemcpy(dg.ms_heavy_tiles, tiles.data(), sizeof(uint32_t) * (rounds + 1), hipMemcpyHostToDevice));
@@ -845,6 +851,8 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     a.heavy_tiles = g.ms_heavy_tiles;
     a.heavy = g.ms_heavy;
     a.heavy_rounds = g.ms_heavy_rounds;
+    a.nz = g.ms_nonzero;
+    a.nz_chunks = static_cast<uint32_t>((g.ms_nonzero + kUcChunk - 1) / kUcChunk);
     a.seen = ms.seen;
     a.next = ms.next;
     a.fmask = ms.fmask;
