@@ -119,11 +119,13 @@ def single_source(gg, args):
         cfg["mode"] = args.mode
     if args.blocks_per_cu is not None:
         cfg["blocks_per_cu"] = args.blocks_per_cu
+    if args.dense_edges:
+        cfg["dense_edges"] = args.dense_edges
     dg.config(**cfg)
     if args.alpha or args.beta or args.no_bottom_up:
         dg.direction_config(not args.no_bottom_up, args.alpha or 0, args.beta or 0)
     tuning = None
-    if not (args.no_tune or args.block or args.k or args.alpha or args.beta or args.no_bottom_up):
+    if not (args.no_tune or args.block or args.k or args.alpha or args.beta or args.no_bottom_up or args.dense_edges):
         t0 = time.time()
         tuning = dg.autotune(pick_sources(G.offsets, 8, SEED + 1), reps=2)
         log(f"autotune ({time.time() - t0:.1f}s): {tuning}")
@@ -384,6 +386,7 @@ def main():
     ap.add_argument("--k", type=int, default=None)
     ap.add_argument("--mode", type=int, default=None)
     ap.add_argument("--blocks-per-cu", type=int, default=None)
+    ap.add_argument("--dense-edges", type=int, default=None, help="frontier edges above which a level compacts a bitmap")
     ap.add_argument("--alpha", type=int, default=None, help="bottom-up switch: frontier edges * alpha > unexplored")
     ap.add_argument("--beta", type=int, default=None, help="back top-down below n / beta frontier vertices")
     ap.add_argument("--no-bottom-up", action="store_true", help="top-down only")
