@@ -681,17 +681,71 @@ bool ms_config_supported(int block, int k) { return find_ms_kernels(block, k) !=
 
 namespace {
 
+// First new id of each degree key (key 0 = the highest degree bucket).
+struct KeyBounds {
+    int64_t first[kDegreeBuckets + 1];
+};
+
+__device__ __forceinline__ int degree_key(const KeyBounds& kb, int32_t u) {
+    int lo = 0, hi = kDegreeBuckets - 1;  // last key whose first id is <= u
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (kb.first[mid] <= u) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // Row i of the degree-ordered copy = row iperm[i] of the graph, columns
-// mapped through perm (entry order kept).  One wave per row.
-__global__ void ms_relabel_kernel(const int64_t* __restrict__ offsets, const int32_t* __restrict__ cols,
-                                  const int32_t* __restrict__ perm, const int32_t* __restrict__ iperm,
-                                  const int64_t* __restrict__ new_offsets, int32_t* __restrict__ new_cols, int64_t n) {
+// mapped through perm and ordered by the neighbours' degree key (highest
+// degrees first, file order inside a key): a bottom-up scan meets the hubs,
+// the vertices most likely to carry frontier bits, first and so stops
+// earlier.  One wave per row, a stable counting sort over the keys: one
+// pass counts (LDS atomics), a wave scan turns the counts into bases, a
+// second pass places 64 entries at a time, key by key, with ballots.
+constexpr int kRelabelBlock = 256;
+__global__ __launch_bounds__(kRelabelBlock) void ms_relabel_kernel(
+    const int64_t* __restrict__ offsets, const int32_t* __restrict__ cols, const int32_t* __restrict__ perm,
+    const int32_t* __restrict__ iperm, const int64_t* __restrict__ new_offsets, int32_t* __restrict__ new_cols,
+    int64_t n, KeyBounds kb) {
+    __shared__ uint32_t s_base[kRelabelBlock / 64][kDegreeBuckets];
     const uint32_t lane = threadIdx.x & 63u;
+    uint32_t* base = s_base[threadIdx.x >> 6];
     const int64_t nwaves = int64_t(gridDim.x) * (blockDim.x / 64);
     for (int64_t i = int64_t(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6); i < n; i += nwaves) {
         const int64_t o = iperm[i];
         const int64_t rb = offsets[o], re = offsets[o + 1], nb = new_offsets[i];
-        for (int64_t j = rb + lane; j < re; j += 64) new_cols[nb + (j - rb)] = perm[cols[j]];
+        if (re == rb) continue;
+        if (lane < kDegreeBuckets) base[lane] = 0;
+        wave_lds_sync();
+        for (int64_t c0 = rb; c0 < re; c0 += 64) {
+            const int64_t j = c0 + lane;
+            if (j < re) atomicAdd(&base[degree_key(kb, perm[cols[j]])], 1u);
+        }
+        wave_lds_sync();
+        const uint32_t cnt = lane < kDegreeBuckets ? base[lane] : 0u;
+        const uint32_t incl = wave_inclusive_scan(cnt);
+        wave_lds_sync();
+        if (lane < kDegreeBuckets) base[lane] = incl - cnt;
+        wave_lds_sync();
+        for (int64_t c0 = rb; c0 < re; c0 += 64) {
+            const int64_t j = c0 + lane;
+            const bool valid = j < re;
+            const int32_t u = valid ? perm[cols[j]] : 0;
+            const int key = valid ? degree_key(kb, u) : -1;
+            uint64_t todo = wave_ballot(valid);
+            while (todo) {
+                const int leader = __ffsll(static_cast<long long>(todo)) - 1;
+                const int kk = __shfl(key, leader, 64);
+                const uint64_t mk = wave_ballot(key == kk);
+                const uint32_t b = base[kk];
+                if (key == kk) new_cols[nb + b + rank_below(mk)] = u;
+                wave_lds_sync();
+                if (static_cast<int>(lane) == leader) base[kk] = b + static_cast<uint32_t>(__popcll(mk));
+                wave_lds_sync();
+                todo &= ~mk;
+            }
+        }
     }
 }
 
@@ -708,6 +762,8 @@ void build_ms_graph(DeviceGraph& dg, const HostGraph& g) {
         ++start[kDegreeBuckets - 1 - bucket[v] + 1];
     }
     for (int b = 0; b < kDegreeBuckets; ++b) start[b + 1] += start[b];
+    KeyBounds kb;
+    for (int b = 0; b <= kDegreeBuckets; ++b) kb.first[b] = start[b];
     std::vector<int32_t> perm(static_cast<size_t>(n)), iperm(static_cast<size_t>(n));
     for (int64_t v = 0; v < n; ++v) {
         const int64_t pos = start[kDegreeBuckets - 1 - bucket[v]]++;
@@ -751,8 +807,8 @@ void build_ms_graph(DeviceGraph& dg, const HostGraph& g) {
     GG_HIP(hipMemcpy(dg.ms_offsets, noff.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
     if (m) {
         const uint32_t grid = static_cast<uint32_t>(std::min<int64_t>((n + 3) / 4, int64_t(dg.num_cus) * 8));
-        hipLaunchKernelGGL(ms_relabel_kernel, dim3(grid), dim3(256), 0, nullptr, dg.offsets, dg.cols, dg.ms_perm,
-                           dg.ms_iperm, dg.ms_offsets, dg.ms_cols, n);
+        hipLaunchKernelGGL(ms_relabel_kernel, dim3(grid), dim3(kRelabelBlock), 0, nullptr, dg.offsets, dg.cols,
+                           dg.ms_perm, dg.ms_iperm, dg.ms_offsets, dg.ms_cols, n, kb);
         GG_HIP(hipGetLastError());
         GG_HIP(hipStreamSynchronize(nullptr));
     }
