@@ -69,7 +69,7 @@ enum : int {
 enum : unsigned long long { MS_ERR_TILE = 1, MS_ERR_RANGE = 2 };
 
 constexpr int kUcBlock = 512;
-constexpr int kUcRounds = 16;
+constexpr int kUcRounds = 4;
 constexpr int kUcChunk = kUcBlock * kUcRounds;
 constexpr int kMaxCapturedLevels = 32;  // deeper searches continue with direct launches
 constexpr int kByteLevels = 254;         // levels kept in the byte lines; 254 = stored directly, 255 = none
@@ -211,7 +211,7 @@ __device__ __forceinline__ void write_levels(int32_t* levels, int64_t n, int64_t
 }
 
 // Chunk c of update_compact(L): thread t takes slots c*span + r*kUcBlock + t
-// (r < 16, below (c+1)*span) of either the vertex range (scan) or the
+// (r < kUcRounds, below (c+1)*span) of either the vertex range (scan) or the
 // touched list.  span = kUcChunk, or kUcBlock for a short list, which then
 // spreads over all workgroups instead of a few.
 __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, int64_t span, bool list, uint32_t count,
