@@ -937,7 +937,11 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     const int waves_per_block = (g.ms_cfg.block + 63) / 64;
     const int per_cu = std::max(1, std::min(occ, 16 / waves_per_block));
     const uint32_t grid = static_cast<uint32_t>(per_cu * g.num_cus);
-    const uint32_t ugrid = static_cast<uint32_t>(std::min<int64_t>(ms.nchunks, int64_t(g.num_cus) * 2));
+    int uocc = 0;
+    GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&uocc, reinterpret_cast<const void*>(ms_update_compact_kernel),
+                                                       kUcBlock, 0));
+    const uint32_t ugrid =
+        static_cast<uint32_t>(std::max<int64_t>(1, std::min<int64_t>(ms.nchunks, int64_t(g.num_cus) * uocc)));
     const uint32_t fgrid = static_cast<uint32_t>(std::min<int64_t>((n + 255) / 256, int64_t(g.num_cus) * 8));
     const uint32_t cgrid = static_cast<uint32_t>(std::min<int64_t>((n * 16 + 255) / 256 + 1, int64_t(g.num_cus) * 8));
 
