@@ -581,16 +581,24 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(MsArgs a, int L) {
 #pragma unroll
             for (int k = 0; k < K; ++k) sv[k] = v[k] >= 0 ? a.seen[v[k]] : ~0ull;
             if (!sparse) {
+                // Bits already in next (as far as this load sees: next only
+                // gains bits during a level) need no atomic.
+                unsigned long long nx[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) nx[k] = (v[k] >= 0 && (fm[k] & ~sv[k])) ? a.next[v[k]] : ~0ull;
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    const unsigned long long mk = fm[k] & ~sv[k];
+                    const unsigned long long mk = fm[k] & ~sv[k] & ~nx[k];
                     if (mk) __hip_atomic_fetch_or(&a.next[v[k]], mk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
             } else {
                 bool first[K];
+                unsigned long long nx[K];  // as in the dense case; a vertex with bits is listed already
+#pragma unroll
+                for (int k = 0; k < K; ++k) nx[k] = (v[k] >= 0 && (fm[k] & ~sv[k])) ? a.next[v[k]] : ~0ull;
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    const unsigned long long mk = fm[k] & ~sv[k];
+                    const unsigned long long mk = fm[k] & ~sv[k] & ~nx[k];
                     first[k] = false;
                     if (mk)
                         first[k] = __hip_atomic_fetch_or(&a.next[v[k]], mk, __ATOMIC_RELAXED,
