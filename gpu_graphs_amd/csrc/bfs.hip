@@ -1072,9 +1072,14 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     a.alpha = static_cast<uint32_t>(g.cfg.alpha);
     a.beta = static_cast<uint32_t>(g.cfg.beta);
 
-    int occ = 0;
-    GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(ks->expand), g.cfg.block, 0));
-    if (occ < 1) fail(GB_EDEVICE, "BFS kernel cannot be resident on a CU");
+    if (occ_kernel != reinterpret_cast<const void*>(ks->expand)) {  // queried once per kernel shape
+        int o = 0;
+        GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, reinterpret_cast<const void*>(ks->expand), g.cfg.block, 0));
+        if (o < 1) fail(GB_EDEVICE, "BFS kernel cannot be resident on a CU");
+        occ_kernel = reinterpret_cast<const void*>(ks->expand);
+        occ_blocks = o;
+    }
+    const int occ = occ_blocks;
     const int waves_per_block = (g.cfg.block + 63) / 64;
     const int default_per_cu = std::max(1, std::min(occ, 16 / waves_per_block));
     const int per_cu = g.cfg.blocks_per_cu > 0 ? std::min(g.cfg.blocks_per_cu, occ) : default_per_cu;

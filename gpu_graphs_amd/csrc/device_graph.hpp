@@ -158,6 +158,8 @@ struct Runner {
     int last_bu_first = -1, last_bu_last = -1;  // levels the previous BFS searched bottom-up
     int depth_hint = 0;                     // levels launched before the first done check
     int short_streak = 0;                   // searches in a row shallower than depth_hint
+    const void* occ_kernel = nullptr;       // expand kernel whose occupancy occ_blocks holds
+    int occ_blocks = 0;
     BfsWorkspace ws;
     MsWorkspace ms;
 
