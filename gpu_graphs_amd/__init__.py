@@ -23,7 +23,7 @@ import numpy as np
 from . import _ffi
 from ._ffi import (GB_AUTO_N, GB_DROP_SELF_LOOPS, GB_EAGAIN, GB_EBUSY, GB_EDEVICE, GB_EINVAL, GB_EIO, GB_ENOMEM,
                    GB_ERANGE, GB_KEEP_ORDER, GB_MODE_ASYNC, GB_MODE_GRAPH, GB_MODE_LEVEL_LOOP, GB_OK, GB_ONE_BASED, GB_SYMMETRIZE,
-                   BatchStats, GBError, Stats, check, lib)
+                   GB_DEEP_AUTO, GB_DEEP_OFF, GB_DEEP_ON, BatchStats, GBError, Stats, check, lib)
 
 GB_DEGREE_BUCKETS = 34
 
@@ -31,7 +31,7 @@ __all__ = [
     "Graph", "DeviceGraph", "DeviceArray", "Server", "csr", "gpuBFS", "GBError", "Stats", "device_count",
     "exclusive_scan", "GB_AUTO_N", "GB_ONE_BASED", "GB_SYMMETRIZE", "GB_KEEP_ORDER", "GB_DROP_SELF_LOOPS",
     "GB_MODE_ASYNC", "GB_MODE_LEVEL_LOOP", "GB_MODE_GRAPH", "GB_OK", "GB_EINVAL", "GB_EIO", "GB_ENOMEM", "GB_EDEVICE", "GB_ERANGE",
-    "GB_EBUSY", "GB_EAGAIN",
+    "GB_EBUSY", "GB_EAGAIN", "GB_DEEP_AUTO", "GB_DEEP_ON", "GB_DEEP_OFF",
 ]
 
 
@@ -254,6 +254,16 @@ class DeviceGraph:
         e, a, b, s = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         check(lib().gb_bfs_get_direction_config(self._h, C.byref(e), C.byref(a), C.byref(b), C.byref(s)))
         return {"enable": bool(e.value), "alpha": a.value, "beta": b.value, "symmetric": bool(s.value)}
+
+    def deep_config(self, mode=GB_DEEP_AUTO):
+        """One launch per level for deep searches with thin frontiers (GB_DEEP_*)."""
+        check(lib().gb_bfs_deep_config(self._h, int(mode)))
+
+    def get_deep_config(self):
+        """{"mode": GB_DEEP_*, "used": whether the last bfs ran in deep mode}."""
+        m, u = C.c_int(), C.c_int()
+        check(lib().gb_bfs_get_deep_config(self._h, C.byref(m), C.byref(u)))
+        return {"mode": m.value, "used": bool(u.value)}
 
     def launch_hints(self):
         """(depth, bu_lo, bu_hi): levels queued before the first done check and

@@ -28,6 +28,10 @@ GB_MODE_ASYNC = 0
 GB_MODE_LEVEL_LOOP = 1
 GB_MODE_GRAPH = 2
 
+GB_DEEP_AUTO = 0
+GB_DEEP_ON = 1
+GB_DEEP_OFF = 2
+
 _NAMES = {
     GB_EINVAL: "GB_EINVAL", GB_EIO: "GB_EIO", GB_ENOMEM: "GB_ENOMEM", GB_EDEVICE: "GB_EDEVICE",
     GB_ERANGE: "GB_ERANGE", GB_EBUSY: "GB_EBUSY", GB_EAGAIN: "GB_EAGAIN",
@@ -160,6 +164,8 @@ SIGNATURES = {
     "gb_batch_direction_config": (C.c_int, [P, C.c_int, C.c_int]),
     "gb_batch_get_direction_config": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "gb_batch_last_bottom_up": (C.c_int, [P, C.POINTER(C.c_int)]),
+    "gb_bfs_deep_config": (C.c_int, [P, C.c_int]),
+    "gb_bfs_get_deep_config": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "gb_bfs_autotune": (C.c_int, [P, P, C.c_int, C.c_int, C.POINTER(TuneResult)]),
     "gb_batch_autotune": (C.c_int, [P, P, C.c_int, C.c_int, C.POINTER(TuneResult)]),
     "gb_bfs_apply_tuning": (C.c_int, [P, C.POINTER(TuneResult)]),

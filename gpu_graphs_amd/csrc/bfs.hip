@@ -104,6 +104,7 @@ constexpr int kBuGroups = 4;                                // 64-vertex groups 
 constexpr int kBuFirst = 4;                                 // row entries each lane probes before the list (= int4 head)
 static_assert(kBuFirst == 4, "the head array holds one int4 per vertex");
 constexpr int kMaxCapturedLevels = 32;                      // deeper searches continue with direct launches
+constexpr int kDeepDepth = 64;                              // learned depth from which GB_DEEP_AUTO applies
 
 struct BfsArgs {
     const int64_t* __restrict__ offsets;
@@ -562,7 +563,7 @@ union LevelLds {
 };
 
 template <int BLOCK, int K>
-__global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(BfsArgs a, int L, int bu_launched, int bu_next) {
+__global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(BfsArgs a, int L, int bu_launched, int bu_next, int deep) {
     __shared__ LevelLds<BLOCK, K> lds;
     TileLds<BLOCK, K>& s = lds.td;
     constexpr uint32_t T = BLOCK * K;
@@ -581,7 +582,7 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(BfsArgs a, int L, int
         return;
     }
     const unsigned long long dir = choose_dir(a, L, E, bu_launched != 0);
-    const bool dense = E > a.dense_edges;
+    const bool dense = !deep && E > a.dense_edges;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         // ring[(L+2)%3] held level L-1's counter; it next counts level L+2.
         a.ctl[CTL_RING0 + (L + 2) % 3] = 0;
@@ -592,6 +593,12 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(BfsArgs a, int L, int
         a.ctl[CTL_REP0 + (L + 1) % 3] =
             dir == DIR_BU ? REP_BITMAP : (dense ? 0ull : (bu_next ? REP_QUEUE | REP_BITMAP : REP_QUEUE));
         a.ctl[CTL_QRES] = 0;
+        if (deep) {
+            // Deep mode has no compact(L): its bookkeeping is done here (the
+            // search ends when expand(L+1) finds level L+1 empty).
+            atomicAdd(&a.ctl[CTL_MREACH], static_cast<unsigned long long>(E));
+            a.ctl[CTL_EXP0 + ((L + 1) & 1)] = a.ctl[CTL_EXP0 + (L & 1)] + E;
+        }
     }
     if (dir == DIR_BU) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -609,6 +616,7 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(BfsArgs a, int L, int
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x)
         expand_tile<BLOCK, K>(a, levels, parents, s, L, t, ntiles, F, E, dense, bu_next != 0, dn_cnt, dn_edges,
                               dn_zero);
+    if (deep) return;  // level L is sparse: its queue is built
     if (dense) {
         // One atomic per workgroup: the next level's (count, edges) and the
         // degree-0 winners (reached, never queued).
@@ -844,7 +852,7 @@ __global__ void bfs_seed_kernel(BfsArgs a) {
     for (uint64_t tt = threadIdx.x; tt * a.tile_edges < d; tt += blockDim.x) a.tile_start[0][tt] = 0;
 }
 
-using ExpandFn = void (*)(BfsArgs, int, int, int);
+using ExpandFn = void (*)(BfsArgs, int, int, int, int);
 
 struct KernelSet {
     int block, k;
@@ -1090,9 +1098,19 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     // Levels [bu_lo, bu_hi], learned from earlier searches of this graph, may
     // go bottom-up; other levels run top-down.
     auto bu_at = [&](int L) { return a.bottom_up && L >= bu_lo && L <= bu_hi; };
+    // Deep mode: graphs whose last searches were deep with thin frontiers
+    // only (no bottom-up window left, no dense level) run one launch per
+    // level, all levels sparse top-down.
+    const bool deep = g.cfg.deep == GB_DEEP_ON ||
+                      (g.cfg.deep == GB_DEEP_AUTO && bu_hi < bu_lo && depth_hint >= kDeepDepth);
+    last_deep = deep;
     auto launch_level = [&](int L) {
         const int bu = bu_at(L) ? 1 : 0, bu1 = bu_at(L + 1) ? 1 : 0, bu2 = bu_at(L + 2) ? 1 : 0;
-        hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(g.cfg.block), 0, stream, a, L, bu, bu1);
+        if (deep) {  // one launch per level: expand(L) keeps compact(L)'s counters
+            hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(g.cfg.block), 0, stream, a, L, 0, 0, 1);
+            return;
+        }
+        hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(g.cfg.block), 0, stream, a, L, bu, bu1, 0);
         hipLaunchKernelGGL(bfs_compact_kernel, dim3(cgrid), dim3(kCompactBlock), 0, stream, a, L, bu1, bu2);
     };
     // The call's parameters go to device memory first (pinned source buffer:
@@ -1180,7 +1198,7 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
             const bool reuse = ws.graph_exec && ws.g_depth == D && ws.g_lo == bu_lo && ws.g_hi == bu_hi &&
                                ws.g_block == g.cfg.block && ws.g_k == g.cfg.edges_per_thread &&
                                ws.g_dense == int64_t(a.dense_edges) && ws.g_bu == int(a.bottom_up) &&
-                               ws.g_alpha == int(a.alpha) && ws.g_beta == int(a.beta);
+                               ws.g_alpha == int(a.alpha) && ws.g_beta == int(a.beta) && ws.g_deep == int(deep);
             if (!reuse) {
                 if (ws.graph_exec) (void)hipGraphExecDestroy(ws.graph_exec);
                 ws.graph_exec = nullptr;
@@ -1210,6 +1228,7 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
                 ws.g_bu = int(a.bottom_up);
                 ws.g_alpha = int(a.alpha);
                 ws.g_beta = int(a.beta);
+                ws.g_deep = int(deep);
             }
             GG_HIP(hipEventRecord(ev_start, stream));
             GG_HIP(hipGraphLaunch(ws.graph_exec, stream));

@@ -398,6 +398,22 @@ int gb_batch_last_bottom_up(const gb_dgraph* dg, int* levels) {
     });
 }
 
+int gb_bfs_deep_config(gb_dgraph* dg, int mode) {
+    return guarded([&] {
+        need(dg, "null device graph");
+        need(mode == GB_DEEP_AUTO || mode == GB_DEEP_ON || mode == GB_DEEP_OFF, "unknown deep mode");
+        dg->d->cfg.deep = mode;
+    });
+}
+
+int gb_bfs_get_deep_config(const gb_dgraph* dg, int* mode, int* used) {
+    return guarded([&] {
+        need(dg, "null device graph");
+        if (mode) *mode = dg->d->cfg.deep;
+        if (used) *used = dg->d->runner->last_deep ? 1 : 0;
+    });
+}
+
 int gb_bfs_autotune(gb_dgraph* dg, const int32_t* sources, int count, int reps, gb_tune_result* out) {
     return guarded([&] {
         need(dg && sources, "null argument");

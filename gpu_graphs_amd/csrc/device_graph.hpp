@@ -50,6 +50,7 @@ struct BfsConfig {
     bool direction_opt = true;    // bottom-up levels on symmetric graphs
     int alpha = 14;               // bottom-up when the frontier grows and its edges * alpha > unexplored edges
     int beta = 24;                // back top-down when the frontier shrinks below n / beta vertices
+    int deep = GB_DEEP_AUTO;      // one launch per level for deep searches with thin frontiers
 };
 
 // Per-call parameters of a single-source BFS, read by the kernels from
@@ -81,7 +82,7 @@ struct BfsWorkspace {
     size_t bytes = 0;
     // Captured launch sequence (GB_MODE_GRAPH) and what it was captured for.
     hipGraphExec_t graph_exec = nullptr;
-    int g_depth = 0, g_lo = 0, g_hi = 0, g_block = 0, g_k = 0, g_bu = 0, g_alpha = 0, g_beta = 0;
+    int g_depth = 0, g_lo = 0, g_hi = 0, g_block = 0, g_k = 0, g_bu = 0, g_alpha = 0, g_beta = 0, g_deep = 0;
     int64_t g_dense = -1;
 };
 
@@ -158,6 +159,7 @@ struct Runner {
     int last_bu_first = -1, last_bu_last = -1;  // levels the previous BFS searched bottom-up
     int depth_hint = 0;                     // levels launched before the first done check
     int short_streak = 0;                   // searches in a row shallower than depth_hint
+    bool last_deep = false;                 // the last search ran in deep mode
     const void* occ_kernel = nullptr;       // expand kernel whose occupancy occ_blocks holds
     int occ_blocks = 0;
     BfsWorkspace ws;

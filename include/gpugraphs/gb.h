@@ -171,6 +171,19 @@ int  gb_bfs_get_config(const gb_dgraph* dg, int* block, int* edges_per_thread,
  * counterpart (top-down only, gpuBFS.cu:66-100). */
 int  gb_bfs_direction_config(gb_dgraph* dg, int enable, int alpha, int beta);
 int  gb_bfs_get_direction_config(const gb_dgraph* dg, int* enable, int* alpha, int* beta, int* symmetric);
+/* Deep mode of gb_bfs (no reference counterpart; the reference pays a host
+ * round trip per level, gpuBFS.cu:66-100): one kernel launch per level
+ * instead of two -- every level is expanded top-down straight into the next
+ * queue, and the level's counters are kept by the expansion itself.  GB_DEEP_AUTO
+ * (default) uses it once searches of the graph have been deep (>= 64 levels)
+ * with thin frontiers only, such as road networks; GB_DEEP_ON always;
+ * GB_DEEP_OFF never.  *used (nullable) tells whether the last gb_bfs ran in
+ * deep mode. */
+#define GB_DEEP_AUTO 0
+#define GB_DEEP_ON   1
+#define GB_DEEP_OFF  2
+int  gb_bfs_deep_config(gb_dgraph* dg, int mode);
+int  gb_bfs_get_deep_config(const gb_dgraph* dg, int* mode, int* used);
 /* Levels the last gb_bfs on this handle searched bottom-up (-1, -1 if none). */
 int  gb_bfs_last_bottom_up(const gb_dgraph* dg, int* first_level, int* last_level);
 /* Launch hints learned from the searches so far: levels launched before the

@@ -269,6 +269,76 @@ def test_deep_grid_with_direction_optimisation(gpu):
         assert st["num_levels"] == ost["num_levels"]
 
 
+def check_search(G, dg, s, parents=True):
+    lv, par, st = dg.bfs(s, parents=parents)
+    want, _, ost = O.bfs(G.offsets, G.cols, s)
+    np.testing.assert_array_equal(lv, want, err_msg=f"source {s}")
+    if parents:
+        assert O.check_parents(G.offsets, G.cols, s, lv, par) == 0, s
+    assert st["n_reached"] == ost["n_reached"] and st["m_reached"] == ost["m_reached"], s
+    assert st["num_levels"] == ost["num_levels"], s
+
+
+@pytest.mark.parametrize("mode", [gg.GB_MODE_GRAPH, gg.GB_MODE_ASYNC, gg.GB_MODE_LEVEL_LOOP])
+@pytest.mark.parametrize("block,k", [(256, 4), (1024, 1), (64, 16)])
+def test_deep_mode_one_launch_per_level(gpu, mode, block, k):
+    # Deep mode: every level sparse top-down in one launch, the last
+    # workgroup doing the level's bookkeeping.
+    G = gg.Graph.grid(400, 300, 0.05, seed=86)
+    dg = gg.DeviceGraph(G, gpu)
+    dg.config(block=block, edges_per_thread=k, mode=mode)
+    dg.deep_config(gg.GB_DEEP_ON)
+    log(f"deep grid block={block} k={k} mode={mode}")
+    for s in (0, 150 * 400 + 200, G.n - 1, 12345):
+        check_search(G, dg, s)
+        assert dg.get_deep_config() == {"mode": gg.GB_DEEP_ON, "used": True}
+
+
+def test_deep_mode_on_wide_levels(gpu):
+    # Forced on a graph with wide levels (a 3000-vertex path ending in an
+    # R-MAT core): every level, hubs included, goes through the queue.
+    core = gg.Graph.rmat(14, seed=87)
+    nc, plen = core.n, 3000
+    cs = np.repeat(np.arange(nc), np.diff(core.offsets))
+    path = np.arange(nc, nc + plen)
+    hub = int(np.argmax(np.diff(core.offsets)))
+    src = np.concatenate([cs, path[:-1], path[1:], [path[-1], hub]]).astype(np.int32)
+    dst = np.concatenate([core.cols, path[1:], path[:-1], [hub, path[-1]]]).astype(np.int32)
+    G = gg.Graph.from_edges(nc + plen, src, dst)
+    dg = gg.DeviceGraph(G, gpu)
+    dg.deep_config(gg.GB_DEEP_ON)
+    for s in (nc, nc + 1500, hub):
+        check_search(G, dg, s)
+    dg.deep_config(gg.GB_DEEP_OFF)
+    check_search(G, dg, nc + 7)
+    assert not dg.get_deep_config()["used"]
+
+
+def test_deep_mode_auto_and_directed(gpu):
+    # GB_DEEP_AUTO engages after deep, thin searches; R-MAT searches never
+    # use it; directed graphs work the same.
+    G = gg.Graph.grid(300, 250, 0.05, seed=88)
+    dg = gg.DeviceGraph(G, gpu)
+    assert dg.get_deep_config()["mode"] == gg.GB_DEEP_AUTO
+    used = []
+    for s in (0, 7, G.n // 2, G.n - 1, 999):
+        check_search(G, dg, s, parents=False)
+        used.append(dg.get_deep_config()["used"])
+    assert not used[0] and used[-1], used
+    R = gg.Graph.rmat(14, seed=89)
+    dr = gg.DeviceGraph(R, gpu)
+    for s in np.nonzero(np.diff(R.offsets))[0][:4]:
+        check_search(R, dr, int(s), parents=False)
+        assert not dr.get_deep_config()["used"]
+    n = 5000
+    src = np.arange(n - 1, dtype=np.int32)
+    G2 = gg.Graph.from_edges(n, src, src + 1)  # one directed path
+    dg2 = gg.DeviceGraph(G2, gpu)
+    dg2.deep_config(gg.GB_DEEP_ON)
+    for s in (0, 2500, n - 1):
+        check_search(G2, dg2, s)
+
+
 def test_graph_replay_serves_any_source_and_buffer(gpu):
     # The captured launch sequence reads the source and the output pointers
     # from device memory: replays with different sources, two level buffers,
