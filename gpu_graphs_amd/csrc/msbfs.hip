@@ -176,8 +176,9 @@ __global__ __launch_bounds__(256) void ms_finish_kernel(MsArgs a) {
         for (int s = 0; s < 64; ++s) {
             if (s < k) {
                 const uint32_t byte = static_cast<uint32_t>(w[s >> 3] >> (8 * (s & 7))) & 0xFFu;
-                if (valid && byte != static_cast<uint32_t>(kByteLevels))
-                    levels[int64_t(s) * a.n + v] = byte == 0xFFu ? -1 : static_cast<int32_t>(byte);
+                if (valid && byte != static_cast<uint32_t>(kByteLevels))  // output only: streamed past the caches
+                    __builtin_nontemporal_store(byte == 0xFFu ? -1 : static_cast<int32_t>(byte),
+                                                &levels[int64_t(s) * a.n + v]);
             }
         }
     }
