@@ -805,6 +805,75 @@ __global__ __launch_bounds__(kCompactBlock) void bfs_compact_kernel(BfsArgs a, i
     }
 }
 
+// Upload of a symmetric graph: each row's neighbours ordered by descending
+// degree bucket (file order inside a bucket).  A bottom-up level probes a row
+// from its start and stops at the first frontier neighbour; the hubs, which
+// are reached early and fill the big frontiers, now come first.  Levels do
+// not depend on the order.  One wave per row, a stable counting sort over the
+// buckets: one pass counts (LDS atomics), a wave scan turns the counts into
+// bases, a second pass places 64 entries at a time, bucket by bucket.
+constexpr int kRowSortBlock = 256;
+
+__device__ __forceinline__ int hub_first_key(const int64_t* offsets, int32_t u) {
+    const int64_t d = offsets[u + 1] - offsets[u];
+    const int b = d <= 0 ? 0 : min(64 - __clzll(static_cast<long long>(d)), kDegreeBuckets - 1);
+    return kDegreeBuckets - 1 - b;  // degree_bucket(d), highest first
+}
+
+__global__ __launch_bounds__(kRowSortBlock) void row_sort_kernel(const int64_t* __restrict__ offsets,
+                                                                 const int32_t* __restrict__ cols,
+                                                                 int32_t* __restrict__ out, int64_t n) {
+    __shared__ uint32_t s_base[kRowSortBlock / 64][kDegreeBuckets];
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t* base = s_base[threadIdx.x >> 6];
+    const int64_t nwaves = int64_t(gridDim.x) * (blockDim.x / 64);
+    for (int64_t v = int64_t(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6); v < n; v += nwaves) {
+        const int64_t rb = offsets[v], re = offsets[v + 1];
+        if (re == rb) continue;
+        if (lane < kDegreeBuckets) base[lane] = 0;
+        wave_lds_sync();
+        for (int64_t c0 = rb; c0 < re; c0 += 64) {
+            const int64_t j = c0 + lane;
+            if (j < re) atomicAdd(&base[hub_first_key(offsets, cols[j])], 1u);
+        }
+        wave_lds_sync();
+        const uint32_t cnt = lane < kDegreeBuckets ? base[lane] : 0u;
+        const uint32_t incl = wave_inclusive_scan(cnt);
+        wave_lds_sync();
+        if (lane < kDegreeBuckets) base[lane] = incl - cnt;
+        wave_lds_sync();
+        for (int64_t c0 = rb; c0 < re; c0 += 64) {
+            const int64_t j = c0 + lane;
+            const bool valid = j < re;
+            const int32_t u = valid ? cols[j] : 0;
+            const int key = valid ? hub_first_key(offsets, u) : -1;
+            uint64_t todo = wave_ballot(valid);
+            while (todo) {
+                const int leader = __ffsll(static_cast<long long>(todo)) - 1;
+                const int kk = __shfl(key, leader, 64);
+                const uint64_t mk = wave_ballot(key == kk);
+                const uint32_t b = base[kk];
+                if (key == kk) out[rb + b + rank_below(mk)] = u;
+                wave_lds_sync();
+                if (static_cast<int>(lane) == leader) base[kk] = b + static_cast<uint32_t>(__popcll(mk));
+                wave_lds_sync();
+                todo &= ~mk;
+            }
+        }
+    }
+}
+
+// The first 4 entries of every row side by side (-1 past the row's end).
+__global__ void row_head_kernel(const int64_t* __restrict__ offsets, const int32_t* __restrict__ cols,
+                                int4* __restrict__ head, int64_t n) {
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride) {
+        const int64_t rb = offsets[v], d = offsets[v + 1] - rb;
+        head[v] = make_int4(d > 0 ? cols[rb] : -1, d > 1 ? cols[rb + 1] : -1, d > 2 ? cols[rb + 2] : -1,
+                            d > 3 ? cols[rb + 3] : -1);
+    }
+}
+
 // levels = -1, parents = -1; visited and queued bitmaps = the isolated
 // vertices (symmetric graphs: nothing can reach them) or 0; frontier bitmaps
 // = 0 when bottom-up levels may run.
@@ -905,21 +974,33 @@ DeviceGraph::DeviceGraph(const HostGraph& g, int dev) : device(dev), n(g.n), m(g
             GG_HIP(hipMalloc(&iso, sizeof(uint32_t) * nwords));
             graph_bytes += sizeof(uint32_t) * nwords;
             GG_HIP(hipMemcpy(iso, h.data(), sizeof(uint32_t) * nwords, hipMemcpyHostToDevice));
+            // Rows hub-first for the bottom-up levels (the host graph keeps
+            // its order).
+            const uint32_t rgrid = static_cast<uint32_t>(std::min<int64_t>((n + 3) / 4, int64_t(num_cus) * 8));
+            if (m) {
+                int32_t* sorted = nullptr;
+                GG_HIP(hipMalloc(&sorted, sizeof(int32_t) * m));
+                hipLaunchKernelGGL(row_sort_kernel, dim3(rgrid), dim3(kRowSortBlock), 0, nullptr, offsets, cols,
+                                   sorted, n);
+                const hipError_t e1 = hipGetLastError();
+                const hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(nullptr) : e1;
+                if (e2 != hipSuccess) {
+                    (void)hipFree(sorted);
+                    GG_HIP(e2);
+                }
+                std::swap(cols, sorted);
+                GG_HIP(hipFree(sorted));
+            }
             // The first neighbours of every vertex side by side: a bottom-up
             // level reads them for 64 consecutive vertices in one coalesced
             // load instead of 64 scattered row starts.
-            std::vector<int4> hd(static_cast<size_t>(n));
-            parallel_for(n, [&](int64_t b, int64_t e, int) {
-                for (int64_t v = b; v < e; ++v) {
-                    const int64_t rb = g.offsets[v], d = g.offsets[v + 1] - rb;
-                    int32_t f[4];
-                    for (int k = 0; k < 4; ++k) f[k] = k < d ? g.cols[rb + k] : -1;
-                    hd[v] = make_int4(f[0], f[1], f[2], f[3]);
-                }
-            });
             GG_HIP(hipMalloc(&head, sizeof(int4) * n));
             graph_bytes += sizeof(int4) * n;
-            GG_HIP(hipMemcpy(head, hd.data(), sizeof(int4) * n, hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(row_head_kernel, dim3(static_cast<uint32_t>(std::min<int64_t>((n + 255) / 256,
+                                                                                          int64_t(num_cus) * 8))),
+                               dim3(256), 0, nullptr, offsets, cols, head, n);
+            GG_HIP(hipGetLastError());
+            GG_HIP(hipStreamSynchronize(nullptr));
         }
         build_ms_graph(*this, g);
         runner = std::make_unique<Runner>(*this);
