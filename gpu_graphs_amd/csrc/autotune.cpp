@@ -98,13 +98,13 @@ void autotune_bfs(DeviceGraph& g, const int32_t* sources, int count, int reps, g
                      [&]() { return shape(c.block, c.edges_per_thread); }});
     if (g.symmetric && c.direction_opt) {
         auto window = [&]() { r.reset_bottom_up_window(); };
-        knobs.push_back({{8, 14, 32, 64, 128},
+        knobs.push_back({{8, 14, 32, 64},
                          [&, window](int64_t v) {
                              c.alpha = int(v);
                              window();
                          },
                          [&]() { return int64_t(c.alpha); }});
-        knobs.push_back({{8, 24, 64, 256, 1024},
+        knobs.push_back({{8, 24, 64, 256},
                          [&, window](int64_t v) {
                              c.beta = int(v);
                              window();
