@@ -1187,8 +1187,11 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     last_deep = deep;
     auto launch_level = [&](int L) {
         const int bu = bu_at(L) ? 1 : 0, bu1 = bu_at(L + 1) ? 1 : 0, bu2 = bu_at(L + 2) ? 1 : 0;
-        if (deep) {  // one launch per level: expand(L) keeps compact(L)'s counters
-            hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(g.cfg.block), 0, stream, a, L, 0, 0, 1);
+        // One launch per level in deep mode, and for level 0 (the source
+        // alone, top-down; the clears have zeroed both frontier bitmaps):
+        // expand(L) builds level L+1's queue and keeps compact(L)'s counters.
+        if (deep || L == 0) {
+            hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(g.cfg.block), 0, stream, a, L, 0, deep ? 0 : bu1, 1);
             return;
         }
         hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(g.cfg.block), 0, stream, a, L, bu, bu1, 0);
@@ -1353,7 +1356,9 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
 
 // Launch hints for the next search of this graph, from the one just done:
 //  * depth_hint: levels queued before the first done check -- the deepest
-//    recent search + 1, lowered after 16 shallower searches in a row;
+//    recent search + 1, lowered after 256 shallower searches in a row (a
+//    change re-captures the replayed sequence, which costs far more host
+//    time than the no-op launches of one spare level);
 //  * bottom-up window: grown to cover the levels this search went bottom-up
 //    (the first learned window replaces the default); after two searches in
 //    a row with only small frontiers (deep, thin graphs) no bottom-up
@@ -1364,7 +1369,7 @@ void Runner::learn_from_search(bool had_dense_level) {
     if (want >= depth_hint) {
         depth_hint = want;
         short_streak = 0;
-    } else if (++short_streak >= 16) {
+    } else if (++short_streak >= 256) {
         depth_hint = want;
         short_streak = 0;
     }

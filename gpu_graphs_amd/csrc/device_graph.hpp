@@ -129,6 +129,8 @@ struct MsWorkspace {
     int64_t nchunks = 0;
     int tile_edges = 0;
     int last_depth = 0;
+    int depth_hint = 0;                   // levels queued before the first done check (see bfs_batch)
+    int short_streak = 0;                 // batches in a row shallower than depth_hint
     int last_bu_levels = 0;               // levels the last batch ran bottom-up
     size_t bytes = 0;
     hipGraphExec_t graph_exec = nullptr;  // captured batch sequence

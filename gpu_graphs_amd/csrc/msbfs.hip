@@ -970,7 +970,7 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
         ++L;
     };
 
-    const int depth_hint = std::min(std::max(ms.last_depth + 1, 4), kMaxCapturedLevels);
+    const int depth_hint = std::min(std::max(ms.depth_hint, 4), kMaxCapturedLevels);
     GG_HIP(hipEventRecord(ev_start, stream));
     if (g.ms_cfg.use_graph) {
         // One captured sequence: parameter upload, clears, depth_hint levels.
@@ -1055,6 +1055,17 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     if (!fin[MS_DONE]) fail(GB_EDEVICE, "internal error: batch did not reach an empty frontier");
     ms.last_depth = static_cast<int>(fin[MS_MAXLEV]) + 1;
     ms.last_bu_levels = static_cast<int>(fin[MS_BULEVELS]);
+    // Levels queued for the next batch: this depth + 1 (an empty level ends a
+    // batch), raised at once and lowered only after 16 shallower batches in a
+    // row, so that batches of slightly different depths replay one capture.
+    const int want = ms.last_depth + 1;
+    if (want >= ms.depth_hint) {
+        ms.depth_hint = want;
+        ms.short_streak = 0;
+    } else if (++ms.short_streak >= 16) {
+        ms.depth_hint = want;
+        ms.short_streak = 0;
+    }
     float ms_elapsed = 0.f;
     GG_HIP(hipEventElapsedTime(&ms_elapsed, ev_start, ev_stop));
     last_ms = ms_elapsed;
