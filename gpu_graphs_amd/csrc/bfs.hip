@@ -874,18 +874,25 @@ __global__ void row_head_kernel(const int64_t* __restrict__ offsets, const int32
     }
 }
 
-// levels = -1, parents = -1; visited and queued bitmaps = the isolated
-// vertices (symmetric graphs: nothing can reach them) or 0; frontier bitmaps
-// = 0 when bottom-up levels may run.
+// One launch before level 0 (a kernel boundary costs more than this work):
+//  * levels = -1, parents = -1; visited and queued bitmaps = the isolated
+//    vertices (symmetric graphs: nothing can reach them) or 0; frontier
+//    bitmaps = 0 when bottom-up levels may run;
+//  * level 0 = the source alone: claimed, level 0, parent = itself, the only
+//    queue entry, and the control block reset.  The thread that clears an
+//    entry of the source also makes the source's store to it.
 __global__ void bfs_clear_kernel(BfsArgs a) {
+    const int32_t s = a.p->source;
     int32_t* const levels = a.p->levels;
     int32_t* const parents = a.p->parents;
+    const int64_t sw = static_cast<uint32_t>(s) >> 5;
+    const uint32_t sbit = 1u << (static_cast<uint32_t>(s) & 31u);
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n; i += stride) {
-        levels[i] = -1;
-        if (parents) parents[i] = -1;
+        levels[i] = i == s ? 0 : -1;
+        if (parents) parents[i] = i == s ? s : -1;
         if (i < a.nwords) {
-            const uint32_t w = a.iso ? a.iso[i] : 0u;
+            const uint32_t w = (a.iso ? a.iso[i] : 0u) | (i == sw ? sbit : 0u);
             a.vis[i] = w;
             a.prev[i] = w;
             if (a.bottom_up) {
@@ -894,21 +901,10 @@ __global__ void bfs_clear_kernel(BfsArgs a) {
             }
         }
     }
-}
-
-// Level 0: the source alone (claimed, level 0, parent = itself).
-__global__ void bfs_seed_kernel(BfsArgs a) {
-    const int32_t s = a.p->source;
-    int32_t* const levels = a.p->levels;
-    int32_t* const parents = a.p->parents;
+    if (blockIdx.x != 0) return;
     const int64_t rb = a.offsets[s];
     const uint32_t d = static_cast<uint32_t>(a.offsets[s + 1] - rb);
     if (threadIdx.x == 0) {
-        const uint32_t bit = 1u << (static_cast<uint32_t>(s) & 31u);
-        a.vis[static_cast<uint32_t>(s) >> 5] |= bit;
-        a.prev[static_cast<uint32_t>(s) >> 5] |= bit;
-        levels[s] = 0;
-        if (parents) parents[s] = s;
         a.q_rb[0][0] = rb;
         a.q_eoff[0][0] = 0;
         a.q_v[0][0] = s;
@@ -1206,7 +1202,6 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     auto launch_prologue = [&]() {
         GG_HIP(hipMemcpyAsync(ws.params, ws.h_params, sizeof(BfsParams), hipMemcpyHostToDevice, stream));
         hipLaunchKernelGGL(bfs_clear_kernel, dim3(clear_grid), dim3(256), 0, stream, a);
-        hipLaunchKernelGGL(bfs_seed_kernel, dim3(1), dim3(256), 0, stream, a);
     };
 
     int L = 0;
