@@ -91,6 +91,8 @@ enum : int {
     CTL_BULAST = 17,
     CTL_EXP0 = 18,     // 2 words: CSR entries of the frontiers of levels < L at exp[L & 1]
     CTL_DCNT0 = 20,    // 2 words: (count << 32 | edges) of the winners of a dense level L at dcnt[(L+1) & 1]
+    CTL_FCNT0 = 22,    // 2 words, one-launch levels: (count << 32 | edges) of level L at fcnt[L & 1]
+    CTL_RCNT0 = 24,    // 2 words, one-launch levels: queued vertices of levels <= L at rcnt[L & 1]
     CTL_WORDS = 32
 };
 enum : unsigned long long { ERR_TILE = 1, ERR_RANGE = 2 };
@@ -111,7 +113,7 @@ struct BfsArgs {
     const int32_t* __restrict__ cols;
     uint32_t* vis;    // claimed vertices
     uint32_t* prev;   // vertices already placed in (or handled by) a frontier queue
-    uint32_t* fb[2];  // frontier bitmaps, by level parity
+    uint32_t* fb[3];  // frontier bitmaps: level L's at fb[L % 3]
     const uint32_t* iso;  // vertices of degree 0 (symmetric graphs), or null
     const int4* head;     // first kBuFirst neighbours of each vertex, -1 padded (symmetric graphs)
     int64_t* q_rb[2];
@@ -127,6 +129,8 @@ struct BfsArgs {
     uint32_t dense_edges;
     uint32_t tile_edges;        // T of the expand kernel (tile_start granularity)
     uint32_t bottom_up;         // direction optimisation enabled
+    uint32_t fused;             // one launch per level (symmetric graphs, not deep mode)
+    int64_t n_edged;            // vertices with at least one CSR entry
     uint32_t alpha, beta;
 };
 
@@ -149,16 +153,31 @@ __device__ __forceinline__ bool go_top_down(const BfsArgs& a, uint32_t F, uint32
 }
 
 // Direction of level L.  Reads only words that no kernel of level L writes.
-__device__ __forceinline__ unsigned long long choose_dir(const BfsArgs& a, int L, uint32_t E, bool bu_launched) {
+// F, E: level L's vertices and CSR entries; F_prev, E_prev_fused: level
+// L-1's (one-launch levels only).
+__device__ __forceinline__ unsigned long long choose_dir(const BfsArgs& a, int L, uint32_t F, uint32_t E,
+                                                         uint32_t F_prev, uint32_t E_prev_fused, bool bu_launched) {
     if (!a.bottom_up || !bu_launched || L == 0) return DIR_TD;
     const unsigned long long rep = a.ctl[CTL_REP0 + L % 3];
     if (!(rep & REP_BITMAP)) return DIR_TD;
-    if (a.ctl[CTL_DIR0 + (L - 1) % 3] == DIR_BU) return (rep & REP_QUEUE) ? DIR_TD : DIR_BU;
+    // Only the bitmap: a bottom-up level that stays bottom-up, or (one-launch
+    // levels) a dense top-down level.
+    if (!(rep & REP_QUEUE)) return DIR_BU;
+    if (a.ctl[CTL_DIR0 + (L - 1) % 3] == DIR_BU) {
+        // compact(L-1) queues level L only to go back top-down; a one-launch
+        // bottom-up level queues it ahead when its frontier shrinks, and the
+        // Beamer test is made here.
+        if (!a.fused) return DIR_TD;
+        return go_top_down(a, F_prev, F) ? DIR_TD : DIR_BU;
+    }
     // After a top-down level: frontier edges against the unexplored edges.
     // EXP[L & 1] = edges of levels <= L-1, EXP[(L + 1) & 1] = of levels <= L-2;
-    // compact(L) is the next kernel to write either.
+    // compact(L) is the next kernel to write either.  A one-launch level
+    // writes EXP[(L + 1) & 1] itself (block 0, while the others decide), so
+    // it takes level L-1's edges from fcnt instead.
     const unsigned long long through_prev = a.ctl[CTL_EXP0 + (L & 1)];
-    const uint32_t E_prev = static_cast<uint32_t>(through_prev - a.ctl[CTL_EXP0 + ((L + 1) & 1)]);
+    const uint32_t E_prev =
+        a.fused ? E_prev_fused : static_cast<uint32_t>(through_prev - a.ctl[CTL_EXP0 + ((L + 1) & 1)]);
     return go_bottom_up(a, static_cast<int64_t>(through_prev), E, E_prev) ? DIR_BU : DIR_TD;
 }
 
@@ -274,6 +293,8 @@ __device__ void expand_tile(const BfsArgs& a, int32_t* levels, int32_t* parents,
         for (int k = 0; k < K; ++k) {
             if (won[k]) {
                 const uint32_t uu = static_cast<uint32_t>(u[k]);
+                // Without compact() the winners mark level L+1's bitmap here.
+                if (bitmap_next && a.fused) atomicOr(&a.fb[(L + 1) % 3][uu >> 5], 1u << (uu & 31u));
                 const uint32_t d = static_cast<uint32_t>(a.offsets[uu + 1] - a.offsets[uu]);
                 if (d) {
                     ++dn_cnt;
@@ -294,7 +315,7 @@ __device__ void expand_tile(const BfsArgs& a, int32_t* levels, int32_t* parents,
         wrb[k] = wre[k] = 0;
         if (won[k]) {
             atomicOr(&a.prev[uu >> 5], 1u << (uu & 31u));  // queued (or degree 0): compact() must skip it
-            if (bitmap_next) atomicOr(&a.fb[nxt][uu >> 5], 1u << (uu & 31u));
+            if (bitmap_next) atomicOr(&a.fb[(L + 1) % 3][uu >> 5], 1u << (uu & 31u));
             wrb[k] = a.offsets[uu];
             wre[k] = a.offsets[uu + 1];
         }
@@ -398,11 +419,11 @@ struct BuWaveLds {
 };
 
 template <int WAVES>
-__device__ void bottom_up_level(const BfsArgs& a, int L, BuWaveLds* s_wave) {
+__device__ void bottom_up_level(const BfsArgs& a, int L, BuWaveLds* s_wave, bool enqueue) {
     __shared__ unsigned long long s_acc[WAVES];
     BuWaveLds& sw = s_wave[threadIdx.x >> 6];
-    const uint32_t* fin = a.fb[L & 1];
-    uint32_t* fout = a.fb[(L + 1) & 1];
+    const uint32_t* fin = a.fb[L % 3];
+    uint32_t* fout = a.fb[(L + 1) % 3];
     int32_t* const levels = a.p->levels;
     int32_t* const parents = a.p->parents;
     const uint32_t lane = threadIdx.x & 63u;
@@ -527,18 +548,71 @@ __device__ void bottom_up_level(const BfsArgs& a, int L, BuWaveLds* s_wave) {
             if (found) {
                 levels[v] = L + 1;
                 if (parents) parents[v] = par[j];
+            }
+            if (found && !enqueue) {  // with `enqueue` the block counts them below
                 ++my_cnt;
                 my_edges += static_cast<uint64_t>(re[j] - rb[j]);
             }
             const uint64_t w = (g0 + j) * 2 + (lane >> 5);
             const uint32_t bits = static_cast<uint32_t>(nb >> (lane & 32u));
-            if ((lane & 31u) == 0 && bits && w < a.nwords) {  // fout was zeroed by compact(L-1)
+            if ((lane & 31u) == 0 && bits && w < a.nwords) {  // fout was zeroed two levels ahead
                 fout[w] = bits;
                 a.vis[w] = vw[j] | bits;
                 a.prev[w] = vw[j] | bits;
             }
         }
         if (nitems) wave_lds_sync();  // the list of the next pass reuses the LDS
+    }
+    if (enqueue) {
+        // Queue level L+1 (a top-down level may follow) from the bitmap words
+        // this workgroup wrote: in each pass its waves covered WAVES *
+        // kBuGroups consecutive groups, 2 words each.  One packed (count <<
+        // 32 | edges) reservation per workgroup and pass, as in compact(); a
+        // winner has an edge (to its parent).  The barrier makes the waves'
+        // word stores visible.
+        __syncthreads();
+        constexpr uint32_t kWords = WAVES * kBuGroups * 2;
+        __shared__ unsigned long long s_wave[WAVES];
+        __shared__ unsigned long long s_base;
+        const int q = (L + 1) & 1;
+        for (uint64_t p0 = static_cast<uint64_t>(blockIdx.x) * WAVES * kBuGroups; p0 < ngroups;
+             p0 += static_cast<uint64_t>(gridDim.x) * WAVES * kBuGroups) {
+            const uint64_t w = p0 * 2 + threadIdx.x;
+            const uint32_t bits = (threadIdx.x < kWords && w < a.nwords) ? fout[w] : 0u;
+            uint32_t cnt = 0, edges = 0;
+            for (uint32_t b = bits; b; b &= b - 1) {
+                const int64_t v = static_cast<int64_t>(w * 32 + __ffs(b) - 1);
+                ++cnt;
+                edges += static_cast<uint32_t>(a.offsets[v + 1] - a.offsets[v]);
+            }
+            const unsigned long long mine = (static_cast<unsigned long long>(cnt) << 32) | edges;
+            unsigned long long total;
+            const unsigned long long excl = block_exclusive_scan<WAVES * 64>(mine, s_wave, total);
+            if (threadIdx.x == 0) {
+                s_base = total ? atomicAdd(&a.ctl[CTL_RING0 + (L + 1) % 3], total) : 0ull;
+                if (total && !reservation_fits(s_base, total, a.n, a.m)) {
+                    atomicOr(&a.ctl[CTL_ERR], ERR_RANGE);
+                    s_base = ~0ull;
+                }
+            }
+            __syncthreads();
+            const unsigned long long base = s_base;
+            __syncthreads();  // s_base is rewritten by the next pass
+            if (!bits || base == ~0ull) continue;
+            uint32_t slot = static_cast<uint32_t>((base + excl) >> 32);
+            uint32_t eo = static_cast<uint32_t>(base + excl);
+            for (uint32_t b = bits; b; b &= b - 1) {
+                const int64_t v = static_cast<int64_t>(w * 32 + __ffs(b) - 1);
+                const int64_t rb = a.offsets[v];
+                const uint32_t d = static_cast<uint32_t>(a.offsets[v + 1] - rb);
+                a.q_rb[q][slot] = rb;
+                a.q_eoff[q][slot] = eo;
+                a.q_v[q][slot] = static_cast<int32_t>(v);
+                mark_tiles(a.tile_start[q], a.tile_edges, eo, d, slot);
+                ++slot;
+                eo += d;
+            }
+        }
     }
     if (bad) atomicOr(&a.ctl[CTL_ERR], ERR_RANGE);
     const unsigned long long mine = (static_cast<unsigned long long>(wave_sum(my_cnt)) << 32) | wave_sum(my_edges);
@@ -581,8 +655,21 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(BfsArgs a, int L, int
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.ctl[CTL_ERR], ERR_RANGE);
         return;
     }
-    const unsigned long long dir = choose_dir(a, L, E, bu_launched != 0);
-    const bool dense = !deep && E > a.dense_edges;
+    // Level L-1's (count << 32 | edges), kept by one-launch levels.
+    const unsigned long long fc_prev = a.ctl[CTL_FCNT0 + ((L + 1) & 1)];
+    const uint32_t F_prev = static_cast<uint32_t>(fc_prev >> 32), E_prev = static_cast<uint32_t>(fc_prev);
+    const unsigned long long dir = choose_dir(a, L, F, E, F_prev, E_prev, bu_launched != 0);
+    // A dense level leaves level L+1 in the bitmaps only.  Without compact()
+    // (one-launch levels) that is allowed only if level L+1 may go bottom-up,
+    // which it then must.
+    const bool dense = !deep && E > a.dense_edges && (!a.fused || bu_next);
+    // A one-launch bottom-up level also queues level L+1 when that level
+    // cannot go bottom-up, or may go back top-down: the frontier shrinks, or
+    // so few vertices are left that level L+1 is below n / beta whatever it
+    // holds.
+    const unsigned long long through = a.ctl[CTL_RCNT0 + ((L + 1) & 1)] + F;  // queued, levels <= L
+    const bool few_left = (a.n_edged - static_cast<int64_t>(through)) * a.beta < a.n;
+    const bool bu_queue = a.fused && dir == DIR_BU && (!bu_next || F < F_prev || few_left);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         // ring[(L+2)%3] held level L-1's counter; it next counts level L+2.
         a.ctl[CTL_RING0 + (L + 2) % 3] = 0;
@@ -591,21 +678,32 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(BfsArgs a, int L, int
         atomicMax(&a.ctl[CTL_MAXE], static_cast<unsigned long long>(E));
         a.ctl[CTL_DIR0 + L % 3] = dir;
         a.ctl[CTL_REP0 + (L + 1) % 3] =
-            dir == DIR_BU ? REP_BITMAP : (dense ? 0ull : (bu_next ? REP_QUEUE | REP_BITMAP : REP_QUEUE));
+            dir == DIR_BU ? (bu_queue ? REP_QUEUE | REP_BITMAP : REP_BITMAP)
+                          : (dense ? (a.fused ? REP_BITMAP : 0ull) : (bu_next ? REP_QUEUE | REP_BITMAP : REP_QUEUE));
         a.ctl[CTL_QRES] = 0;
-        if (deep) {
-            // Deep mode has no compact(L): its bookkeeping is done here (the
-            // search ends when expand(L+1) finds level L+1 empty).
+        if (deep || a.fused) {
+            // One-launch levels have no compact(L): its bookkeeping is done
+            // here (the search ends when expand(L+1) finds level L+1 empty).
             atomicAdd(&a.ctl[CTL_MREACH], static_cast<unsigned long long>(E));
             a.ctl[CTL_EXP0 + ((L + 1) & 1)] = a.ctl[CTL_EXP0 + (L & 1)] + E;
         }
+        if (a.fused) {
+            a.ctl[CTL_FCNT0 + (L & 1)] = cnt;
+            a.ctl[CTL_RCNT0 + (L & 1)] = through;
+        }
+    }
+    if (a.fused) {
+        // fb[(L+2) % 3] held level L-1's frontier; level L+1 writes its
+        // winners there.
+        uint32_t* fz = a.fb[(L + 2) % 3];
+        for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < a.nwords; i += gridDim.x * BLOCK) fz[i] = 0u;
     }
     if (dir == DIR_BU) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             atomicMin(&a.ctl[CTL_BUFIRST], static_cast<unsigned long long>(L));
             atomicMax(&a.ctl[CTL_BULAST], static_cast<unsigned long long>(L));
         }
-        bottom_up_level<BLOCK / 64>(a, L, lds.bu);
+        bottom_up_level<BLOCK / 64>(a, L, lds.bu, bu_queue);
         return;
     }
     int32_t* const levels = a.p->levels;
@@ -637,7 +735,9 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(BfsArgs a, int L, int
                 tot += s_tot[i];
                 zt += s_z[i];
             }
-            if (tot) atomicAdd(&a.ctl[CTL_DCNT0 + ((L + 1) & 1)], tot);
+            // compact(L) reads the count (and builds the queue); one-launch
+            // levels count level L+1 where expand(L+1) reads it.
+            if (tot) atomicAdd(&a.ctl[a.fused ? CTL_RING0 + (L + 1) % 3 : CTL_DCNT0 + ((L + 1) & 1)], tot);
             if (zt) {
                 atomicAdd(&a.ctl[CTL_NREACH], static_cast<unsigned long long>(zt));
                 atomicMax(&a.ctl[CTL_MAXLEV], static_cast<unsigned long long>(L + 1));
@@ -663,7 +763,7 @@ __device__ void compact_chunk(const BfsArgs& a, int L, uint32_t c, bool from_bit
     __shared__ uint32_t s_ok;
     const uint32_t tid = threadIdx.x;
     const int64_t vbase = int64_t(c) * kChunk;
-    const uint32_t* fnew = a.fb[(L + 1) & 1];
+    const uint32_t* fnew = a.fb[(L + 1) % 3];
     uint32_t newmask = 0;  // bit r: vertex vbase + r*kCompactBlock + tid is new (degree > 0)
     uint32_t kept = 0, edges = 0;
 #pragma unroll
@@ -690,7 +790,7 @@ __device__ void compact_chunk(const BfsArgs& a, int L, uint32_t c, bool from_bit
     if (!from_bitmap) {
         // fb[(L+1)&1] = new, prev |= vis for the chunk's words: every read of
         // prev above happened before the block scan's barriers.
-        uint32_t* fout = a.fb[(L + 1) & 1];
+        uint32_t* fout = a.fb[(L + 1) % 3];
         for (uint32_t i = tid; i < kChunk / 32; i += kCompactBlock) {
             const int64_t w = vbase / 32 + i;
             if (w < a.nwords) {
@@ -733,9 +833,9 @@ __device__ void compact_chunk(const BfsArgs& a, int L, uint32_t c, bool from_bit
 }
 
 // `bu_next`: level L+1 may go bottom-up; `bu_next2`: level L+2 may.  With
-// either, fb[L & 1] (read by a bottom-up level L if at all, next written as
-// level L+2's bitmap by a bottom-up or sparse top-down expand(L+1), which
-// only set bits) is zeroed here.  Level L+1's
+// either, fb[(L+2) % 3] (level L-1's frontier, next written as level L+2's
+// bitmap by a bottom-up or sparse top-down expand(L+1), which only set bits)
+// is zeroed here.  Level L+1's
 // counter ring[(L+1)%3] was zeroed by expand(L-1); a dense top-down level L
 // left it alone and the chunks' reservations build it up.
 __global__ __launch_bounds__(kCompactBlock) void bfs_compact_kernel(BfsArgs a, int L, int bu_next, int bu_next2) {
@@ -743,7 +843,7 @@ __global__ __launch_bounds__(kCompactBlock) void bfs_compact_kernel(BfsArgs a, i
     if (bu_next || bu_next2) {
         // fb[L & 1] is the output bitmap of level L+1: bottom-up and sparse
         // top-down winners only set bits in it.
-        uint32_t* fz = a.fb[L & 1];
+        uint32_t* fz = a.fb[(L + 2) % 3];
         for (uint32_t i = blockIdx.x * kCompactBlock + threadIdx.x; i < a.nwords; i += gridDim.x * kCompactBlock)
             fz[i] = 0u;
     }
@@ -790,7 +890,7 @@ __global__ __launch_bounds__(kCompactBlock) void bfs_compact_kernel(BfsArgs a, i
             a.ctl[CTL_RING0 + (L + 1) % 3] = nx;
             a.ctl[CTL_REP0 + (L + 1) % 3] = REP_BITMAP;
         }
-        uint32_t* fout = a.fb[(L + 1) & 1];
+        uint32_t* fout = a.fb[(L + 1) % 3];
         for (uint32_t i = blockIdx.x * kCompactBlock + threadIdx.x; i < a.nwords; i += gridDim.x * kCompactBlock) {
             const uint32_t vw = a.vis[i];
             fout[i] = vw & ~a.prev[i];
@@ -898,6 +998,7 @@ __global__ void bfs_clear_kernel(BfsArgs a) {
             if (a.bottom_up) {
                 a.fb[0][i] = 0u;
                 a.fb[1][i] = 0u;
+                a.fb[2][i] = 0u;
             }
         }
     }
@@ -954,6 +1055,7 @@ DeviceGraph::DeviceGraph(const HostGraph& g, int dev) : device(dev), n(g.n), m(g
     num_cus = prop.multiProcessorCount;
     cfg.dense_edges = default_dense_edges();
     symmetric = is_symmetric(g);
+    for (int64_t v = 0; v < n; ++v) n_edged += g.offsets[v + 1] > g.offsets[v] ? 1 : 0;
     try {
         GG_HIP(hipMalloc(&offsets, sizeof(int64_t) * (n + 1)));
         GG_HIP(hipMalloc(&cols, sizeof(int32_t) * std::max<int64_t>(m, 1)));
@@ -1060,7 +1162,7 @@ Runner::~Runner() {
 
 void Runner::free_workspace() {
     if (ws.graph_exec) (void)hipGraphExecDestroy(ws.graph_exec);
-    void* ptrs[] = {ws.vis,    ws.prev,    ws.fb[0],  ws.fb[1],      ws.q_rb[0],       ws.q_rb[1],
+    void* ptrs[] = {ws.vis,    ws.prev,    ws.fb[0],  ws.fb[1],  ws.fb[2],  ws.q_rb[0],       ws.q_rb[1],
                     ws.q_eoff[0], ws.q_eoff[1], ws.q_v[0], ws.q_v[1], ws.ctl, ws.levels, ws.parents,
                     ws.tile_start[0], ws.tile_start[1], ws.params};
     for (void* p : ptrs)
@@ -1099,6 +1201,7 @@ void Runner::ensure_workspace() {
         alloc(&ws.prev, sizeof(uint32_t) * nwords);
         alloc(&ws.fb[0], sizeof(uint32_t) * nwords);
         alloc(&ws.fb[1], sizeof(uint32_t) * nwords);
+        alloc(&ws.fb[2], sizeof(uint32_t) * nwords);
         for (int i = 0; i < 2; ++i) {
             alloc(&ws.q_rb[i], sizeof(int64_t) * n);
             alloc(&ws.q_eoff[i], sizeof(uint32_t) * n);
@@ -1137,6 +1240,7 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     a.prev = ws.prev;
     a.fb[0] = ws.fb[0];
     a.fb[1] = ws.fb[1];
+    a.fb[2] = ws.fb[2];
     a.iso = g.iso;
     a.head = g.head;
     for (int i = 0; i < 2; ++i) {
@@ -1181,11 +1285,21 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     const bool deep = g.cfg.deep == GB_DEEP_ON ||
                       (g.cfg.deep == GB_DEEP_AUTO && bu_hi < bu_lo && depth_hint >= kDeepDepth);
     last_deep = deep;
+    // Symmetric graphs outside deep mode run one launch per level: the
+    // expansion keeps compact()'s counters, and the frontier forms the next
+    // level needs are built by the level itself.
+    a.fused = (a.bottom_up && !deep) ? 1u : 0u;
+    a.n_edged = g.n_edged;
+    last_fused = a.fused != 0;
     auto launch_level = [&](int L) {
         const int bu = bu_at(L) ? 1 : 0, bu1 = bu_at(L + 1) ? 1 : 0, bu2 = bu_at(L + 2) ? 1 : 0;
         // One launch per level in deep mode, and for level 0 (the source
         // alone, top-down; the clears have zeroed both frontier bitmaps):
         // expand(L) builds level L+1's queue and keeps compact(L)'s counters.
+        if (a.fused) {
+            hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(g.cfg.block), 0, stream, a, L, bu, bu1, 0);
+            return;
+        }
         if (deep || L == 0) {
             hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(g.cfg.block), 0, stream, a, L, 0, deep ? 0 : bu1, 1);
             return;
@@ -1360,7 +1474,9 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
 //    launches at all, until a search has a dense level again.
 // Both only shape launches; any level without a bottom-up launch runs top-down.
 void Runner::learn_from_search(bool had_dense_level) {
-    const int want = last_num_levels;  // compact() of the last level sets the done flag
+    // compact() of the last level sets the done flag; with one launch per
+    // level, expand() of the empty level after it does.
+    const int want = last_num_levels + (last_fused ? 1 : 0);
     if (want >= depth_hint) {
         depth_hint = want;
         short_streak = 0;

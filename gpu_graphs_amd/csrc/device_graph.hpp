@@ -66,7 +66,7 @@ struct BfsParams {
 struct BfsWorkspace {
     uint32_t* vis = nullptr;                    // ceil(n/32) words: claimed vertices
     uint32_t* prev = nullptr;                   // ceil(n/32) words: vertices already queued
-    uint32_t* fb[2] = {nullptr, nullptr};       // ceil(n/32) words each: frontier bitmaps by level parity
+    uint32_t* fb[3] = {nullptr, nullptr, nullptr};  // ceil(n/32) words each: level L's frontier bitmap at fb[L % 3]
     int64_t* q_rb[2] = {nullptr, nullptr};      // frontier: first CSR entry of the vertex
     uint32_t* q_eoff[2] = {nullptr, nullptr};   // frontier: exclusive prefix of degrees in the level
     int32_t* q_v[2] = {nullptr, nullptr};       // frontier: vertex id
@@ -162,6 +162,7 @@ struct Runner {
     int depth_hint = 0;                     // levels launched before the first done check
     int short_streak = 0;                   // searches in a row shallower than depth_hint
     bool last_deep = false;                 // the last search ran in deep mode
+    bool last_fused = false;                // the last search ran one launch per level
     const void* occ_kernel = nullptr;       // expand kernel whose occupancy occ_blocks holds
     int occ_blocks = 0;
     BfsWorkspace ws;
@@ -206,6 +207,7 @@ struct Runner {
 struct DeviceGraph {
     int device = 0;
     int64_t n = 0, m = 0;
+    int64_t n_edged = 0;  // vertices with at least one CSR entry
     int64_t* offsets = nullptr;  // n + 1
     int32_t* cols = nullptr;     // m
     uint32_t* iso = nullptr;     // ceil(n/32) words: degree-0 vertices (symmetric graphs only)

@@ -47,7 +47,7 @@ class HipExecutor : public Executor {
     size_t workspace_bytes(int gid, bool plain) override {
         DeviceGraph* g = graph(gid);
         // single-source workspace: two queues, bitmaps, tile index, staging
-        const size_t single = size_t(g->n) * (2 * (8 + 4 + 4) + 8) + size_t(g->m / 64 + 64) * 8 + (size_t(g->n) / 4);
+        const size_t single = size_t(g->n) * (2 * (8 + 4 + 4) + 8) + size_t(g->m / 64 + 64) * 8 + (size_t(g->n) / 4) + size_t(g->n) / 8;
         return plain ? single : single + batch_bytes_estimate(g->n, g->m, 64, false);
     }
 
