@@ -988,9 +988,29 @@ __global__ void bfs_clear_kernel(BfsArgs a) {
     const int64_t sw = static_cast<uint32_t>(s) >> 5;
     const uint32_t sbit = 1u << (static_cast<uint32_t>(s) & 31u);
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n; i += stride) {
-        levels[i] = i == s ? 0 : -1;
-        if (parents) parents[i] = i == s ? s : -1;
+    // Thread i clears entries 4i..4i+3 with 16-byte stores when the output
+    // rows allow it (entry i otherwise), and bitmap word i.
+    const bool vec = ((reinterpret_cast<uintptr_t>(levels) | reinterpret_cast<uintptr_t>(parents)) & 15u) == 0;
+    const int64_t nitems = vec ? (a.n + 3) / 4 : a.n;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nitems; i += stride) {
+        if (!vec) {
+            levels[i] = i == s ? 0 : -1;
+            if (parents) parents[i] = i == s ? s : -1;
+        } else if (i * 4 + 3 < a.n) {
+            int32_t lv[4], pv[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                lv[k] = i * 4 + k == s ? 0 : -1;
+                pv[k] = i * 4 + k == s ? s : -1;
+            }
+            *reinterpret_cast<int4*>(levels + i * 4) = make_int4(lv[0], lv[1], lv[2], lv[3]);
+            if (parents) *reinterpret_cast<int4*>(parents + i * 4) = make_int4(pv[0], pv[1], pv[2], pv[3]);
+        } else {
+            for (int64_t v = i * 4; v < a.n; ++v) {
+                levels[v] = v == s ? 0 : -1;
+                if (parents) parents[v] = v == s ? s : -1;
+            }
+        }
         if (i < a.nwords) {
             const uint32_t w = (a.iso ? a.iso[i] : 0u) | (i == sw ? sbit : 0u);
             a.vis[i] = w;
