@@ -256,7 +256,7 @@ struct DeviceGraph {
 bool bfs_config_supported(int block, int k);
 // Rows this long or longer are "heavy" in the batched bottom-up step: their
 // entries are scanned in tiles of this size spread over all workgroups.
-constexpr int kMsHeavyRow = 1024;
+constexpr int kMsHeavyRow = 256;
 // Builds the degree-ordered copy of `g` (msbfs.hip); called by the
 // DeviceGraph constructor with the device current.
 void build_ms_graph(DeviceGraph& dg, const HostGraph& g);

@@ -374,7 +374,8 @@ __device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask) {
     // that has one), so a row's first tile is scanned early and its later
     // tiles are skipped once the row's unseen sources are covered.  A wave
     // checks 64 tiles at once (a lane each), then scans the open ones
-    // together, 16 entries per lane, and ORs what it finds into next.
+    // together, kMsHeavyRow / 64 entries per lane, and ORs what it finds
+    // into next.
     const uint32_t ntiles = a.heavy_rounds > 0 ? a.heavy_tiles[a.heavy_rounds] : 0u;
     const uint32_t nwaves = gridDim.x * (blockDim.x / 64);
     for (uint32_t t0 = (blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6)) * 64u; t0 < ntiles;
