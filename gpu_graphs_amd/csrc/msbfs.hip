@@ -459,28 +459,45 @@ __device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask) {
                 if ((acc & un) == un) break;
             }
         }
+        // Longer rows: 8 at a time, one per 8-lane sub-wave, 8 entries per
+        // step; each sub-wave stops when its row's unseen sources are
+        // covered, and the wave goes on while any sub-wave has work.
         uint64_t hm = wave_ballot(un && deg > a.low_degree);
+        const uint32_t sg = lane >> 3, sl = lane & 7u;
         while (hm) {
-            const int hl = __ffsll(static_cast<long long>(hm)) - 1;
-            hm &= hm - 1;
+            uint64_t mm = hm;  // this sub-wave's row: the sg-th set bit of hm
+            for (uint32_t k = 0; k < sg; ++k) mm &= mm - 1;
+            const int hl = mm ? __ffsll(static_cast<long long>(mm)) - 1 : static_cast<int>(lane);
+            // (every lane takes part in the shuffles: a lane reads only from
+            // active lanes)
             const int64_t hrb = __shfl(rb, hl, 64);
-            const int64_t hre = __shfl(re, hl, 64);
+            const int64_t hre_row = __shfl(re, hl, 64);
+            const int64_t hre = mm ? hre_row : hrb;
             const unsigned long long hun = __shfl(un, hl, 64);
             unsigned long long hacc = 0;
-            for (int64_t b0 = hrb; b0 < hre; b0 += 64) {
-                const int64_t i = b0 + lane;
+            bool open = hrb < hre;
+            for (int64_t b0 = hrb; wave_ballot(open); b0 += 8) {
                 unsigned long long x = 0;
-                if (i < hre) {
+                const int64_t i = b0 + sl;
+                if (open && i < hre) {
                     const int32_t u = a.cols[i];
                     if (u < 0 || u >= a.n) bad = true;
                     else x = a.fmask[u];
                 }
 #pragma unroll
-                for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o, 64);
-                hacc |= x;
-                if ((hacc & hun) == hun) break;  // uniform: hacc is the same in every lane
+                for (int o = 4; o > 0; o >>= 1) x |= __shfl_xor(x, o, 8);
+                if (open) {
+                    hacc |= x;
+                    open = (hacc & hun) != hun && b0 + 8 < hre;  // same in the 8 lanes of the sub-wave
+                }
             }
-            if (static_cast<int>(lane) == hl) acc = hacc;
+            // The owner of each row takes its sub-wave's result.
+            const uint64_t round = hm;
+            for (uint32_t k = 0; k < 8 && hm; ++k) hm &= hm - 1;  // the rows taken this round
+            const uint64_t taken = round & ~hm;
+            const uint32_t rank = static_cast<uint32_t>(__popcll(round & ((1ull << lane) - 1)));
+            const unsigned long long got = __shfl(hacc, static_cast<int>((rank & 7u) * 8), 64);
+            if ((taken >> lane) & 1ull) acc = got;
         }
         const unsigned long long nb = acc & un;
         if (nb) a.next[v] = nb;
