@@ -108,8 +108,7 @@ struct MsParams {
 
 // Device workspace of the batched multi-source BFS.
 struct MsWorkspace {
-    unsigned long long* seen = nullptr;   // n: sources that reached each vertex
-    unsigned long long* next = nullptr;   // n: sources reaching it in the current level
+    unsigned long long* sn = nullptr;     // 2n: per vertex, sources that reached it and sources reaching it now
     unsigned long long* fmask = nullptr;  // n: sources that reached it in the last scanned level
     unsigned long long* lv8 = nullptr;    // 8n: one level byte per (vertex, source)
     int64_t* q_rb = nullptr;              // queue of active vertices

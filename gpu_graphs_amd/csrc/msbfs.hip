@@ -75,11 +75,17 @@ constexpr int kMaxCapturedLevels = 32;  // deeper searches continue with direct 
 constexpr int kByteLevels = 254;         // levels kept in the byte lines; 254 = stored directly, 255 = none
 constexpr int kMaxLevelBatch = 128;     // levels enqueued between done checks
 
+// Per vertex, side by side: a top-down probe reads both in one 16-byte load
+// (one cache line instead of two).
+struct __align__(16) SeenNext {
+    unsigned long long seen;  // sources that reached the vertex
+    unsigned long long next;  // sources reaching it in the level being expanded
+};
+
 struct MsArgs {
     const int64_t* __restrict__ offsets;
     const int32_t* __restrict__ cols;
-    unsigned long long* seen;
-    unsigned long long* next;
+    SeenNext* sn;
     unsigned long long* fmask;   // per vertex: sources that reached it in the last scanned level
     unsigned long long* lv8;     // per vertex: 8 words = one level byte per source
     const MsParams* p;           // sources, k, output rows (device memory)
@@ -120,8 +126,8 @@ __global__ void ms_clear_kernel(MsArgs a) {
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
     const int64_t tid0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     for (int64_t i = tid0; i < a.n; i += stride) {
-        a.seen[i] = 0;
-        a.next[i] = 0;
+        a.sn[i].seen = 0;
+        a.sn[i].next = 0;
     }
     ulonglong2* l2 = reinterpret_cast<ulonglong2*>(a.lv8);
     const ulonglong2 ones = make_ulonglong2(~0ull, ~0ull);
@@ -192,7 +198,7 @@ __global__ void ms_seed_kernel(MsArgs a) {
     __syncthreads();
     if (threadIdx.x < static_cast<uint32_t>(a.p->k)) {
         const int32_t s = a.perm[a.p->sources[threadIdx.x]];
-        const unsigned long long old = atomicOr(&a.next[s], 1ull << threadIdx.x);
+        const unsigned long long old = atomicOr(&a.sn[s].next, 1ull << threadIdx.x);
         if (old == 0) a.touched[atomicAdd(&a.ctl[MS_TCNT0], 1ull)] = s;
     }
     if (threadIdx.x == 0) a.ctl[MS_TVALID0] = 1;
@@ -239,12 +245,12 @@ __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, int64_t span, bo
         }
         unsigned long long nv = 0, sv = 0;
         if (v >= 0) {
-            const unsigned long long nx = a.next[v];
+            const unsigned long long nx = a.sn[v].next;
             if (nx) {
-                sv = a.seen[v];
+                sv = a.sn[v].seen;
                 nv = nx & ~sv;
-                a.seen[v] = sv | nv;
-                a.next[v] = 0;
+                a.sn[v].seen = sv | nv;
+                a.sn[v].next = 0;
             }
             a.fmask[v] = nv;  // a scan covers every vertex: the next level may go bottom-up
         }
@@ -392,8 +398,8 @@ __device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask) {
             }
             v = static_cast<int64_t>(t - a.heavy_tiles[lo]);  // the rows of a round are a prefix
             const unsigned long long got =
-                __hip_atomic_load(&a.next[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            un = kmask & ~a.seen[v];
+                __hip_atomic_load(&a.sn[v].next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            un = kmask & ~a.sn[v].seen;
             if (!(un & ~got)) un = 0;
             rb = a.offsets[v] + int64_t(lo) * kMsHeavyRow;
             re = std::min<int64_t>(a.offsets[v + 1], rb + kMsHeavyRow);
@@ -421,7 +427,7 @@ __device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask) {
             }
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) acc |= __shfl_xor(acc, o, 64);
-            if (lane == 0 && (acc & hun)) atomicOr(&a.next[hv], acc & hun);
+            if (lane == 0 && (acc & hun)) atomicOr(&a.sn[hv].next, acc & hun);
         }
     }
     // Light rows: a wave owns 64 consecutive vertices (vertices without edges,
@@ -433,7 +439,7 @@ __device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask) {
          g < ngroups; g += wstride) {
         const int64_t v = static_cast<int64_t>(g * 64 + lane);
         unsigned long long un = 0;
-        if (v >= a.heavy && v < a.nz) un = kmask & ~a.seen[v];
+        if (v >= a.heavy && v < a.nz) un = kmask & ~a.sn[v].seen;
         if (!wave_ballot(un != 0)) continue;
         int64_t rb = 0, re = 0;
         if (un) {
@@ -500,7 +506,7 @@ __device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask) {
             if ((taken >> lane) & 1ull) acc = got;
         }
         const unsigned long long nb = acc & un;
-        if (nb) a.next[v] = nb;
+        if (nb) a.sn[v].next = nb;
     }
     if (bad) atomicOr(&a.ctl[MS_ERR], MS_ERR_RANGE);
 }
@@ -596,31 +602,38 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(MsArgs a, int L) {
                 }
             }
             if (bad) atomicOr(&a.ctl[MS_ERR], MS_ERR_RANGE);
-            unsigned long long sv[K];
+            unsigned long long sv[K], nv0[K];
 #pragma unroll
-            for (int k = 0; k < K; ++k) sv[k] = v[k] >= 0 ? a.seen[v[k]] : ~0ull;
+            for (int k = 0; k < K; ++k) {
+                sv[k] = nv0[k] = ~0ull;
+                if (v[k] >= 0) {
+                    const SeenNext r = a.sn[v[k]];
+                    sv[k] = r.seen;
+                    nv0[k] = r.next;
+                }
+            }
             if (!sparse) {
                 // Bits already in next (as far as this load sees: next only
                 // gains bits during a level) need no atomic.
                 unsigned long long nx[K];
 #pragma unroll
-                for (int k = 0; k < K; ++k) nx[k] = (v[k] >= 0 && (fm[k] & ~sv[k])) ? a.next[v[k]] : ~0ull;
+                for (int k = 0; k < K; ++k) nx[k] = (v[k] >= 0 && (fm[k] & ~sv[k])) ? nv0[k] : ~0ull;
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     const unsigned long long mk = fm[k] & ~sv[k] & ~nx[k];
-                    if (mk) __hip_atomic_fetch_or(&a.next[v[k]], mk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (mk) __hip_atomic_fetch_or(&a.sn[v[k]].next, mk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
             } else {
                 bool first[K];
                 unsigned long long nx[K];  // as in the dense case; a vertex with bits is listed already
 #pragma unroll
-                for (int k = 0; k < K; ++k) nx[k] = (v[k] >= 0 && (fm[k] & ~sv[k])) ? a.next[v[k]] : ~0ull;
+                for (int k = 0; k < K; ++k) nx[k] = (v[k] >= 0 && (fm[k] & ~sv[k])) ? nv0[k] : ~0ull;
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     const unsigned long long mk = fm[k] & ~sv[k] & ~nx[k];
                     first[k] = false;
                     if (mk)
-                        first[k] = __hip_atomic_fetch_or(&a.next[v[k]], mk, __ATOMIC_RELAXED,
+                        first[k] = __hip_atomic_fetch_or(&a.sn[v[k]].next, mk, __ATOMIC_RELAXED,
                                                          __HIP_MEMORY_SCOPE_AGENT) == 0;
                 }
                 // Wave-aggregated appends: one counter atomic per wave and k.
@@ -675,7 +688,7 @@ __global__ __launch_bounds__(256) void ms_stats_kernel(MsArgs a, unsigned long l
         unsigned long long sv = 0;
         unsigned long long d = 0;
         if (v < a.n) {
-            sv = a.seen[v];
+            sv = a.sn[v].seen;
             d = static_cast<unsigned long long>(a.offsets[v + 1] - a.offsets[v]);
         }
         unsigned long long any = sv;
@@ -843,7 +856,7 @@ void build_ms_graph(DeviceGraph& dg, const HostGraph& g) {
 
 void Runner::free_ms_workspace() {
     if (ms.graph_exec) (void)hipGraphExecDestroy(ms.graph_exec);
-    void* ptrs[] = {ms.seen,       ms.next,    ms.fmask,  ms.lv8,     ms.q_rb,    ms.q_eoff, ms.q_v,    ms.q_fm,
+    void* ptrs[] = {ms.sn,         ms.fmask,  ms.lv8,     ms.q_rb,    ms.q_eoff, ms.q_v,    ms.q_fm,
                     ms.tile_start, ms.touched, ms.ctl,    ms.params,  ms.levels, ms.partial};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -875,8 +888,7 @@ void Runner::ensure_ms_workspace() {
         bytes += sz;
     };
     try {
-        alloc(&ms.seen, sizeof(unsigned long long) * n);
-        alloc(&ms.next, sizeof(unsigned long long) * n);
+        alloc(&ms.sn, sizeof(SeenNext) * n);
         alloc(&ms.fmask, sizeof(unsigned long long) * n);
         alloc(&ms.lv8, sizeof(unsigned long long) * 8 * n);
         alloc(&ms.q_rb, sizeof(int64_t) * n);
@@ -936,8 +948,7 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     a.heavy_rounds = g.ms_heavy_rounds;
     a.nz = g.ms_nonzero;
     a.nz_chunks = static_cast<uint32_t>((g.ms_nonzero + kUcChunk - 1) / kUcChunk);
-    a.seen = ms.seen;
-    a.next = ms.next;
+    a.sn = reinterpret_cast<SeenNext*>(ms.sn);
     a.fmask = ms.fmask;
     a.lv8 = ms.lv8;
     a.p = ms.params;
