@@ -219,7 +219,7 @@ struct DeviceGraph {
     int32_t* ms_perm = nullptr;     // n: original id -> new id
     int32_t* ms_iperm = nullptr;    // n: new id -> original id
     int64_t ms_heavy = 0;           // vertices [0, ms_heavy) have rows of >= kMsHeavyRow entries, by descending degree
-    int64_t ms_nonzero = 0;         // vertices [ms_nonzero, n) have no edges
+    int64_t ms_nonzero = 0;         // symmetric graphs: vertices [ms_nonzero, n) have no edges (else n)
     int32_t ms_heavy_rounds = 0;    // tiles of kMsHeavyRow entries in the longest row
     uint32_t* ms_heavy_tiles = nullptr;  // ms_heavy_rounds + 1: first tile of each round (tile r of every row that has one)
     bool symmetric = false;      // every edge has its reverse: bottom-up search is valid

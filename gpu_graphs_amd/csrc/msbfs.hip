@@ -831,8 +831,12 @@ void build_ms_graph(DeviceGraph& dg, const HostGraph& g) {
     }
     dg.ms_heavy = nh;
     dg.ms_heavy_rounds = rounds;
+    // On a symmetric graph a vertex without edges has no in-edges either: it
+    // is reached only as a source, at level 0, and the scans skip it.  On a
+    // directed graph a vertex without out-edges can be reached at any level.
     int64_t nz = n;
-    while (nz > 0 && degree_of(iperm[nz - 1]) == 0) --nz;
+    if (dg.symmetric)
+        while (nz > 0 && degree_of(iperm[nz - 1]) == 0) --nz;
     dg.ms_nonzero = nz;
     GG_HIP(hipMalloc(&dg.ms_heavy_tiles, sizeof(uint32_t) * (rounds + 1)));
     dg.graph_bytes += sizeof(uint32_t) * (rounds + 1);

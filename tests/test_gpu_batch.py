@@ -66,6 +66,26 @@ def test_batch_equals_single_source(gpu):
     np.testing.assert_array_equal(levels, O.bfs_multi(G.offsets, G.cols, sources))
 
 
+def test_directed_sinks_on_scanned_levels(gpu):
+    # Vertices without out-edges (sinks) reached by levels whose update scans
+    # all vertices: on a directed graph they are reachable, unlike the
+    # edgeless vertices of a symmetric graph, and the scan must visit them.
+    n = 5000
+    rng = np.random.default_rng(12)
+    src = np.concatenate([np.zeros(2000, np.int64), rng.integers(0, 100, 8000)]).astype(np.int32)
+    dst = np.concatenate([np.arange(1, 2001), rng.integers(100, n, 8000)]).astype(np.int32)
+    G = gg.Graph.from_edges(n, src, dst)
+    assert not G.is_symmetric
+    deg = np.diff(G.offsets)
+    assert (deg == 0).sum() > 1000
+    dg = gg.DeviceGraph(G, gpu)
+    dg.batch_config(sparse_edges=1)  # every level with more than one edge is dense: its update scans
+    for sources in (np.array([0, 1, 2, 50, 99], np.int32), np.arange(64, dtype=np.int32)):
+        log(f"directed sinks, {len(sources)} sources")
+        levels, _ = dg.bfs_batch(sources)
+        np.testing.assert_array_equal(levels, O.bfs_multi(G.offsets, G.cols, sources))
+
+
 @pytest.mark.parametrize("name", ["wiki-Vote", "test2", "tree", "test_isolated_node"])
 def test_fixture_batches(gpu, name, golden_levels):
     g = load_fixture(name)
