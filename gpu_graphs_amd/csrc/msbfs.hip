@@ -11,7 +11,7 @@
 //       seen |= new, the level of every source in new recorded, and (degree
 //       > 0) an entry (vertex, first CSR entry, edge prefix, mask = new) in
 //       the queue of active vertices; a block scan orders the entries of an
-//       8192-vertex chunk and one packed atomicAdd per chunk reserves their
+//       2048-vertex chunk and one packed atomicAdd per chunk reserves their
 //       slots and edge range, as in the single-source compaction.
 //   expand(L): edge-parallel tiles over the queue; for edge (u, v) the bits
 //       mask(u) & ~seen(v) are OR-ed into next(v) with a 64-bit atomicOr.
