@@ -189,8 +189,9 @@ def single_source(gg, args):
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "one BFS: the replayed launch sequence (clear, seed, then bfs_expand_kernel "
-                      "(top-down or bottom-up) and bfs_compact_kernel per level), HIP events on its stream",
+            "kernel": "one BFS: the replayed launch sequence (bfs_clear_kernel, which seeds the source, then "
+                      "one bfs_expand_kernel per level, top-down or bottom-up; bfs_compact_kernel only on "
+                      "directed graphs), HIP events on its stream",
             "kernel_ms_avg": round(dev_s * 1e3 / len(stats), 4),
             "alg_bytes_per_launch": int(sum(b_alg) / len(b_alg)),
         },

@@ -28,10 +28,13 @@
 // bottom-up when its frontier edges * alpha exceed the edges of the vertices
 // not yet reached by every source.
 // Level writes are staged as one byte per (vertex, source): each vertex owns
-// a 64-byte line, cleared to 0xFF, in which update_compact sets the bytes of
-// its new sources with a few 64-bit read-modify-writes.  After the last level
-// ms_finish transposes the lines into the k rows of n int32 (0xFF -> -1) with
-// full 256-byte stores.  Levels from 254 up are stored into the rows
+// a 64-byte line.  The first level that reaches the vertex writes the whole
+// line (its new sources' bytes, 0xFF elsewhere) and sets the vertex's bit in
+// a `reached` bitmap; later levels set the bytes of their new sources with a
+// few 64-bit read-modify-writes.  The lines are never cleared: after the last
+// level ms_finish transposes the lines of reached vertices into the k rows of
+// n int32 (0xFF -> -1) with full 256-byte stores, and writes -1 for the
+// others.  Levels from 254 up are stored into the rows
 // directly (the wave walks the union of its lanes' new bits) and marked 254
 // in the line, which the transpose leaves alone.
 // Sources, k and the output buffer are read from device memory (MsParams), so
@@ -88,6 +91,7 @@ struct MsArgs {
     SeenNext* sn;
     unsigned long long* fmask;   // per vertex: sources that reached it in the last scanned level
     unsigned long long* lv8;     // per vertex: 8 words = one level byte per source
+    unsigned long long* reached; // bit per vertex: its lv8 line has been written in this batch
     const MsParams* p;           // sources, k, output rows (device memory)
     const int32_t* perm;         // original id -> id of the degree-ordered copy
     const int32_t* iperm;        // id of the degree-ordered copy -> original id
@@ -121,7 +125,7 @@ struct MsTileLds {
     unsigned long long fm[T + 1];
 };
 
-// seen = next = 0; level lines = 0xFF (not reached).
+// seen = next = 0; no vertex reached (the level lines are left as they are).
 __global__ void ms_clear_kernel(MsArgs a) {
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
     const int64_t tid0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -129,9 +133,7 @@ __global__ void ms_clear_kernel(MsArgs a) {
         a.sn[i].seen = 0;
         a.sn[i].next = 0;
     }
-    ulonglong2* l2 = reinterpret_cast<ulonglong2*>(a.lv8);
-    const ulonglong2 ones = make_ulonglong2(~0ull, ~0ull);
-    for (int64_t i = tid0; i < a.n * 4; i += stride) l2[i] = ones;
+    for (int64_t i = tid0; i < (a.n + 63) / 64; i += stride) a.reached[i] = 0;
 }
 
 // Byte i of the result is 0xFF when bit i of b is set (b < 256).
@@ -141,11 +143,23 @@ __device__ __forceinline__ unsigned long long spread_bits_to_bytes(unsigned long
     return (nz >> 7) * 0xFFull;
 }
 
-// Sets byte s of v's level line to `value` for every bit s of nv.
+// Sets byte s of v's level line to `value` for every bit s of nv; `first`:
+// the line has not been written in this batch, and its other bytes become
+// 0xFF (not reached), with four 16-byte stores and no read.
 __device__ __forceinline__ void set_level_bytes(unsigned long long* lv8, int64_t v, unsigned long long nv,
-                                                unsigned long long value) {
+                                                unsigned long long value, bool first) {
     unsigned long long* line = lv8 + v * 8;
     const unsigned long long vb = value * 0x0101010101010101ull;
+    if (first) {
+        ulonglong2* l2 = reinterpret_cast<ulonglong2*>(line);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const unsigned long long m0 = spread_bits_to_bytes((nv >> (16 * j)) & 0xFFull);
+            const unsigned long long m1 = spread_bits_to_bytes((nv >> (16 * j + 8)) & 0xFFull);
+            l2[j] = make_ulonglong2((vb & m0) | ~m0, (vb & m1) | ~m1);
+        }
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         const unsigned long long b = (nv >> (8 * j)) & 0xFFull;
@@ -171,10 +185,11 @@ __global__ __launch_bounds__(256) void ms_finish_kernel(MsArgs a) {
         const bool valid = v < a.n;
         unsigned long long w[8];
         const int64_t vn = valid ? a.perm[v] : 0;
+        const bool reached = valid && ((a.reached[vn >> 6] >> (vn & 63)) & 1ull);
         const ulonglong2* line = reinterpret_cast<const ulonglong2*>(a.lv8 + vn * 8);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const ulonglong2 x = valid ? line[j] : make_ulonglong2(~0ull, ~0ull);
+            const ulonglong2 x = reached ? line[j] : make_ulonglong2(~0ull, ~0ull);
             w[2 * j] = x.x;
             w[2 * j + 1] = x.y;
         }
@@ -257,12 +272,22 @@ __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, int64_t span, bo
         newbits[r] = nv;
         act |= nv;
         verts[r] = static_cast<int32_t>(v);
+        const bool first = nv && !sv;  // the first level that reaches v
+        if (list) {
+            if (first) atomicOr(&a.reached[v >> 6], 1ull << (v & 63));
+        } else {
+            // a wave holds the 64 vertices of one bitmap word, which no other
+            // wave of this kernel touches
+            const uint64_t fb = wave_ballot(first);
+            const int64_t w = (ibase + r * kUcBlock + (tid & ~63u)) >> 6;
+            if (fb && (tid & 63u) == 0) a.reached[w] |= fb;
+        }
         if (L < kByteLevels) {
-            if (nv) set_level_bytes(a.lv8, v, nv, static_cast<unsigned long long>(L));
+            if (nv) set_level_bytes(a.lv8, v, nv, static_cast<unsigned long long>(L), first);
         } else {
             // every lane of the wave takes part in the shuffles of write_levels
             if (__ballot(nv != 0)) write_levels(levels, a.n, nv ? a.iperm[v] : 0, nv, L);
-            if (nv) set_level_bytes(a.lv8, v, nv, static_cast<unsigned long long>(kByteLevels));
+            if (nv) set_level_bytes(a.lv8, v, nv, static_cast<unsigned long long>(kByteLevels), first);
         }
         if (nv) {
             const uint32_t d = static_cast<uint32_t>(a.offsets[v + 1] - a.offsets[v]);
@@ -860,8 +885,8 @@ void build_ms_graph(DeviceGraph& dg, const HostGraph& g) {
 
 void Runner::free_ms_workspace() {
     if (ms.graph_exec) (void)hipGraphExecDestroy(ms.graph_exec);
-    void* ptrs[] = {ms.sn,         ms.fmask,  ms.lv8,     ms.q_rb,    ms.q_eoff, ms.q_v,    ms.q_fm,
-                    ms.tile_start, ms.touched, ms.ctl,    ms.params,  ms.levels, ms.partial};
+    void* ptrs[] = {ms.sn,      ms.fmask,      ms.lv8,     ms.reached, ms.q_rb,   ms.q_eoff, ms.q_v,
+                    ms.q_fm,    ms.tile_start, ms.touched, ms.ctl,     ms.params, ms.levels, ms.partial};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (ms.h_params) (void)hipHostFree(ms.h_params);
@@ -895,6 +920,7 @@ void Runner::ensure_ms_workspace() {
         alloc(&ms.sn, sizeof(SeenNext) * n);
         alloc(&ms.fmask, sizeof(unsigned long long) * n);
         alloc(&ms.lv8, sizeof(unsigned long long) * 8 * n);
+        alloc(&ms.reached, sizeof(unsigned long long) * ((n + 63) / 64));
         alloc(&ms.q_rb, sizeof(int64_t) * n);
         alloc(&ms.q_eoff, sizeof(uint32_t) * n);
         alloc(&ms.q_v, sizeof(int32_t) * n);
@@ -924,6 +950,7 @@ void Runner::prepare_batch() {
 
 size_t batch_bytes_estimate(int64_t n_vertices, int64_t m_entries, int k, bool host_levels) {
     size_t b = sizeof(unsigned long long) * n_vertices * 12 + (sizeof(int64_t) + 12) * n_vertices +
+               sizeof(unsigned long long) * ((n_vertices + 63) / 64) +
                sizeof(uint32_t) * (m_entries / 256 + 2) + 1024 * 64 * 16 + 4096;
     if (host_levels) b += sizeof(int32_t) * n_vertices * k;
     return b;
@@ -955,6 +982,7 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     a.sn = reinterpret_cast<SeenNext*>(ms.sn);
     a.fmask = ms.fmask;
     a.lv8 = ms.lv8;
+    a.reached = ms.reached;
     a.p = ms.params;
     a.q_rb = ms.q_rb;
     a.q_eoff = ms.q_eoff;

@@ -251,6 +251,36 @@ def test_bottom_up_batch_on_mixed_graph_and_directed(gpu):
     assert dd.batch_last_bottom_up() == 0
 
 
+def test_level_lines_of_earlier_batches_are_not_read(gpu):
+    # The per-vertex level lines are not cleared between batches: the first
+    # level that reaches a vertex rewrites its whole line, and vertices no
+    # source reaches must come out -1 whatever an earlier batch left there.
+    base = gg.Graph.rmat(12, seed=86)
+    n0 = base.n
+    src = [np.repeat(np.arange(n0), np.diff(base.offsets)).astype(np.int64)]
+    dst = [base.cols.astype(np.int64)]
+    for c in range(10):  # ten 3-vertex paths, each its own component
+        a, b, c3 = n0 + 3 * c, n0 + 3 * c + 1, n0 + 3 * c + 2
+        src += [np.array([a, b, b, c3])]
+        dst += [np.array([b, a, c3, b])]
+    G = gg.Graph.from_edges(n0 + 30, np.concatenate(src).astype(np.int32), np.concatenate(dst).astype(np.int32))
+    dg = gg.DeviceGraph(G, gpu)
+    deg = np.diff(G.offsets)
+    rng = np.random.default_rng(86)
+    giant = rng.choice(np.nonzero(deg[:n0])[0], 64, replace=False).astype(np.int32)
+    paths = (n0 + 3 * np.arange(10)).astype(np.int32)
+    batches = [
+        np.concatenate([giant[:54], paths]),        # every component reached
+        np.concatenate([paths[::-1], giant[10:]]),  # the paths' lines now under other sources
+        giant[:20],                                 # the paths unreached: their old lines stay behind
+        paths[[4, 1, 7]],                           # only three paths: the giant component unreached
+        giant[30:64],
+    ]
+    for i, sources in enumerate(batches):
+        log(f"stale lines, batch {i} ({len(sources)} sources)")
+        check_batch(G, dg, sources)
+
+
 def test_levels_past_the_byte_range(gpu):
     # A 700-vertex path: levels go past 254, where the batch stores levels
     # into the rows directly instead of the byte lines.
