@@ -71,11 +71,13 @@ enum : int {
 };
 enum : unsigned long long { MS_ERR_TILE = 1, MS_ERR_RANGE = 2 };
 
-// update_compact: 2048-vertex chunks of 1024 threads x 2 (R-MAT s24, 64-source
-// batch: 8.14 ms against 8.22 ms for 512 x 4, 8.59 for 1024 x 1, 8.69 for
-// 512 x 2, 8.55 for 512 x 8, 8.86 for 256 x 8, 10.3 for 512 x 1).
-constexpr int kUcBlock = 1024;
-constexpr int kUcRounds = 2;
+// update_compact: 2048-vertex chunks of 512 threads x 4.  Swept on 64-source
+// batches: at R-MAT s24, 1024 x 2 took 8.14 ms against 8.22 ms (1024 x 1
+// 8.59, 512 x 2 8.69, 512 x 8 8.55, 256 x 8 8.86, 512 x 1 10.3), but on a
+// 2000 x 2000 grid (thousands of thin levels) 1024 x 2 took 265 ms against
+// 244 ms, and at R-MAT s20 0.88 ms against 0.85 ms.
+constexpr int kUcBlock = 512;
+constexpr int kUcRounds = 4;
 constexpr int kUcChunk = kUcBlock * kUcRounds;
 constexpr int kMaxCapturedLevels = 32;  // deeper searches continue with direct launches
 constexpr int kByteLevels = 254;         // levels kept in the byte lines; 254 = stored directly, 255 = none
