@@ -279,6 +279,21 @@ def test_level_lines_of_earlier_batches_are_not_read(gpu):
     for i, sources in enumerate(batches):
         log(f"stale lines, batch {i} ({len(sources)} sources)")
         check_batch(G, dg, sources)
+    # Directed: every vertex is scanned on dense levels (sinks included), and
+    # which vertices a batch reaches depends on where its sources sit.
+    rng = np.random.default_rng(87)
+    n = 6000
+    src = np.concatenate([rng.integers(0, 3000, 30000), rng.integers(3000, n, 4000)])
+    dst = np.concatenate([rng.integers(0, n, 30000), rng.integers(3000, n, 4000)])
+    D = gg.Graph.from_edges(n, src.astype(np.int32), dst.astype(np.int32))
+    assert not D.is_symmetric
+    dd = gg.DeviceGraph(D, gpu)
+    dd.batch_config(sparse_edges=1)  # dense levels: the updates scan
+    for i, sources in enumerate((np.arange(0, 6000, 94, dtype=np.int32)[:64],  # both halves
+                                 np.arange(3000, 3064, dtype=np.int32),       # the half that cannot reach [0, 3000)
+                                 np.arange(0, 10, dtype=np.int32))):
+        log(f"stale lines, directed batch {i}")
+        check_batch(D, dd, sources)
 
 
 def test_levels_past_the_byte_range(gpu):
