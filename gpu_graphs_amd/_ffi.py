@@ -182,6 +182,8 @@ SIGNATURES = {
     "gb_server_reset_stats": (C.c_int, [P]),
     "gb_server_free": (None, [P]),
     "gb_exclusive_scan_i32": (C.c_int, [C.c_int, P, P, C.c_int64]),
+    "gb_exclusive_scan_i32_noalloc": (C.c_int, [C.c_int, P, P, C.c_int64]),
+    "gb_exclusive_scan_noalloc_max": (C.c_int64, []),
     "gb_dmalloc": (C.c_int, [C.c_int, C.c_size_t, PP]),
     "gb_dfree": (C.c_int, [C.c_int, P]),
     "gb_memcpy_htod": (C.c_int, [C.c_int, P, P, C.c_size_t]),

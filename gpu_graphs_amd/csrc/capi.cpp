@@ -467,6 +467,18 @@ int gb_exclusive_scan_i32(int device, const int32_t* d_in, int32_t* d_out, int64
     });
 }
 
+int gb_exclusive_scan_i32_noalloc(int device, const int32_t* d_in, int32_t* d_out, int64_t n) {
+    return guarded([&] {
+        if (n > 0) {
+            gg::check_device_ptr(device, d_in, "scan input");
+            gg::check_device_ptr(device, d_out, "scan output");
+        }
+        gg::exclusive_scan_i32_noalloc(device, d_in, d_out, n);
+    });
+}
+
+int64_t gb_exclusive_scan_noalloc_max(void) { return gg::exclusive_scan_noalloc_max(); }
+
 int gb_dmalloc(int device, size_t bytes, void** ptr) {
     return guarded([&] {
         need(ptr, "null argument");

@@ -272,5 +272,10 @@ void autotune_batch(DeviceGraph& g, const int32_t* sources, int k, int reps, gb_
 
 // Device-wide exclusive scan of int32 (scan.hip).
 void exclusive_scan_i32(int device, const int32_t* d_in, int32_t* d_out, int64_t n);
+// The same scan with its partials in a fixed device array: no allocation,
+// at most exclusive_scan_noalloc_max() items; overlapping calls on one GPU
+// share that array and race.
+void exclusive_scan_i32_noalloc(int device, const int32_t* d_in, int32_t* d_out, int64_t n);
+int64_t exclusive_scan_noalloc_max();
 
 }  // namespace gg

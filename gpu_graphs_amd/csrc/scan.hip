@@ -11,10 +11,12 @@
 //   3. tile_scan:    thread-serial scan of 16 items, wave64 scan of thread
 //                    totals, LDS-staged scan of the 4 wave totals, + partial
 // The partial array is cached per device and only grows, so steady-state
-// calls allocate nothing.
+// calls allocate nothing.  exclusive_scan_i32_noalloc never allocates: its
+// partials live in a fixed device array (`scratch`), which bounds its length.
 #include <hip/hip_runtime.h>
 
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "device_graph.hpp"
@@ -109,7 +111,66 @@ ScanCache& cache() {
     return c;
 }
 
+// The three launches of one scan on the legacy default stream (ordered after
+// the caller's copies), then a wait for them.
+void run_scan(const int32_t* d_in, int32_t* d_out, int64_t n, int64_t tiles, uint32_t* partial) {
+    const uint32_t* in = reinterpret_cast<const uint32_t*>(d_in);
+    uint32_t* out = reinterpret_cast<uint32_t*>(d_out);
+    hipStream_t s = nullptr;
+    hipLaunchKernelGGL(tile_reduce, dim3(static_cast<uint32_t>(tiles)), dim3(kBlock), 0, s, in, partial, n);
+    hipLaunchKernelGGL(partial_scan, dim3(1), dim3(1024), 0, s, partial, tiles);
+    hipLaunchKernelGGL(tile_scan, dim3(static_cast<uint32_t>(tiles)), dim3(kBlock), 0, s, in, out, partial, n);
+    GG_HIP(hipGetLastError());
+    GG_HIP(hipStreamSynchronize(s));
+}
+
 }  // namespace
+
+// Partials of the allocation-free scan: one per 4096-item tile, so inputs of
+// up to 2^18 tiles = 2^30 items (1 MiB of device memory, held by every GPU
+// the library runs on until the process ends).  Host code never passes
+// this name to a kernel: it stands for a device address only on the device;
+// the host uses the address hipGetSymbolAddress returns for each device.
+constexpr int64_t kNoAllocMaxTiles = int64_t(1) << 18;
+static __device__ __align__(256) uint32_t scratch[kNoAllocMaxTiles];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+
+// Device address of `scratch` on `device` (current), looked up once per
+// device with hipGetSymbolAddress; fixed-size table, no allocation.
+uint32_t* scratch_on(int device) {
+    static std::mutex mu;
+    static uint32_t* ptr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) fail(GB_EINVAL, "device id outside [0, 64)");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            fail(GB_EDEVICE, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+        ptr[device] = static_cast<uint32_t*>(p);
+    }
+    return ptr[device];
+}
+
+}  // namespace
+
+int64_t exclusive_scan_noalloc_max() { return kNoAllocMaxTiles * kTile; }
+
+void exclusive_scan_i32_noalloc(int device, const int32_t* d_in, int32_t* d_out, int64_t n) {
+    if (n < 0) fail(GB_EINVAL, "negative scan length");
+    if (n > kNoAllocMaxTiles * kTile)
+        fail(GB_EINVAL, "scan of " + std::to_string(n) + " items exceeds the allocation-free maximum of " +
+                            std::to_string(kNoAllocMaxTiles * kTile) + " (use gb_exclusive_scan_i32)");
+    if (n == 0) return;
+    if (!d_in || !d_out) fail(GB_EINVAL, "null scan buffer");
+    DeviceGuard guard(device);
+    // Every partial the scan reads, tile_reduce writes first in the same
+    // call: nothing from an earlier call needs clearing.
+    run_scan(d_in, d_out, n, (n + kTile - 1) / kTile, scratch_on(device));
+}
 
 void exclusive_scan_i32(int device, const int32_t* d_in, int32_t* d_out, int64_t n) {
     if (n < 0) fail(GB_EINVAL, "negative scan length");
@@ -135,14 +196,7 @@ void exclusive_scan_i32(int device, const int32_t* d_in, int32_t* d_out, int64_t
         }
         partial = c.partial[device];
     }
-    const uint32_t* in = reinterpret_cast<const uint32_t*>(d_in);
-    uint32_t* out = reinterpret_cast<uint32_t*>(d_out);
-    hipStream_t s = nullptr;  // legacy default stream: ordered after the caller's copies
-    hipLaunchKernelGGL(tile_reduce, dim3(static_cast<uint32_t>(tiles)), dim3(kBlock), 0, s, in, partial, n);
-    hipLaunchKernelGGL(partial_scan, dim3(1), dim3(1024), 0, s, partial, tiles);
-    hipLaunchKernelGGL(tile_scan, dim3(static_cast<uint32_t>(tiles)), dim3(kBlock), 0, s, in, out, partial, n);
-    GG_HIP(hipGetLastError());
-    GG_HIP(hipStreamSynchronize(s));
+    run_scan(d_in, d_out, n, tiles, partial);
 }
 
 }  // namespace gg

@@ -313,6 +313,18 @@ void gb_server_free(gb_server* srv);
  *      In-place (d_out == d_in) is allowed.  No allocation per call. ----- */
 int  gb_exclusive_scan_i32(int device, const int32_t* d_in, int32_t* d_out, int64_t n);
 
+/* ---- allocation-free exclusive scan.  No counterpart in the reference: it
+ *      is for callers that cannot allocate device memory (code under HIP
+ *      graph capture, tight serving loops, hosts that forbid hipMalloc).
+ *      Same result as gb_exclusive_scan_i32, but it calls no hipMalloc,
+ *      hipFree or allocating library routine: its per-tile partials live in
+ *      a fixed 1 MiB device array, one per GPU, held until the process ends.
+ *      n may be at most gb_exclusive_scan_noalloc_max() (2^30); longer
+ *      inputs fail with GB_EINVAL.  Calls that overlap on one GPU share the
+ *      array and race: serialise them per device. ----------------------- */
+int  gb_exclusive_scan_i32_noalloc(int device, const int32_t* d_in, int32_t* d_out, int64_t n);
+int64_t gb_exclusive_scan_noalloc_max(void);
+
 /* ---- device memory helpers (so hosts need no other GPU runtime) -------- */
 int  gb_dmalloc(int device, size_t bytes, void** ptr);
 int  gb_dfree(int device, void* ptr);
