@@ -501,7 +501,8 @@ class gpuBFS:
 
     The constructor runs the whole BFS on one GPU (the graph is uploaded on
     first use and kept resident).  ``host_distance`` holds int32 levels, -1
-    for unreached; ``exec_time`` is device time in ms; ``iteration`` is the
+    for unreached; ``exec_time`` is device time in ms; ``copy_time`` the ms of
+    the device-to-host copy of the levels; ``iteration`` is the
     number of BFS levels; ``total_edges_traversed`` the CSR entries scanned.
     """
 
@@ -511,6 +512,7 @@ class gpuBFS:
         self.host_distance = levels
         self.parents = par
         self.exec_time = st["ms"]
+        self.copy_time = st["copy_ms"]
         self.iteration = st["num_levels"]
         self.total_edges_traversed = st["m_reached"]
         self.vertices_reached = st["n_reached"]

@@ -54,6 +54,7 @@ class Stats(C.Structure):
         ("reserved", C.c_int32),
         ("n_reached", C.c_int64),
         ("m_reached", C.c_int64),
+        ("copy_ms", C.c_double),
     ]
 
     def as_dict(self):

@@ -1123,12 +1123,14 @@ DeviceGraph::DeviceGraph(const HostGraph& g, int dev) : device(dev), n(g.n), m(g
         build_ms_graph(*this, g);
         runner = std::make_unique<Runner>(*this);
     } catch (...) {
-        this->~DeviceGraph();
+        release();  // members are then destroyed by the unwinding, once
         throw;
     }
 }
 
-DeviceGraph::~DeviceGraph() {
+DeviceGraph::~DeviceGraph() { release(); }
+
+void DeviceGraph::release() {
     (void)hipSetDevice(device);
     runner.reset();
     if (offsets) (void)hipFree(offsets);
@@ -1160,12 +1162,14 @@ Runner::Runner(DeviceGraph& graph) : g(graph), device(graph.device), n(graph.n),
         GG_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_ctl), sizeof(unsigned long long) * 2 * kCtlWords,
                              hipHostMallocDefault));
     } catch (...) {
-        this->~Runner();
+        release();
         throw;
     }
 }
 
-Runner::~Runner() {
+Runner::~Runner() { release(); }
+
+void Runner::release() {
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
     free_workspace();
@@ -1480,6 +1484,7 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
         st->reserved = 0;
         st->n_reached = static_cast<int64_t>(fin[CTL_NREACH]);
         st->m_reached = static_cast<int64_t>(fin[CTL_MREACH]);
+        st->copy_ms = 0;
     }
 }
 
@@ -1533,9 +1538,17 @@ void Runner::bfs_to_host(int32_t source, int32_t* h_levels, int32_t* h_parents, 
     DeviceGuard guard(device);
     ensure_workspace();
     bfs(source, ws.levels, h_parents ? ws.parents : nullptr, st);
+    // The search has ended (bfs() waited for it): the events time the copy alone.
+    GG_HIP(hipEventRecord(ev_start, stream));
     GG_HIP(hipMemcpyAsync(h_levels, ws.levels, sizeof(int32_t) * n, hipMemcpyDeviceToHost, stream));
     if (h_parents) GG_HIP(hipMemcpyAsync(h_parents, ws.parents, sizeof(int32_t) * n, hipMemcpyDeviceToHost, stream));
+    GG_HIP(hipEventRecord(ev_stop, stream));
     GG_HIP(hipStreamSynchronize(stream));
+    if (st) {
+        float cms = 0.f;
+        GG_HIP(hipEventElapsedTime(&cms, ev_start, ev_stop));
+        st->copy_ms = cms;
+    }
 }
 
 }  // namespace gg

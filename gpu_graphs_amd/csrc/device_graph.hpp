@@ -197,6 +197,8 @@ struct Runner {
     }
 
    private:
+    // Frees everything the runner holds (destructor, failed constructor).
+    void release();
     void learn_from_search(bool had_dense_level);
     void ensure_workspace();
     void free_workspace();
@@ -249,6 +251,8 @@ struct DeviceGraph {
     }
 
     size_t memory_bytes() const { return graph_bytes + (runner ? runner->bytes() : 0); }
+    // Frees the device copies and the runner (destructor, failed constructor).
+    void release();
     int64_t default_dense_edges() const { return std::max<int64_t>(int64_t(1) << 19, m / 256); }
     int64_t default_ms_sparse_edges() const { return std::max<int64_t>(int64_t(1) << 16, n / 16); }
 };

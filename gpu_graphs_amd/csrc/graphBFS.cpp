@@ -42,9 +42,10 @@ int main(int argc, char* argv[]) {
         if (print) bfs.print_distance(graph);
         if (json) {
             std::printf("{\"file\": \"%s\", \"source\": %d, \"n\": %lld, \"m\": %lld, \"levels\": %d, "
-                        "\"edges_traversed\": %lld, \"ms\": %.4f}\n",
+                        "\"edges_traversed\": %lld, \"ms\": %.4f, \"copy_ms\": %.4f}\n",
                         file, source, static_cast<long long>(graph.num_nodes), static_cast<long long>(graph.num_edges),
-                        bfs.iteration, static_cast<long long>(bfs.total_edges_traversed), bfs.exec_time);
+                        bfs.iteration, static_cast<long long>(bfs.total_edges_traversed), bfs.exec_time,
+                        bfs.copy_time);
         } else {
             std::printf("0 GPU BFS Time: %g ms\n", bfs.exec_time);
         }

@@ -79,6 +79,9 @@ typedef struct gb_stats {
     int32_t reserved;
     int64_t n_reached;
     int64_t m_reached;
+    double  copy_ms;      /* gb_bfs: device-to-host copy of levels (+ parents), the
+                           * reference's get_device_distance (gpuBFS.cu:118-137),
+                           * hipEvent time outside `ms`; 0 for gb_bfs_device       */
 } gb_stats;
 
 /* ---- errors ------------------------------------------------------------ */

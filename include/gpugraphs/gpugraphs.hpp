@@ -82,6 +82,7 @@ class gpuBFS {
         check(gb_bfs(graph.device(device), source, distance_.data(), nullptr, &st));
         host_distance = distance_.data();
         exec_time = static_cast<float>(st.ms);
+        copy_time = static_cast<float>(st.copy_ms);
         iteration = st.num_levels;
         total_edges_traversed = st.m_reached;
     }
@@ -93,6 +94,7 @@ class gpuBFS {
 
     int32_t* host_distance = nullptr;
     float exec_time = 0;
+    float copy_time = 0;  // ms of the device-to-host copy of host_distance (get_device_distance)
     int64_t total_edges_traversed = 0;
     int iteration = 0;
 

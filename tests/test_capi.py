@@ -62,3 +62,10 @@ def test_cli_usage():
     assert r.returncode == 1 and "Usage" in r.stderr
     r = subprocess.run([exe, "/nonexistent.txt", "0"], capture_output=True, text=True)
     assert r.returncode == 2 and "unable to open" in r.stderr
+
+
+def test_facade_program_usage():
+    exe = os.path.join(ROOT, "gpu_graphs_amd", "facade_main")
+    assert os.path.exists(exe), "facade_main not built"
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 1 and "Usage" in r.stderr
