@@ -102,8 +102,11 @@ struct MsConfig {
 struct MsParams {
     int32_t sources[64];
     int32_t k;
+    int32_t clear_all;              // clear the state of every vertex (else of those with edges)
+    int32_t* levels;                // k rows of n int32
+    int32_t prev_sources[64];       // the previous batch's sources (their state is cleared)
+    int32_t prev_k;
     int32_t pad;
-    int32_t* levels;  // k rows of n int32
 };
 
 // Device workspace of the batched multi-source BFS.
@@ -132,6 +135,9 @@ struct MsWorkspace {
     int depth_hint = 0;                   // levels queued before the first done check (see bfs_batch)
     int short_streak = 0;                 // batches in a row shallower than depth_hint
     int last_bu_levels = 0;               // levels the last batch ran bottom-up
+    bool tail_clean = false;              // vertices without edges hold no state but prev_sources'
+    int32_t prev_sources[64] = {};        // sources of the last batch that completed
+    int prev_k = 0;
     size_t bytes = 0;
     hipGraphExec_t graph_exec = nullptr;  // captured batch sequence
     int graph_depth = 0, graph_block = 0, graph_kpt = 0, graph_bu = -1, graph_alpha = 0, graph_low = 0;

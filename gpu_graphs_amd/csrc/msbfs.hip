@@ -30,8 +30,8 @@
 // Level writes are staged as one byte per (vertex, source): each vertex owns
 // a 64-byte line.  The first level that reaches the vertex writes the whole
 // line (its new sources' bytes, 0xFF elsewhere) and sets the vertex's bit in
-// a `reached` bitmap; later levels set the bytes of their new sources with a
-// few 64-bit read-modify-writes.  The lines are never cleared: after the last
+// a `reached` bitmap; later levels read the line, merge the bytes of their
+// new sources and write it back whole.  The lines are never cleared: after the last
 // level ms_finish transposes the lines of reached vertices into the k rows of
 // n int32 (0xFF -> -1) with full 256-byte stores, and writes -1 for the
 // others.  Levels from 254 up are stored into the rows
@@ -131,10 +131,14 @@ struct MsTileLds {
 };
 
 // seen = next = 0; no vertex reached (the level lines are left as they are).
+// Vertices without edges (ids from nz on, symmetric graphs) are only ever
+// touched as sources, so after the first batch only [0, nz) is cleared here
+// and ms_seed clears the previous batch's sources.
 __global__ void ms_clear_kernel(MsArgs a) {
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
     const int64_t tid0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    for (int64_t i = tid0; i < a.n; i += stride) {
+    const int64_t end = a.p->clear_all ? a.n : a.nz;
+    for (int64_t i = tid0; i < end; i += stride) {
         a.sn[i].seen = 0;
         a.sn[i].next = 0;
     }
@@ -148,37 +152,12 @@ __device__ __forceinline__ unsigned long long spread_bits_to_bytes(unsigned long
     return (nz >> 7) * 0xFFull;
 }
 
-// Sets byte s of v's level line to `value` for every bit s of nv; `first`:
-// the line has not been written in this batch, and its other bytes become
-// 0xFF (not reached), with four 16-byte stores and no read.
-__device__ __forceinline__ void set_level_bytes(unsigned long long* lv8, int64_t v, unsigned long long nv,
-                                                unsigned long long value, bool first) {
-    unsigned long long* line = lv8 + v * 8;
-    const unsigned long long vb = value * 0x0101010101010101ull;
-    if (first) {
-        ulonglong2* l2 = reinterpret_cast<ulonglong2*>(line);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const unsigned long long m0 = spread_bits_to_bytes((nv >> (16 * j)) & 0xFFull);
-            const unsigned long long m1 = spread_bits_to_bytes((nv >> (16 * j + 8)) & 0xFFull);
-            l2[j] = make_ulonglong2((vb & m0) | ~m0, (vb & m1) | ~m1);
-        }
-        return;
-    }
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const unsigned long long b = (nv >> (8 * j)) & 0xFFull;
-        if (b) {
-            const unsigned long long bm = spread_bits_to_bytes(b);
-            line[j] = (line[j] & ~bm) | (vb & bm);
-        }
-    }
-}
-
 // After the last level: level rows [k][n] from the byte lines, a wave per 64
 // vertices, one full 256-byte store per source (byte 255 -> -1; byte 254 was
 // stored directly and is left alone).
-__global__ __launch_bounds__(256) void ms_finish_kernel(MsArgs a) {
+__global__ __launch_bounds__(256) void ms_finish_kernel(MsArgs a, int only_if_done) {
+    // Replayed right after the first levels: a no-op unless the search has ended.
+    if (only_if_done && (!a.ctl[MS_DONE] || a.ctl[MS_ERR])) return;
     const int k = a.p->k;
     int32_t* const levels = a.p->levels;
     const uint32_t lane = threadIdx.x & 63u;
@@ -215,6 +194,13 @@ __global__ __launch_bounds__(256) void ms_finish_kernel(MsArgs a) {
 __global__ void ms_seed_kernel(MsArgs a) {
     if (threadIdx.x == 0)
         for (int i = 0; i < MS_WORDS; ++i) a.ctl[i] = 0;
+    if (!a.p->clear_all && threadIdx.x < static_cast<uint32_t>(a.p->prev_k)) {
+        const int32_t s = a.perm[a.p->prev_sources[threadIdx.x]];
+        if (s >= a.nz) {
+            a.sn[s].seen = 0;
+            a.sn[s].next = 0;
+        }
+    }
     __syncthreads();
     if (threadIdx.x < static_cast<uint32_t>(a.p->k)) {
         const int32_t s = a.perm[a.p->sources[threadIdx.x]];
@@ -237,10 +223,27 @@ __device__ __forceinline__ void write_levels(int32_t* levels, int64_t n, int64_t
     }
 }
 
+// Word of a level line with byte s set to the byte of `vb` for every bit s
+// of the 8-bit group nv8, other bytes kept from `old`.
+__device__ __forceinline__ unsigned long long merge_level_word(unsigned long long old, unsigned long long nv8,
+                                                               unsigned long long vb) {
+    const unsigned long long bm = spread_bits_to_bytes(nv8);
+    return (old & ~bm) | (vb & bm);
+}
+
 // Chunk c of update_compact(L): thread t takes slots c*span + r*kUcBlock + t
 // (r < kUcRounds, below (c+1)*span) of either the vertex range (scan) or the
 // touched list.  span = kUcChunk, or kUcBlock for a short list, which then
 // spreads over all workgroups instead of a few.
+//
+// The loads of all of a thread's slots are issued before any result is used
+// and before any store (slot vertices; then state and row bounds; then the
+// level lines that already hold bytes), so a chunk costs three dependent
+// memory round trips, not three per slot.  A vertex's line is rewritten
+// whole (four 16-byte stores): its first level writes 0xFF for the sources
+// that have not reached it, later levels merge their bytes into the line
+// they read.  Each vertex is in at most one slot of a launch (the touched
+// list holds distinct vertices), so no other thread writes its line or state.
 __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, int64_t span, bool list, uint32_t count,
                             unsigned long long kmask, unsigned long long& fulldeg, unsigned long long& act) {
     __shared__ unsigned long long s_wave[kUcBlock / 64];
@@ -250,10 +253,8 @@ __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, int64_t span, bo
     const int64_t ibase = int64_t(c) * span;
     const int64_t iend = ibase + span;
     int32_t* levels = a.p->levels;
-    uint32_t active = 0;  // bit r: the vertex of slot r goes to the queue
-    uint32_t kept = 0, edges = 0;
-    unsigned long long newbits[kUcRounds];
-    int32_t verts[kUcRounds];
+    // Step 1: the vertex of each slot.
+    int64_t vs[kUcRounds];
 #pragma unroll
     for (int r = 0; r < kUcRounds; ++r) {
         const int64_t i = ibase + r * kUcBlock + tid;
@@ -263,45 +264,75 @@ __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, int64_t span, bo
         } else if (i < a.nz) {  // a scan comes after level 0: vertices without edges get nothing
             v = i;
         }
-        unsigned long long nv = 0, sv = 0;
+        vs[r] = v;
+    }
+    // Step 2: seen / next and the row bounds.
+    ulonglong2 st[kUcRounds];
+    int64_t rb[kUcRounds], re[kUcRounds];
+#pragma unroll
+    for (int r = 0; r < kUcRounds; ++r) {
+        st[r] = make_ulonglong2(0ull, 0ull);
+        rb[r] = re[r] = 0;
+        if (vs[r] >= 0) {
+            st[r] = *reinterpret_cast<const ulonglong2*>(a.sn + vs[r]);
+            rb[r] = a.offsets[vs[r]];
+            re[r] = a.offsets[vs[r] + 1];
+        }
+    }
+    // Step 3: new bits; the lines of vertices reached at an earlier level.
+    unsigned long long newbits[kUcRounds];
+    bool first[kUcRounds];
+    ulonglong2 line[kUcRounds][4];
+#pragma unroll
+    for (int r = 0; r < kUcRounds; ++r) {
+        const unsigned long long sv = st[r].x;
+        newbits[r] = st[r].y & ~sv;
+        first[r] = newbits[r] && !sv;  // the first level that reaches the vertex
+        const ulonglong2* src = reinterpret_cast<const ulonglong2*>(a.lv8 + (vs[r] >= 0 ? vs[r] : 0) * 8);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            line[r][j] = (newbits[r] && !first[r]) ? src[j] : make_ulonglong2(~0ull, ~0ull);
+    }
+    // Step 4: stores.
+    uint32_t active = 0;  // bit r: the vertex of slot r goes to the queue
+    uint32_t kept = 0, edges = 0;
+    const unsigned long long lv = static_cast<unsigned long long>(L < kByteLevels ? L : kByteLevels);
+    const unsigned long long vb = lv * 0x0101010101010101ull;
+#pragma unroll
+    for (int r = 0; r < kUcRounds; ++r) {
+        const int64_t v = vs[r];
+        const unsigned long long nv = newbits[r];
         if (v >= 0) {
-            const unsigned long long nx = a.sn[v].next;
-            if (nx) {
-                sv = a.sn[v].seen;
-                nv = nx & ~sv;
-                a.sn[v].seen = sv | nv;
-                a.sn[v].next = 0;
-            }
+            if (st[r].y) *reinterpret_cast<ulonglong2*>(a.sn + v) = make_ulonglong2(st[r].x | nv, 0ull);
             a.fmask[v] = nv;  // a scan covers every vertex: the next level may go bottom-up
         }
-        newbits[r] = nv;
         act |= nv;
-        verts[r] = static_cast<int32_t>(v);
-        const bool first = nv && !sv;  // the first level that reaches v
         if (list) {
-            if (first) atomicOr(&a.reached[v >> 6], 1ull << (v & 63));
+            if (first[r]) atomicOr(&a.reached[v >> 6], 1ull << (v & 63));
         } else {
             // a wave holds the 64 vertices of one bitmap word, which no other
             // wave of this kernel touches
-            const uint64_t fb = wave_ballot(first);
+            const uint64_t fb = wave_ballot(first[r]);
             const int64_t w = (ibase + r * kUcBlock + (tid & ~63u)) >> 6;
             if (fb && (tid & 63u) == 0) a.reached[w] |= fb;
         }
-        if (L < kByteLevels) {
-            if (nv) set_level_bytes(a.lv8, v, nv, static_cast<unsigned long long>(L), first);
-        } else {
+        if (L >= kByteLevels) {
             // every lane of the wave takes part in the shuffles of write_levels
             if (__ballot(nv != 0)) write_levels(levels, a.n, nv ? a.iperm[v] : 0, nv, L);
-            if (nv) set_level_bytes(a.lv8, v, nv, static_cast<unsigned long long>(kByteLevels), first);
         }
         if (nv) {
-            const uint32_t d = static_cast<uint32_t>(a.offsets[v + 1] - a.offsets[v]);
+            ulonglong2* dst = reinterpret_cast<ulonglong2*>(a.lv8 + v * 8);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                dst[j] = make_ulonglong2(merge_level_word(line[r][j].x, (nv >> (16 * j)) & 0xFFull, vb),
+                                         merge_level_word(line[r][j].y, (nv >> (16 * j + 8)) & 0xFFull, vb));
+            const uint32_t d = static_cast<uint32_t>(re[r] - rb[r]);
             if (d) {
                 active |= 1u << r;
                 ++kept;
                 edges += d;
             }
-            if (((sv | nv) & kmask) == kmask) fulldeg += d;  // every source has reached v now
+            if (((st[r].x | nv) & kmask) == kmask) fulldeg += d;  // every source has reached v now
         }
     }
     bool anynew = false;
@@ -332,12 +363,10 @@ __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, int64_t span, bo
 #pragma unroll
     for (int r = 0; r < kUcRounds; ++r) {
         if ((active >> r) & 1u) {
-            const int64_t v = verts[r];
-            const int64_t rb = a.offsets[v];
-            const uint32_t d = static_cast<uint32_t>(a.offsets[v + 1] - rb);
-            a.q_rb[slot] = rb;
+            const uint32_t d = static_cast<uint32_t>(re[r] - rb[r]);
+            a.q_rb[slot] = rb[r];
             a.q_eoff[slot] = eo;
-            a.q_v[slot] = static_cast<int32_t>(v);
+            a.q_v[slot] = static_cast<int32_t>(vs[r]);
             a.q_fm[slot] = newbits[r];
             mark_tiles(a.tile_start, a.tile_edges, eo, d, slot);
             ++slot;
@@ -1015,8 +1044,12 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     int uocc = 0;
     GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&uocc, reinterpret_cast<const void*>(ms_update_compact_kernel),
                                                        kUcBlock, 0));
-    const uint32_t ugrid =
-        static_cast<uint32_t>(std::max<int64_t>(1, std::min<int64_t>(ms.nchunks, int64_t(g.num_cus) * uocc)));
+    // At least 2 workgroups per CU whatever the occupancy query reports (it
+    // grants 1 for this kernel's registers): chunks are independent, and the
+    // extra workgroups cost nothing where they cannot be resident (R-MAT s24
+    // 64-source batch 7.51 -> 7.47 ms, s20 0.955 -> 0.944 ms).
+    const uint32_t ugrid = static_cast<uint32_t>(
+        std::max<int64_t>(1, std::min<int64_t>(ms.nchunks, int64_t(g.num_cus) * std::max(uocc, 2))));
     // finish: up to 64 workgroups per CU (R-MAT s24, 64-source batch: 8.08 ms
     // against 8.24 ms with 8 per CU, 8.15 with 32, 8.10 with one 64-vertex
     // group per wave).
@@ -1027,6 +1060,11 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     std::memcpy(ms.h_params->sources, h_sources, sizeof(int32_t) * k);
     ms.h_params->k = k;
     ms.h_params->levels = d_levels;
+    // A fresh workspace, or one a failed batch left, is cleared whole.
+    ms.h_params->clear_all = ms.tail_clean ? 0 : 1;
+    std::memcpy(ms.h_params->prev_sources, ms.prev_sources, sizeof(ms.prev_sources));
+    ms.h_params->prev_k = ms.tail_clean ? ms.prev_k : 0;
+    ms.tail_clean = false;
     auto launch_prologue = [&]() {
         GG_HIP(hipMemcpyAsync(ms.params, ms.h_params, sizeof(MsParams), hipMemcpyHostToDevice, stream));
         hipLaunchKernelGGL(ms_clear_kernel, dim3(cgrid), dim3(256), 0, stream, a);
@@ -1040,11 +1078,29 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     };
 
     const int depth_hint = std::min(std::max(ms.depth_hint, 4), kMaxCapturedLevels);
+    // The finish runs right after the first depth_hint levels when they have
+    // ended the search (the kernel checks the done flag), and the control
+    // block is snapshotted behind it: a batch that fits the hint costs the
+    // host one wait.
+    auto launch_finish = [&](int only_if_done) {
+        hipLaunchKernelGGL(ms_finish_kernel, dim3(fgrid), dim3(256), 0, stream, a, only_if_done);
+    };
+    auto snapshot_to = [&](int slot) {
+        GG_HIP(hipMemcpyAsync(ms.h_ctl + slot * kCtlWords, ms.ctl, sizeof(unsigned long long) * kCtlWords,
+                              hipMemcpyDeviceToHost, stream));
+    };
+    auto first_part = [&]() {
+        launch_prologue();
+        for (int i = 0; i < depth_hint; ++i) launch_level();
+        launch_finish(1);
+        snapshot_to(0);
+    };
     GG_HIP(hipEventRecord(ev_start, stream));
     if (g.ms_cfg.use_graph) {
-        // One captured sequence: parameter upload, clears, depth_hint levels.
-        // Everything batch-specific is read from device memory, so the graph
-        // is rebuilt only when the kernel shape or the depth hint changes.
+        // One captured sequence: parameter upload, clears, depth_hint levels,
+        // the conditional finish and the snapshot.  Everything batch-specific
+        // is read from device memory, so the graph is rebuilt only when the
+        // kernel shape or the depth hint changes.
         const bool reuse = ms.graph_exec && ms.graph_depth == depth_hint && ms.graph_block == g.ms_cfg.block &&
                            ms.graph_kpt == g.ms_cfg.edges_per_thread && ms.graph_sparse == a.sparse_edges &&
                            ms.graph_bu == int(a.bottom_up) && ms.graph_alpha == int(a.alpha) &&
@@ -1055,8 +1111,7 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
             hipGraph_t captured = nullptr;
             GG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             try {
-                launch_prologue();
-                for (int i = 0; i < depth_hint; ++i) launch_level();
+                first_part();
             } catch (...) {
                 hipGraph_t dummy = nullptr;
                 (void)hipStreamEndCapture(stream, &dummy);
@@ -1077,27 +1132,27 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
             GG_HIP(hipEventRecord(ev_start, stream));
         }
         GG_HIP(hipGraphLaunch(ms.graph_exec, stream));
-        L = depth_hint;
     } else {
-        launch_prologue();
-        for (int i = 0; i < depth_hint; ++i) launch_level();
+        first_part();
     }
+    L = depth_hint;
     GG_HIP(hipGetLastError());
-    // Deeper than the hint: batches of levels of doubling size (at most
-    // kMaxLevelBatch) are pipelined -- the done flag of batch b-1 is checked
-    // while batch b runs; levels past the end are no-ops.
-    auto snapshot = [&](int slot) {
-        GG_HIP(hipMemcpyAsync(ms.h_ctl + slot * kCtlWords, ms.ctl, sizeof(unsigned long long) * kCtlWords,
-                              hipMemcpyDeviceToHost, stream));
-        GG_HIP(hipEventRecord(ev_batch[slot], stream));
-    };
+    GG_HIP(hipEventRecord(ev_stop, stream));
+    GG_HIP(hipEventRecord(ev_batch[0], stream));
     auto done = [&](int slot) {
         GG_HIP(hipEventSynchronize(ev_batch[slot]));
         const unsigned long long* c = ms.h_ctl + slot * kCtlWords;
         return c[MS_DONE] != 0 || c[MS_ERR] != 0;
     };
-    snapshot(0);
+    int last = 0;  // h_ctl slot of the final control block
     if (!done(0)) {
+        // Deeper than the hint: batches of levels of doubling size (at most
+        // kMaxLevelBatch) are pipelined -- the done flag of batch b-1 is
+        // checked while batch b runs; levels past the end are no-ops.
+        auto snapshot = [&](int slot) {
+            snapshot_to(slot);
+            GG_HIP(hipEventRecord(ev_batch[slot], stream));
+        };
         int batch = 8;
         for (int i = 0; i < batch; ++i) launch_level();
         GG_HIP(hipGetLastError());
@@ -1108,20 +1163,28 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
             for (int i = 0; i < batch; ++i) launch_level();
             GG_HIP(hipGetLastError());
             snapshot(b & 1);
-            if (done((b - 1) & 1)) break;
+            if (done((b - 1) & 1)) {
+                last = b & 1;
+                break;
+            }
         }
+        // All levels have run (later ones were no-ops): level rows from the
+        // byte lines, then the final control block (behind the snapshot
+        // still in flight to the same slot).
+        launch_finish(0);
+        GG_HIP(hipGetLastError());
+        GG_HIP(hipEventRecord(ev_stop, stream));
+        snapshot(last);
+        GG_HIP(hipEventSynchronize(ev_batch[last]));
     }
-    // All levels have run (later ones are no-ops): level rows from the byte lines.
-    hipLaunchKernelGGL(ms_finish_kernel, dim3(fgrid), dim3(256), 0, stream, a);
-    GG_HIP(hipGetLastError());
-    GG_HIP(hipEventRecord(ev_stop, stream));
-    unsigned long long fin[kCtlWords];
-    GG_HIP(hipMemcpyAsync(fin, ms.ctl, sizeof(fin), hipMemcpyDeviceToHost, stream));
-    GG_HIP(hipStreamSynchronize(stream));
+    const unsigned long long* fin = ms.h_ctl + last * kCtlWords;
     if (fin[MS_ERR] & MS_ERR_TILE) fail(GB_EDEVICE, "internal error: inconsistent frontier tile index");
     if (fin[MS_ERR] & MS_ERR_RANGE)
         fail(GB_EDEVICE, "internal error: frontier entry or column index out of range");
     if (!fin[MS_DONE]) fail(GB_EDEVICE, "internal error: batch did not reach an empty frontier");
+    std::memcpy(ms.prev_sources, h_sources, sizeof(int32_t) * k);
+    ms.prev_k = k;
+    ms.tail_clean = true;
     ms.last_depth = static_cast<int>(fin[MS_MAXLEV]) + 1;
     ms.last_bu_levels = static_cast<int>(fin[MS_BULEVELS]);
     // Levels queued for the next batch: this depth + 1 (an empty level ends a

@@ -126,15 +126,59 @@ void Server::fail_query_locked(Query& q, int status, const std::string& err) {
     ++failed_;
 }
 
+// Queries of graph gid waiting for a batch (stale entries at the front of
+// its queue are dropped), counted up to `cap`.
+int Server::waiting_locked(int gid, int cap) {
+    std::deque<uint64_t>& pq = pending_[gid];
+    auto waiting = [&](uint64_t t) {
+        auto qi = queries_.find(t);
+        return qi != queries_.end() && qi->second.status == GB_EAGAIN && !qi->second.batch;
+    };
+    while (!pq.empty() && !waiting(pq.front())) pq.pop_front();
+    int k = 0;
+    for (uint64_t t : pq)
+        if (waiting(t) && ++k == cap) break;
+    return k;
+}
+
+// The graph to start a batch on now, or -1.  Graphs whose batches are
+// expected to run over long_batch_ms hold at most lanes - 1 lanes together,
+// so a short query never waits behind them; when that cap is reached, their
+// arrivals gather for the next batch.  Among the eligible graphs, the one
+// whose oldest query has waited longest goes first.
+int Server::pick_graph_locked() {
+    int idle = 0, long_busy = 0;
+    for (const Lane& l : lanes_) {
+        if (!l.busy) ++idle;
+        else if (l.long_job) ++long_busy;
+    }
+    if (idle == 0) return -1;
+    int best = -1;
+    Clock::time_point best_arrival{};
+    for (auto& kv : pending_) {
+        const int gid = kv.first;
+        const int k = waiting_locked(gid, cfg_.max_batch);
+        if (k == 0) continue;
+        if (cfg_.lanes > 1 && long_busy >= cfg_.lanes - 1 && exec_->expected_ms(gid, k) > cfg_.long_batch_ms)
+            continue;
+        const Clock::time_point a = queries_[kv.second.front()].arrival;
+        if (best < 0 || a < best_arrival) {
+            best = gid;
+            best_arrival = a;
+        }
+    }
+    return best;
+}
+
 bool Server::try_dispatch(std::unique_lock<std::mutex>&) {
-    // Oldest query still waiting decides the graph (first come, first served).
     while (!arrival_order_.empty()) {
         auto it = queries_.find(arrival_order_.front());
         if (it != queries_.end() && it->second.status == GB_EAGAIN && !it->second.batch) break;
         arrival_order_.pop_front();
     }
     if (arrival_order_.empty()) return true;
-    const int gid = queries_[arrival_order_.front()].gid;
+    const int gid = pick_graph_locked();
+    if (gid < 0) return false;  // wait for a lane to finish (or a new arrival)
     std::deque<uint64_t>& pq = pending_[gid];
 
     int lane = -1;
@@ -146,16 +190,8 @@ bool Server::try_dispatch(std::unique_lock<std::mutex>&) {
 
     const int64_t n = exec_->vertices(gid);
     const int64_t row_bytes = n * int64_t(sizeof(int32_t));
-    int k = 0;  // queries of this graph that can go into the batch
-    for (uint64_t t : pq) {
-        auto qi = queries_.find(t);
-        if (qi != queries_.end() && qi->second.status == GB_EAGAIN && !qi->second.batch && ++k == cfg_.max_batch)
-            break;
-    }
-    if (k == 0) {  // only stale entries
-        pq.clear();
-        return true;
-    }
+    int k = waiting_locked(gid, cfg_.max_batch);  // queries of this graph that can go into the batch
+    if (k == 0) return true;
     const int64_t ws_bytes =
         lanes_[lane].has_workspace.count(gid) ? 0 : static_cast<int64_t>(exec_->workspace_bytes(gid, cfg_.plain));
     std::vector<Buffer>& pool = free_buffers_[gid];
@@ -265,6 +301,7 @@ bool Server::try_dispatch(std::unique_lock<std::mutex>&) {
     b->live = static_cast<int>(b->tickets.size());
     lanes_[lane].job = b;
     lanes_[lane].busy = true;
+    lanes_[lane].long_job = exec_->expected_ms(gid, static_cast<int>(b->tickets.size())) > cfg_.long_batch_ms;
     ++in_flight_;
     ++batches_;
     batched_queries_ += static_cast<int64_t>(b->tickets.size());
@@ -296,6 +333,7 @@ void Server::lane_loop(int lane) {
         finish_batch(b, status, err);
         lanes_[lane].job = nullptr;
         lanes_[lane].busy = false;
+        lanes_[lane].long_job = false;
         --in_flight_;
         cv_sched_.notify_all();
         cv_done_.notify_all();

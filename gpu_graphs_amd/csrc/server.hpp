@@ -6,7 +6,12 @@
 // queries on one graph and hands them to idle lanes.  Each lane executes one
 // batch at a time on its own GPU stream, so batches on different lanes run
 // concurrently on the device, and queries that arrive while lanes are busy
-// join the next batch as soon as a lane and memory free up.  Admission is by
+// join the next batch as soon as a lane and memory free up.  Graphs whose
+// batches are expected to run long (deep graphs) hold at most lanes - 1 lanes
+// together, so short queries always find a lane; once they hold that many,
+// their arrivals gather for their next batch.  Among the graphs that may
+// start a batch, the one whose oldest query has waited longest goes first.
+// Admission is by
 // a device-memory budget: a batch of k queries needs a result buffer of k
 // rows of n int32 (rounded up to a power of two, pooled per graph and size)
 // and, the first time a lane serves a graph, that lane's workspace.  When
@@ -40,6 +45,7 @@ struct ServerConfig {
     bool plain = false;            // one single-source BFS per query, no batching
     int64_t memory_budget = 0;     // bytes of device memory for workspaces + results
     int64_t queue_capacity = 1 << 20;
+    double long_batch_ms = 20.0;   // batches expected to run longer hold at most lanes - 1 lanes
 };
 
 class Executor {
@@ -59,6 +65,13 @@ class Executor {
     virtual void run(int lane, int gid, const int32_t* sources, int k, int32_t* result, bool plain) = 0;
     // Copies n int32 of a result row to the host.
     virtual void fetch(const int32_t* row, int32_t* host, int64_t n) = 0;
+    // Expected run time of a batch of k queries on graph gid in ms (learned
+    // from earlier runs), or a negative value when not known yet.
+    virtual double expected_ms(int gid, int k) {
+        (void)gid;
+        (void)k;
+        return -1.0;
+    }
 };
 
 struct QueryLatency {
@@ -127,12 +140,15 @@ class Server {
         std::thread thr;
         std::shared_ptr<Batch> job;
         bool busy = false;
+        bool long_job = false;  // the running batch was expected to take over long_batch_ms
         std::map<int, bool> has_workspace;
     };
 
     void scheduler_loop();
     void lane_loop(int lane);
     bool try_dispatch(std::unique_lock<std::mutex>& lk);
+    int pick_graph_locked();
+    int waiting_locked(int gid, int cap);
     void finish_batch(const std::shared_ptr<Batch>& b, int status, const std::string& err);
     void release_buffer_locked(Buffer buf);
     void fail_query_locked(Query& q, int status, const std::string& err);

@@ -12,6 +12,7 @@
 // running averages of the device time of past runs on that graph.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -86,11 +87,21 @@ class HipExecutor : public Executor {
                 r->bfs(sources[i], result + int64_t(i) * g->n, nullptr, nullptr);
                 total += r->last_ms;
             }
-            record(gid, true, total / k);
+            record(gid, true, 1, total / k);
         } else {
             r->bfs_batch(sources, k, result, nullptr);
-            record(gid, false, r->last_ms);
+            record(gid, false, k, r->last_ms);
         }
+    }
+
+    // The cheaper of the two ways run() can take, from the running averages.
+    double expected_ms(int gid, int k) override {
+        std::lock_guard<std::mutex> lk(cost_mu_);
+        const Cost& c = cost_[gid];
+        const double single = c.single_ms >= 0 ? k * c.single_ms : -1.0;
+        const double batch = k == 1 ? -1.0 : batch_estimate(c, k);
+        if (batch < 0) return single;
+        return single < 0 ? batch : std::min(single, batch);
     }
 
     void fetch(const int32_t* row, int32_t* host, int64_t n) override {
@@ -103,10 +114,31 @@ class HipExecutor : public Executor {
     int device() const { return device_; }
 
    private:
+    // Running averages of device time per graph: per query run alone, and per
+    // multi-source batch by size class (k in (2^(c-1), 2^c], c = 0..6): on
+    // deep graphs a batch of 64 costs several times a batch of 4.
+    static constexpr int kClasses = 7;
     struct Cost {
         double single_ms = -1;  // per query
-        double batch_ms = -1;   // per multi-source batch
+        double batch_ms[kClasses] = {-1, -1, -1, -1, -1, -1, -1};
     };
+    static int size_class(int k) {
+        int c = 0;
+        while ((1 << c) < k && c < kClasses - 1) ++c;
+        return c;
+    }
+    // Batch cost of size k: its class's average, else the nearest class
+    // measured (classes below scaled up by the size ratio, an upper bound:
+    // a batch never costs more than its queries one at a time).
+    static double batch_estimate(const Cost& c, int k) {
+        const int kc = size_class(k);
+        if (c.batch_ms[kc] >= 0) return c.batch_ms[kc];
+        for (int d = 1; d < kClasses; ++d) {
+            if (kc + d < kClasses && c.batch_ms[kc + d] >= 0) return c.batch_ms[kc + d];
+            if (kc - d >= 0 && c.batch_ms[kc - d] >= 0) return c.batch_ms[kc - d] * double(1 << d);
+        }
+        return -1;
+    }
 
     // Unknown costs are learned first: a lone query runs single-source, a
     // batch of two or more runs multi-source.
@@ -114,14 +146,15 @@ class HipExecutor : public Executor {
         std::lock_guard<std::mutex> lk(cost_mu_);
         const Cost& c = cost_[gid];
         if (k == 1 && c.single_ms < 0) return true;
-        if (c.batch_ms < 0) return k == 1;
+        const double b = batch_estimate(c, k);
+        if (b < 0) return k == 1;
         if (c.single_ms < 0) return false;
-        return k * c.single_ms <= c.batch_ms;
+        return k * c.single_ms <= b;
     }
 
-    void record(int gid, bool single, double ms) {
+    void record(int gid, bool single, int k, double ms) {
         std::lock_guard<std::mutex> lk(cost_mu_);
-        double& avg = single ? cost_[gid].single_ms : cost_[gid].batch_ms;
+        double& avg = single ? cost_[gid].single_ms : cost_[gid].batch_ms[size_class(k)];
         avg = avg < 0 ? ms : 0.7 * avg + 0.3 * ms;
     }
 

@@ -74,6 +74,11 @@ struct FakeExec : Executor {
     void fetch(const int32_t* row, int32_t* host, int64_t count) override {
         std::memcpy(host, row, sizeof(int32_t) * count);
     }
+    std::map<int, double> expect;  // expected batch ms per graph (set before the first submit)
+    double expected_ms(int gid, int) override {
+        auto it = expect.find(gid);
+        return it == expect.end() ? -1.0 : it->second;
+    }
 };
 
 struct Fixture {
@@ -354,6 +359,71 @@ static void test_idle_workspaces_make_room_for_other_graphs() {
     CHECK(st.memory_used <= cfg.memory_budget);
 }
 
+static void test_arrivals_for_a_long_graph_gather() {
+    // Batches of graph 0 run long and may hold one of the two lanes: queries
+    // arriving while one runs wait for the next batch, the other lane stays
+    // free for short queries.
+    ServerConfig cfg;
+    cfg.lanes = 2;
+    Fixture f(cfg, {{0, 100}}, 100);
+    f.ex->expect = {{0, 100.0}};
+    uint64_t first = f.srv->submit(0, 1);
+    std::this_thread::sleep_for(std::chrono::milliseconds(20));
+    std::vector<uint64_t> later;
+    for (int i = 0; i < 10; ++i) later.push_back(f.srv->submit(0, 10 + i));
+    QueryLatency lat;
+    CHECK(f.srv->wait(first, 10000, nullptr, &lat) == GB_OK);
+    CHECK(lat.batch_size == 1);
+    for (uint64_t t : later) {
+        CHECK(f.srv->wait(t, 10000, nullptr, &lat) == GB_OK);
+        CHECK(lat.batch_size == 10);
+        f.srv->release(t);
+    }
+    f.srv->release(first);
+    CHECK(f.ex->max_running.load() == 1);
+}
+
+static void test_short_graph_uses_idle_lanes() {
+    // Short batches (no expectation): a second query of a running graph
+    // starts at once on an idle lane.
+    ServerConfig cfg;
+    cfg.lanes = 4;
+    Fixture f(cfg, {{0, 100}}, 200);
+    uint64_t a = f.srv->submit(0, 1);
+    std::this_thread::sleep_for(std::chrono::milliseconds(20));
+    uint64_t b = f.srv->submit(0, 2);
+    QueryLatency la, lb;
+    CHECK(f.srv->wait(a, 10000, nullptr, &la) == GB_OK);
+    CHECK(f.srv->wait(b, 10000, nullptr, &lb) == GB_OK);
+    CHECK(la.batch_size == 1 && lb.batch_size == 1);
+    CHECK(lb.queue_ms < 120);  // did not wait for a's 200 ms batch
+    f.srv->release(a);
+    f.srv->release(b);
+}
+
+static void test_long_batches_leave_a_lane_for_short_ones() {
+    // Three graphs with long batches and one short graph on three lanes: the
+    // long graphs hold at most two lanes, so the short query starts at once
+    // and the third long query waits for a long lane.
+    ServerConfig cfg;
+    cfg.lanes = 3;
+    Fixture f(cfg, {{0, 100}, {1, 100}, {2, 100}, {3, 100}}, 250);
+    f.ex->expect = {{0, 300.0}, {2, 300.0}, {3, 300.0}};
+    uint64_t a = f.srv->submit(0, 1);
+    uint64_t b = f.srv->submit(2, 1);
+    uint64_t c = f.srv->submit(3, 1);
+    std::this_thread::sleep_for(std::chrono::milliseconds(20));
+    uint64_t s = f.srv->submit(1, 1);
+    QueryLatency ls, lc;
+    CHECK(f.srv->wait(s, 10000, nullptr, &ls) == GB_OK);
+    CHECK(ls.queue_ms < 150);
+    CHECK(f.srv->wait(c, 10000, nullptr, &lc) == GB_OK);
+    CHECK(lc.queue_ms >= 150);
+    for (uint64_t t : {a, b}) CHECK(f.srv->wait(t, 10000, nullptr, nullptr) == GB_OK);
+    for (uint64_t t : {a, b, c, s}) f.srv->release(t);
+    CHECK(f.ex->max_running.load() <= 3);
+}
+
 int main() {
     struct {
         const char* name;
@@ -368,6 +438,9 @@ int main() {
                  {"backpressure_and_cancel", test_backpressure_and_cancel},
                  {"shutdown_with_pending", test_shutdown_with_pending},
                  {"errors", test_errors},
+                 {"long_graph_arrivals_gather", test_arrivals_for_a_long_graph_gather},
+                 {"short_graph_uses_idle_lanes", test_short_graph_uses_idle_lanes},
+                 {"long_batches_leave_a_lane", test_long_batches_leave_a_lane_for_short_ones},
                  {"idle_workspaces_make_room", test_idle_workspaces_make_room_for_other_graphs}};
     for (auto& t : tests) {
         const int before = g_failures;

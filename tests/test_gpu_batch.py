@@ -296,6 +296,32 @@ def test_level_lines_of_earlier_batches_are_not_read(gpu):
         check_batch(D, dd, sources)
 
 
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_edgeless_sources_across_batches(gpu, use_graph):
+    # After the first batch only vertices with edges are cleared; an edgeless
+    # source is cleared by the next batch's seed.  Its stale state would show
+    # as a wrong row or count under the source that takes its bit next.
+    G = gg.Graph.rmat(13, seed=7)
+    assert G.is_symmetric
+    dg = gg.DeviceGraph(G, gpu)
+    dg.batch_config(use_graph=use_graph)
+    deg = np.diff(G.offsets)
+    iso = np.nonzero(deg == 0)[0][:4].astype(np.int32)
+    normal = np.nonzero(deg)[0][:60].astype(np.int32)
+    assert iso.size == 4
+    batches = [
+        np.concatenate([normal[:3], iso[:1], normal[3:20]]),          # iso[0] at bit 3
+        np.concatenate([normal[20:30], iso[:1], normal[30:40]]),      # bit 3 now a normal source, iso[0] at bit 10
+        normal[:64],                                                  # no edgeless source
+        np.array([iso[1], iso[1], iso[2], normal[5]], dtype=np.int32),  # repeats
+        np.concatenate([iso, normal[:8]]),
+        normal[10:50],
+    ]
+    for i, sources in enumerate(batches):
+        log(f"edgeless sources, batch {i} ({len(sources)} sources)")
+        check_batch(G, dg, sources)
+
+
 def test_levels_past_the_byte_range(gpu):
     # A 700-vertex path: levels go past 254, where the batch stores levels
     # into the rows directly instead of the byte lines.

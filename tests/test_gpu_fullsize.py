@@ -72,9 +72,11 @@ def test_road_grid_7000_every_bench_source(gpu):
     dg = gg.DeviceGraph(G, gpu)
     sources = bench.grid_sources(G)
     assert len(sources) == 16
-    check_every_source(G, dg, sources[:2], min_levels=1000)
-    assert dg.get_deep_config()["used"]  # deep mode engaged after the first searches
-    check_every_source(G, dg, sources[2:], min_levels=1000)
+    # Two thin, deep searches teach the handle the graph's shape; the third
+    # runs one launch per level (deep mode).
+    check_every_source(G, dg, sources[:3], min_levels=1000)
+    assert dg.get_deep_config()["used"]
+    check_every_source(G, dg, sources[3:], min_levels=1000)
 
 
 def test_rmat24_two_batches_every_row(gpu):
