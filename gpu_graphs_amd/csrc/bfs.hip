@@ -93,6 +93,7 @@ enum : int {
     CTL_DCNT0 = 20,    // 2 words: (count << 32 | edges) of the winners of a dense level L at dcnt[(L+1) & 1]
     CTL_FCNT0 = 22,    // 2 words, one-launch levels: (count << 32 | edges) of level L at fcnt[L & 1]
     CTL_RCNT0 = 24,    // 2 words, one-launch levels: queued vertices of levels <= L at rcnt[L & 1]
+    CTL_DEEPE0 = 26,   // 2 words, deep levels: CSR entries of level L's frontier at deepe[L & 1]
     CTL_WORDS = 32
 };
 enum : unsigned long long { ERR_TILE = 1, ERR_RANGE = 2 };
@@ -905,6 +906,150 @@ __global__ __launch_bounds__(kCompactBlock) void bfs_compact_kernel(BfsArgs a, i
     }
 }
 
+// Deep, thin levels (road-like graphs: thousands of levels of a few thousand
+// vertices).  A level's cost there is its chain of dependent memory round
+// trips, not bytes, so these levels use a kernel with a short chain: a thread
+// takes one frontier vertex (queue entries are bare vertex ids, loaded
+// together with the level's counter), loads its row bounds, then its columns,
+// claims them with atomicOr on the visited bitmap (no probe first: few
+// claims per level), and each wave reserves its winners' queue slots with
+// one atomicAdd -- five round trips against about nine of the edge-balanced
+// kernel.  Rows longer than kDeepRow go to the wave, which claims 64 entries
+// per step.  Level L's CSR entries are summed during its expansion
+// (deepe[L & 1]) and booked by block 0 of level L+1; the search ends when a
+// level finds its frontier empty.
+constexpr int kDeepBlock = 256;
+constexpr int kDeepRow = 8;
+
+__global__ __launch_bounds__(kDeepBlock) void bfs_deep_kernel(BfsArgs a, int L) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const int cur = L & 1, nxt = cur ^ 1;
+    const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * kDeepBlock + threadIdx.x;
+    const uint64_t gstride = static_cast<uint64_t>(gridDim.x) * kDeepBlock;
+    const uint64_t n = static_cast<uint64_t>(a.n);
+    // This thread's first frontier entry is loaded beside the counters; it
+    // is used only if it lies below the frontier size.
+    const int32_t v_first = gtid < n ? a.q_v[cur][gtid] : -1;
+    const unsigned long long stop = a.ctl[CTL_DONE] | a.ctl[CTL_ERR];
+    const unsigned long long cnt = a.ctl[CTL_RING0 + L % 3];
+    if (stop) return;
+    const uint32_t F = static_cast<uint32_t>(cnt >> 32);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const unsigned long long e_prev = a.ctl[CTL_DEEPE0 + ((L + 1) & 1)];  // level L-1's entries
+        a.ctl[CTL_DEEPE0 + ((L + 1) & 1)] = 0;                                 // level L+1 sums there next
+        atomicAdd(&a.ctl[CTL_MREACH], e_prev);
+        atomicMax(&a.ctl[CTL_MAXE], e_prev);
+        if (F == 0) {
+            a.ctl[CTL_DONE] = 1;
+        } else {
+            a.ctl[CTL_RING0 + (L + 2) % 3] = 0;  // held level L-1's counter; counts level L+2 next
+            atomicAdd(&a.ctl[CTL_NREACH], static_cast<unsigned long long>(F));
+            atomicMax(&a.ctl[CTL_MAXLEV], static_cast<unsigned long long>(L));
+        }
+    }
+    if (F == 0) return;
+    if (F > n) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.ctl[CTL_ERR], ERR_RANGE);
+        return;
+    }
+    int32_t* const levels = a.p->levels;
+    int32_t* const parents = a.p->parents;
+    unsigned long long* const ring_next = &a.ctl[CTL_RING0 + (L + 1) % 3];
+    bool bad = false;
+    uint64_t my_edges = 0;
+    // Reserves c queue slots for each lane (wave-uniform call, one atomicAdd
+    // per wave); returns the lane's first slot, or ~0u if the queue would
+    // overflow (reported as an error).
+    auto reserve = [&](uint32_t c) -> uint32_t {
+        const uint32_t incl = wave_inclusive_scan(c);
+        const uint32_t tot = __shfl(incl, 63, 64);
+        if (tot == 0) return 0u;
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(ring_next, static_cast<unsigned long long>(tot) << 32);
+        base = __shfl(base, 0, 64);
+        if ((base >> 32) + tot > n) {
+            bad = true;
+            return ~0u;
+        }
+        return static_cast<uint32_t>(base >> 32) + incl - c;
+    };
+    for (uint64_t i = gtid;; i += gstride) {
+        const bool has = i < F;
+        if (!wave_ballot(has)) break;  // the lanes of a wave hold consecutive entries
+        int32_t v = -1;
+        if (has) {
+            v = i == gtid ? v_first : a.q_v[cur][i];
+            if (static_cast<uint64_t>(static_cast<uint32_t>(v)) >= n) {
+                bad = true;
+                v = -1;
+            }
+        }
+        int64_t rb = 0, re = 0;
+        if (v >= 0) {
+            rb = a.offsets[v];
+            re = a.offsets[v + 1];
+        }
+        const uint32_t deg = static_cast<uint32_t>(re - rb);
+        my_edges += deg;
+        const bool short_row = v >= 0 && deg <= kDeepRow;
+        int32_t u[kDeepRow];
+#pragma unroll
+        for (int j = 0; j < kDeepRow; ++j) {
+            u[j] = (short_row && static_cast<uint32_t>(j) < deg) ? a.cols[rb + j] : -1;
+            if (u[j] >= 0 && static_cast<uint64_t>(u[j]) >= n) {
+                bad = true;
+                u[j] = -1;
+            }
+        }
+        uint32_t old[kDeepRow];
+#pragma unroll
+        for (int j = 0; j < kDeepRow; ++j)
+            old[j] = u[j] >= 0 ? atomicOr(&a.vis[static_cast<uint32_t>(u[j]) >> 5], 1u << (u[j] & 31)) : ~0u;
+        uint32_t won = 0;  // bit j: u[j] claimed by this lane
+#pragma unroll
+        for (int j = 0; j < kDeepRow; ++j) {
+            if (u[j] >= 0 && !((old[j] >> (u[j] & 31)) & 1u)) {
+                levels[u[j]] = L + 1;
+                if (parents) parents[u[j]] = v;
+                won |= 1u << j;
+            }
+        }
+        const uint32_t slot0 = reserve(static_cast<uint32_t>(__popc(won)));
+        if (won && slot0 != ~0u) {
+            uint32_t r = 0;
+#pragma unroll
+            for (int j = 0; j < kDeepRow; ++j)
+                if ((won >> j) & 1u) a.q_v[nxt][slot0 + r++] = u[j];
+        }
+        // Long rows, one at a time by the whole wave.
+        uint64_t lm = wave_ballot(v >= 0 && !short_row);
+        while (lm) {
+            const int hl = __ffsll(static_cast<long long>(lm)) - 1;
+            lm &= lm - 1;
+            const int32_t hv = __shfl(v, hl, 64);
+            const int64_t hrb = __shfl(rb, hl, 64), hre = __shfl(re, hl, 64);
+            for (int64_t b = hrb; b < hre; b += 64) {
+                int32_t uu = b + lane < hre ? a.cols[b + lane] : -1;
+                if (uu >= 0 && static_cast<uint64_t>(uu) >= n) {
+                    bad = true;
+                    uu = -1;
+                }
+                const uint32_t o = uu >= 0 ? atomicOr(&a.vis[static_cast<uint32_t>(uu) >> 5], 1u << (uu & 31)) : ~0u;
+                const bool w = uu >= 0 && !((o >> (uu & 31)) & 1u);
+                if (w) {
+                    levels[uu] = L + 1;
+                    if (parents) parents[uu] = hv;
+                }
+                const uint32_t slot = reserve(w ? 1u : 0u);
+                if (w && slot != ~0u) a.q_v[nxt][slot] = uu;
+            }
+        }
+    }
+    const uint64_t e = wave_sum(my_edges);
+    if (lane == 0 && e) atomicAdd(&a.ctl[CTL_DEEPE0 + (L & 1)], static_cast<unsigned long long>(e));
+    if (bad) atomicOr(&a.ctl[CTL_ERR], ERR_RANGE);
+}
+
 // Upload of a symmetric graph: each row's neighbours ordered by descending
 // degree bucket (file order inside a bucket).  A bottom-up level probes a row
 // from its start and stops at the first frontier neighbour; the hubs, which
@@ -1300,6 +1445,8 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     // compact(): chunks are independent; up to 2 workgroups of 1024 per CU.
     const uint32_t cgrid = static_cast<uint32_t>(std::min<int64_t>(ws.nchunks, int64_t(g.num_cus) * 2));
     const uint32_t clear_grid = static_cast<uint32_t>(std::min<int64_t>((n + 255) / 256, int64_t(g.num_cus) * 8));
+    const uint32_t deep_grid = static_cast<uint32_t>(
+        std::max<int64_t>(1, std::min<int64_t>((n + kDeepBlock - 1) / kDeepBlock, int64_t(g.num_cus) * 2)));
     // Levels [bu_lo, bu_hi], learned from earlier searches of this graph, may
     // go bottom-up; other levels run top-down.
     auto bu_at = [&](int L) { return a.bottom_up && L >= bu_lo && L <= bu_hi; };
@@ -1324,8 +1471,12 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
             hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(g.cfg.block), 0, stream, a, L, bu, bu1, 0);
             return;
         }
-        if (deep || L == 0) {
-            hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(g.cfg.block), 0, stream, a, L, 0, deep ? 0 : bu1, 1);
+        if (deep) {
+            hipLaunchKernelGGL(bfs_deep_kernel, dim3(deep_grid), dim3(kDeepBlock), 0, stream, a, L);
+            return;
+        }
+        if (L == 0) {
+            hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(g.cfg.block), 0, stream, a, L, 0, bu1, 1);
             return;
         }
         hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(g.cfg.block), 0, stream, a, L, bu, bu1, 0);

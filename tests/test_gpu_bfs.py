@@ -282,8 +282,8 @@ def check_search(G, dg, s, parents=True):
 @pytest.mark.parametrize("mode", [gg.GB_MODE_GRAPH, gg.GB_MODE_ASYNC, gg.GB_MODE_LEVEL_LOOP])
 @pytest.mark.parametrize("block,k", [(256, 4), (1024, 1), (64, 16)])
 def test_deep_mode_one_launch_per_level(gpu, mode, block, k):
-    # Deep mode: every level sparse top-down in one launch, the last
-    # workgroup doing the level's bookkeeping.
+    # Deep mode: every level in one launch of the deep kernel (a thread per
+    # frontier vertex), block 0 of each level doing the bookkeeping.
     G = gg.Graph.grid(400, 300, 0.05, seed=86)
     dg = gg.DeviceGraph(G, gpu)
     dg.config(block=block, edges_per_thread=k, mode=mode)
@@ -312,6 +312,31 @@ def test_deep_mode_on_wide_levels(gpu):
     dg.deep_config(gg.GB_DEEP_OFF)
     check_search(G, dg, nc + 7)
     assert not dg.get_deep_config()["used"]
+
+
+def test_deep_mode_row_lengths_and_edgeless_sources(gpu):
+    # A 2000-vertex path whose every 100th vertex also has a fan of leaves,
+    # with fans of 1..8 (claimed by the thread) and 9, 63, 64, 65, 300
+    # entries (claimed by the wave 64 at a time), and some edgeless vertices.
+    plen = 2000
+    fans = [1, 2, 7, 8, 9, 63, 64, 65, 300] * 3
+    src, dst, nxt = [], [], plen
+    for i in range(plen - 1):
+        src += [i, i + 1]
+        dst += [i + 1, i]
+    for j, f in enumerate(fans):
+        hub = 100 * (j % 20) + 50
+        for _ in range(f):
+            src += [hub, nxt]
+            dst += [nxt, hub]
+            nxt += 1
+    n = nxt + 5  # five edgeless vertices at the end
+    G = gg.Graph.from_edges(n, np.array(src, dtype=np.int32), np.array(dst, dtype=np.int32))
+    dg = gg.DeviceGraph(G, gpu)
+    dg.deep_config(gg.GB_DEEP_ON)
+    for s in (0, 1234, plen - 1, plen + 10, n - 1, n - 3):
+        check_search(G, dg, s)
+        assert dg.get_deep_config()["used"]
 
 
 def test_deep_mode_auto_and_directed(gpu):
