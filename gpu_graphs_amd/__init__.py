@@ -315,6 +315,16 @@ class DeviceGraph:
         check(lib().gb_batch_get_direction_config(self._h, C.byref(e), C.byref(a)))
         return {"enable": bool(e.value), "alpha": a.value}
 
+    def batch_deep_config(self, mode=GB_DEEP_AUTO):
+        """Fused launches per level for deep batches with thin frontiers (GB_DEEP_*)."""
+        check(lib().gb_batch_deep_config(self._h, int(mode)))
+
+    def get_batch_deep_config(self):
+        """{"mode": GB_DEEP_*, "used": whether the last batch ran fused levels}."""
+        m, u = C.c_int(), C.c_int()
+        check(lib().gb_batch_get_deep_config(self._h, C.byref(m), C.byref(u)))
+        return {"mode": m.value, "used": bool(u.value)}
+
     def batch_last_bottom_up(self):
         """Number of levels the last bfs_batch ran bottom-up."""
         v = C.c_int()

@@ -166,6 +166,8 @@ SIGNATURES = {
     "gb_batch_get_direction_config": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "gb_batch_last_bottom_up": (C.c_int, [P, C.POINTER(C.c_int)]),
     "gb_bfs_deep_config": (C.c_int, [P, C.c_int]),
+    "gb_batch_deep_config": (C.c_int, [P, C.c_int]),
+    "gb_batch_get_deep_config": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "gb_bfs_get_deep_config": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "gb_bfs_autotune": (C.c_int, [P, P, C.c_int, C.c_int, C.POINTER(TuneResult)]),
     "gb_batch_autotune": (C.c_int, [P, P, C.c_int, C.c_int, C.POINTER(TuneResult)]),

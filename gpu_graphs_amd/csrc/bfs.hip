@@ -1203,6 +1203,12 @@ const KernelSet* find_kernels(int block, int k) {
     return nullptr;
 }
 
+int min_tile_edges() {
+    int t = kKernels[0].block * kKernels[0].k;
+    for (const auto& ks : kKernels) t = std::min(t, ks.block * ks.k);
+    return t;
+}
+
 }  // namespace
 
 bool bfs_config_supported(int block, int k) { return find_kernels(block, k) != nullptr; }
@@ -1341,23 +1347,12 @@ void Runner::free_workspace() {
 }
 
 void Runner::ensure_workspace() {
-    const int T = g.cfg.block * g.cfg.edges_per_thread;
+    // The tile indices are sized for the smallest tile of any supported
+    // kernel shape, so a change of shape (the autotuner makes many) never
+    // reallocates a buffer; only the captured sequence is rebuilt.
+    const int T = min_tile_edges();
     const int64_t tile_cap = (m + T - 1) / T + 2;
-    if (ws.ctl && ws.tile_edges == T) return;
-    if (ws.ctl) {  // only the tile index depends on T (and the captured sequence uses it)
-        if (ws.graph_exec) (void)hipGraphExecDestroy(ws.graph_exec);
-        ws.graph_exec = nullptr;
-        (void)hipFree(ws.tile_start[0]);
-        (void)hipFree(ws.tile_start[1]);
-        ws.bytes -= 2 * sizeof(uint32_t) * ws.tile_cap;
-        ws.tile_start[0] = ws.tile_start[1] = nullptr;
-        GG_HIP(hipMalloc(&ws.tile_start[0], sizeof(uint32_t) * tile_cap));
-        GG_HIP(hipMalloc(&ws.tile_start[1], sizeof(uint32_t) * tile_cap));
-        ws.tile_cap = tile_cap;
-        ws.tile_edges = T;
-        ws.bytes += 2 * sizeof(uint32_t) * tile_cap;
-        return;
-    }
+    if (ws.ctl) return;
     const int64_t nwords = (n + 31) / 32;
     const int64_t nchunks = (n + kChunk - 1) / kChunk;
     size_t bytes = 0;

@@ -398,6 +398,22 @@ int gb_batch_last_bottom_up(const gb_dgraph* dg, int* levels) {
     });
 }
 
+int gb_batch_deep_config(gb_dgraph* dg, int mode) {
+    return guarded([&] {
+        need(dg, "null device graph");
+        need(mode == GB_DEEP_AUTO || mode == GB_DEEP_ON || mode == GB_DEEP_OFF, "unknown deep mode");
+        dg->d->ms_cfg.deep = mode;
+    });
+}
+
+int gb_batch_get_deep_config(const gb_dgraph* dg, int* mode, int* used) {
+    return guarded([&] {
+        need(dg, "null device graph");
+        if (mode) *mode = dg->d->ms_cfg.deep;
+        if (used) *used = dg->d->runner->ms.last_deep ? 1 : 0;
+    });
+}
+
 int gb_bfs_deep_config(gb_dgraph* dg, int mode) {
     return guarded([&] {
         need(dg, "null device graph");

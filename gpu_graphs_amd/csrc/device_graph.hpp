@@ -95,6 +95,7 @@ struct MsConfig {
     bool direction_opt = true;  // bottom-up levels on symmetric graphs
     int alpha = 2;              // bottom-up when frontier edges * alpha > edges not yet fully seen
     int low_degree = 32;        // bottom-up: rows up to this long are scanned by one lane
+    int deep = GB_DEEP_AUTO;    // one fused launch per level for deep batches with thin frontiers
 };
 
 // Per-batch parameters, read by the kernels from device memory so that one
@@ -121,6 +122,8 @@ struct MsWorkspace {
     unsigned long long* q_fm = nullptr;   // their frontier masks
     uint32_t* tile_start = nullptr;
     int32_t* touched = nullptr;           // n: vertices first reached in a sparse level
+    int32_t* touched2 = nullptr;          // n: deep batches: the list of odd levels (touched: even)
+    unsigned long long* next1 = nullptr;  // n: deep batches: next of odd levels (even: sn's next)
     unsigned long long* ctl = nullptr;
     MsParams* params = nullptr;           // device
     MsParams* h_params = nullptr;         // pinned (read by the captured copy)
@@ -136,11 +139,14 @@ struct MsWorkspace {
     int short_streak = 0;                 // batches in a row shallower than depth_hint
     int last_bu_levels = 0;               // levels the last batch ran bottom-up
     bool tail_clean = false;              // vertices without edges hold no state but prev_sources'
+    bool deep_learned = false;            // the last batch was deep with thin frontiers: fused levels next
+    bool last_deep = false;               // the last batch ran fused levels
     int32_t prev_sources[64] = {};        // sources of the last batch that completed
     int prev_k = 0;
     size_t bytes = 0;
     hipGraphExec_t graph_exec = nullptr;  // captured batch sequence
-    int graph_depth = 0, graph_block = 0, graph_kpt = 0, graph_bu = -1, graph_alpha = 0, graph_low = 0;
+    int graph_depth = 0, graph_block = 0, graph_kpt = 0, graph_bu = -1, graph_alpha = 0, graph_low = 0,
+        graph_deep = -1;
     int64_t graph_sparse = -1;
 };
 

@@ -67,7 +67,10 @@ enum : int {
     MS_FULLDEG = 9,    // CSR entries of the vertices every source has reached
     MS_BULEVELS = 10,  // levels run bottom-up
     MS_ACT0 = 11,      // act[L & 1]: sources with a non-empty level-L frontier
-    MS_WORDS = 16
+    MS_DCNT0 = 13,     // 3 words, fused (deep) levels: length of level L's touched list at dcnt[L % 3]
+    MS_MAXF = 16,      // fused levels: longest touched list
+    MS_MAXE = 17,      // largest frontier (CSR entries) of a top-down level
+    MS_WORDS = 20
 };
 enum : unsigned long long { MS_ERR_TILE = 1, MS_ERR_RANGE = 2 };
 
@@ -111,6 +114,8 @@ struct MsArgs {
     unsigned long long* q_fm;    // frontier mask of each queue entry
     uint32_t* tile_start;
     int32_t* touched;
+    int32_t* touched2;           // fused levels: touched list of odd levels (even ones use touched)
+    unsigned long long* next1;   // fused levels: next of odd levels (even ones use sn's next)
     unsigned long long* ctl;
     int64_t n;
     int64_t m;
@@ -141,6 +146,7 @@ __global__ void ms_clear_kernel(MsArgs a) {
     for (int64_t i = tid0; i < end; i += stride) {
         a.sn[i].seen = 0;
         a.sn[i].next = 0;
+        a.next1[i] = 0;
     }
     for (int64_t i = tid0; i < (a.n + 63) / 64; i += stride) a.reached[i] = 0;
 }
@@ -207,7 +213,11 @@ __global__ void ms_seed_kernel(MsArgs a) {
         const unsigned long long old = atomicOr(&a.sn[s].next, 1ull << threadIdx.x);
         if (old == 0) a.touched[atomicAdd(&a.ctl[MS_TCNT0], 1ull)] = s;
     }
-    if (threadIdx.x == 0) a.ctl[MS_TVALID0] = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a.ctl[MS_TVALID0] = 1;
+        a.ctl[MS_DCNT0] = a.ctl[MS_TCNT0];  // the list of level 0 for fused levels
+    }
 }
 
 // Stores level L for the new bits of this lane's vertex: the wave walks the
@@ -596,6 +606,7 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(MsArgs a, int L) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.ctl[MS_TVALID0 + npar] = (sparse && !bottom_up) ? 1 : 0;
         if (bottom_up) atomicAdd(&a.ctl[MS_BULEVELS], 1ull);
+        else atomicMax(&a.ctl[MS_MAXE], static_cast<unsigned long long>(E));
     }
     if (bottom_up) {
         // Only sources with a non-empty level-L frontier can reach anything
@@ -714,6 +725,187 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(MsArgs a, int L) {
         }
         __syncthreads();
     }
+}
+
+// Deep, thin batches (road-like graphs: thousands of levels of a few
+// thousand vertices per source).  There a level's cost is its chain of
+// dependent memory round trips, so the update and the expansion of a level
+// run fused in one launch: a thread takes one vertex of the level's touched
+// list (loaded beside the list's length), takes its new bits (next & ~seen),
+// writes its level bytes, and pushes the new bits along its row -- a probe
+// of the target's seen / next, an atomicOr whose old value tells whether the
+// target was untouched, and one reservation per wave to append the newly
+// touched targets to the next list.  Rows longer than kMsDeepRow go to the
+// wave, 64 entries per step.  `next` and the touched list are double-buffered
+// by level parity (even levels read sn's next and the touched list, odd ones
+// next1 and touched2, each writing the other), so the bits and list of level
+// L+1 never meet those being consumed; list lengths are triple-buffered
+// (dcnt[L % 3]: level L reads it, writes dcnt[(L+1) % 3] and zeroes
+// dcnt[(L+2) % 3], last read by level L-1).  A level whose list is empty ends
+// the batch.  No frontier masks or bottom-up state are kept: a fused batch
+// runs top-down only.
+constexpr int kMsDeepBlock = 256;
+constexpr int kMsDeepRow = 8;
+constexpr int kMsDeepDepth = 64;  // levels of a batch from which fused levels are learned
+
+__global__ __launch_bounds__(kMsDeepBlock) void ms_deep_kernel(MsArgs a, int L) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const bool odd = (L & 1) != 0;
+    const int32_t* const tin = odd ? a.touched2 : a.touched;
+    int32_t* const tout = odd ? a.touched : a.touched2;
+    const uint64_t n = static_cast<uint64_t>(a.n);
+    const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * kMsDeepBlock + threadIdx.x;
+    const uint64_t gstride = static_cast<uint64_t>(gridDim.x) * kMsDeepBlock;
+    const int32_t v_first = gtid < n ? tin[gtid] : -1;  // used only below the list's length
+    const unsigned long long stop = a.ctl[MS_DONE] | a.ctl[MS_ERR];
+    const unsigned long long F64 = a.ctl[MS_DCNT0 + L % 3];
+    if (stop) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.ctl[MS_DCNT0 + (L + 2) % 3] = 0;
+        if (F64 == 0) a.ctl[MS_DONE] = 1;
+        else atomicMax(&a.ctl[MS_MAXF], F64);
+    }
+    if (F64 == 0) return;
+    if (F64 > n) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.ctl[MS_ERR], MS_ERR_RANGE);
+        return;
+    }
+    const uint32_t F = static_cast<uint32_t>(F64);
+    int32_t* const levels = a.p->levels;
+    unsigned long long* const cnt_next = &a.ctl[MS_DCNT0 + (L + 1) % 3];
+    const unsigned long long vb = static_cast<unsigned long long>(L < kByteLevels ? L : kByteLevels) *
+                                  0x0101010101010101ull;
+    bool bad = false, anynew = false;
+    // next of level L+1 (the words this level ORs into)
+    auto next_out = [&](int64_t w) -> unsigned long long* { return odd ? &a.sn[w].next : &a.next1[w]; };
+    // Reserves c list slots for each lane (wave-uniform call, one atomicAdd
+    // per wave); the lane's first slot, or ~0u if the list would overflow.
+    auto reserve = [&](uint32_t c) -> uint32_t {
+        const uint32_t incl = wave_inclusive_scan(c);
+        const uint32_t tot = __shfl(incl, 63, 64);
+        if (tot == 0) return 0u;
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(cnt_next, static_cast<unsigned long long>(tot));
+        base = __shfl(base, 0, 64);
+        if (base + tot > n) {
+            bad = true;
+            return ~0u;
+        }
+        return static_cast<uint32_t>(base) + incl - c;
+    };
+    for (uint64_t i = gtid;; i += gstride) {
+        const bool has = i < F;
+        if (!wave_ballot(has)) break;  // the lanes of a wave hold consecutive entries
+        int64_t v = -1;
+        if (has) {
+            const int32_t t = i == gtid ? v_first : tin[i];
+            if (static_cast<uint64_t>(static_cast<uint32_t>(t)) >= n) bad = true;
+            else v = t;
+        }
+        // The vertex's state and row bounds.
+        unsigned long long sv = 0, nx = 0;
+        int64_t rb = 0, re = 0;
+        if (v >= 0) {
+            const ulonglong2 st = *reinterpret_cast<const ulonglong2*>(a.sn + v);
+            sv = st.x;
+            nx = odd ? a.next1[v] : st.y;
+            rb = a.offsets[v];
+            re = a.offsets[v + 1];
+        }
+        const unsigned long long nv = nx & ~sv;
+        const bool first = nv && !sv;
+        const uint32_t deg = static_cast<uint32_t>(re - rb);
+        const bool short_row = nv && deg <= kMsDeepRow;
+        // Its line (when an earlier level wrote it) and, for a short row, its
+        // columns and their targets' state.
+        ulonglong2 line[4];
+        const ulonglong2* src = reinterpret_cast<const ulonglong2*>(a.lv8 + (v >= 0 ? v : 0) * 8);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) line[j] = (nv && !first) ? src[j] : make_ulonglong2(~0ull, ~0ull);
+        int32_t u[kMsDeepRow];
+#pragma unroll
+        for (int j = 0; j < kMsDeepRow; ++j) {
+            u[j] = (short_row && static_cast<uint32_t>(j) < deg) ? a.cols[rb + j] : -1;
+            if (u[j] >= 0 && static_cast<uint64_t>(u[j]) >= n) {
+                bad = true;
+                u[j] = -1;
+            }
+        }
+        unsigned long long mk[kMsDeepRow];
+#pragma unroll
+        for (int j = 0; j < kMsDeepRow; ++j) {
+            mk[j] = 0;
+            if (u[j] >= 0) {
+                const ulonglong2 t = *reinterpret_cast<const ulonglong2*>(a.sn + u[j]);
+                mk[j] = nv & ~t.x & ~(odd ? t.y : a.next1[u[j]]);
+            }
+        }
+        // The vertex's own state: only words no other thread of this level
+        // writes (its seen and its next of level L).
+        if (v >= 0 && nx) {
+            if (odd) {
+                a.sn[v].seen = sv | nv;
+                a.next1[v] = 0;
+            } else {
+                *reinterpret_cast<ulonglong2*>(a.sn + v) = make_ulonglong2(sv | nv, 0ull);
+            }
+        }
+        if (first) atomicOr(&a.reached[v >> 6], 1ull << (v & 63));
+        if (L >= kByteLevels) {
+            // every lane of the wave takes part in the shuffles of write_levels
+            if (__ballot(nv != 0)) write_levels(levels, a.n, nv ? a.iperm[v] : 0, nv, L);
+        }
+        if (nv) {
+            anynew = true;
+            ulonglong2* dst = reinterpret_cast<ulonglong2*>(a.lv8 + v * 8);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                dst[j] = make_ulonglong2(merge_level_word(line[j].x, (nv >> (16 * j)) & 0xFFull, vb),
+                                         merge_level_word(line[j].y, (nv >> (16 * j + 8)) & 0xFFull, vb));
+        }
+        // Pushes along a short row; targets it turns from untouched to
+        // touched join the next list.
+        uint32_t claim = 0;
+#pragma unroll
+        for (int j = 0; j < kMsDeepRow; ++j)
+            if (mk[j] && __hip_atomic_fetch_or(next_out(u[j]), mk[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+                claim |= 1u << j;
+        const uint32_t slot0 = reserve(static_cast<uint32_t>(__popc(claim)));
+        if (claim && slot0 != ~0u) {
+            uint32_t r = 0;
+#pragma unroll
+            for (int j = 0; j < kMsDeepRow; ++j)
+                if ((claim >> j) & 1u) tout[slot0 + r++] = u[j];
+        }
+        // Long rows, one at a time by the whole wave.
+        uint64_t lm = wave_ballot(nv && deg > kMsDeepRow);
+        while (lm) {
+            const int hl = __ffsll(static_cast<long long>(lm)) - 1;
+            lm &= lm - 1;
+            const int64_t hrb = __shfl(rb, hl, 64), hre = __shfl(re, hl, 64);
+            const unsigned long long hnv = __shfl(nv, hl, 64);
+            for (int64_t b = hrb; b < hre; b += 64) {
+                int32_t w = b + lane < hre ? a.cols[b + lane] : -1;
+                if (w >= 0 && static_cast<uint64_t>(w) >= n) {
+                    bad = true;
+                    w = -1;
+                }
+                unsigned long long m = 0;
+                if (w >= 0) {
+                    const ulonglong2 t = *reinterpret_cast<const ulonglong2*>(a.sn + w);
+                    m = hnv & ~t.x & ~(odd ? t.y : a.next1[w]);
+                }
+                const bool c = m && __hip_atomic_fetch_or(next_out(w), m, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT) == 0;
+                const uint32_t slot = reserve(c ? 1u : 0u);
+                if (c && slot != ~0u) tout[slot] = w;
+            }
+        }
+    }
+    // Every writer of this level stores the same L and levels run in order,
+    // so the final value is the deepest level reached.
+    if (wave_ballot(anynew) && lane == 0) a.ctl[MS_MAXLEV] = static_cast<unsigned long long>(L);
+    if (bad) atomicOr(&a.ctl[MS_ERR], MS_ERR_RANGE);
 }
 
 using MsExpandFn = void (*)(MsArgs, int);
@@ -920,7 +1112,8 @@ void build_ms_graph(DeviceGraph& dg, const HostGraph& g) {
 void Runner::free_ms_workspace() {
     if (ms.graph_exec) (void)hipGraphExecDestroy(ms.graph_exec);
     void* ptrs[] = {ms.sn,      ms.fmask,      ms.lv8,     ms.reached, ms.q_rb,   ms.q_eoff, ms.q_v,
-                    ms.q_fm,    ms.tile_start, ms.touched, ms.ctl,     ms.params, ms.levels, ms.partial};
+                    ms.q_fm,    ms.tile_start, ms.touched, ms.ctl,     ms.params, ms.levels, ms.partial,
+                    ms.touched2, ms.next1};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (ms.h_params) (void)hipHostFree(ms.h_params);
@@ -929,21 +1122,12 @@ void Runner::free_ms_workspace() {
 }
 
 void Runner::ensure_ms_workspace() {
-    const int T = g.ms_cfg.block * g.ms_cfg.edges_per_thread;
+    // Sized for the smallest tile of any supported shape: a change of shape
+    // only rebuilds the captured sequence (its key includes the shape).
+    int T = kMsKernels[0].block * kMsKernels[0].k;
+    for (const auto& ks : kMsKernels) T = std::min(T, ks.block * ks.k);
     const int64_t tile_cap = (m + T - 1) / T + 2;
-    if (ms.ctl && ms.tile_edges == T) return;
-    if (ms.ctl) {
-        (void)hipFree(ms.tile_start);
-        ms.bytes -= sizeof(uint32_t) * ms.tile_cap;
-        ms.tile_start = nullptr;
-        GG_HIP(hipMalloc(&ms.tile_start, sizeof(uint32_t) * tile_cap));
-        ms.bytes += sizeof(uint32_t) * tile_cap;
-        ms.tile_cap = tile_cap;
-        ms.tile_edges = T;
-        if (ms.graph_exec) (void)hipGraphExecDestroy(ms.graph_exec);
-        ms.graph_exec = nullptr;
-        return;
-    }
+    if (ms.ctl) return;
     const int64_t nchunks = (n + kUcChunk - 1) / kUcChunk;
     size_t bytes = 0;
     auto alloc = [&](auto** p, size_t sz) {
@@ -961,6 +1145,8 @@ void Runner::ensure_ms_workspace() {
         alloc(&ms.q_fm, sizeof(unsigned long long) * n);
         alloc(&ms.tile_start, sizeof(uint32_t) * tile_cap);
         alloc(&ms.touched, sizeof(int32_t) * n);
+        alloc(&ms.touched2, sizeof(int32_t) * n);
+        alloc(&ms.next1, sizeof(unsigned long long) * n);
         alloc(&ms.ctl, sizeof(unsigned long long) * kCtlWords);
         alloc(&ms.params, sizeof(MsParams));
         alloc(&ms.partial, sizeof(unsigned long long) * kStatsBlocks * 64 * 2);
@@ -983,7 +1169,7 @@ void Runner::prepare_batch() {
 }
 
 size_t batch_bytes_estimate(int64_t n_vertices, int64_t m_entries, int k, bool host_levels) {
-    size_t b = sizeof(unsigned long long) * n_vertices * 12 + (sizeof(int64_t) + 12) * n_vertices +
+    size_t b = sizeof(unsigned long long) * n_vertices * 13 + (sizeof(int64_t) + 16) * n_vertices +
                sizeof(unsigned long long) * ((n_vertices + 63) / 64) +
                sizeof(uint32_t) * (m_entries / 256 + 2) + 1024 * 64 * 16 + 4096;
     if (host_levels) b += sizeof(int32_t) * n_vertices * k;
@@ -1024,6 +1210,8 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     a.q_fm = ms.q_fm;
     a.tile_start = ms.tile_start;
     a.touched = ms.touched;
+    a.touched2 = ms.touched2;
+    a.next1 = ms.next1;
     a.ctl = ms.ctl;
     a.n = n;
     a.m = m;
@@ -1070,10 +1258,19 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
         hipLaunchKernelGGL(ms_clear_kernel, dim3(cgrid), dim3(256), 0, stream, a);
         hipLaunchKernelGGL(ms_seed_kernel, dim3(1), dim3(64), 0, stream, a);
     };
+    // Fused levels (ms_deep_kernel) for batches on graphs whose last batch
+    // was deep (>= kMsDeepDepth levels) with thin, top-down frontiers only.
+    const bool deep = g.ms_cfg.deep == GB_DEEP_ON || (g.ms_cfg.deep == GB_DEEP_AUTO && ms.deep_learned);
+    const uint32_t dgrid = static_cast<uint32_t>(
+        std::max<int64_t>(1, std::min<int64_t>((n + kMsDeepBlock - 1) / kMsDeepBlock, int64_t(g.num_cus) * 2)));
     int L = 0;
     auto launch_level = [&]() {
-        hipLaunchKernelGGL(ms_update_compact_kernel, dim3(ugrid), dim3(kUcBlock), 0, stream, a, L);
-        hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(g.ms_cfg.block), 0, stream, a, L);
+        if (deep) {
+            hipLaunchKernelGGL(ms_deep_kernel, dim3(dgrid), dim3(kMsDeepBlock), 0, stream, a, L);
+        } else {
+            hipLaunchKernelGGL(ms_update_compact_kernel, dim3(ugrid), dim3(kUcBlock), 0, stream, a, L);
+            hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(g.ms_cfg.block), 0, stream, a, L);
+        }
         ++L;
     };
 
@@ -1104,7 +1301,7 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
         const bool reuse = ms.graph_exec && ms.graph_depth == depth_hint && ms.graph_block == g.ms_cfg.block &&
                            ms.graph_kpt == g.ms_cfg.edges_per_thread && ms.graph_sparse == a.sparse_edges &&
                            ms.graph_bu == int(a.bottom_up) && ms.graph_alpha == int(a.alpha) &&
-                           ms.graph_low == int(a.low_degree);
+                           ms.graph_low == int(a.low_degree) && ms.graph_deep == int(deep);
         if (!reuse) {
             if (ms.graph_exec) (void)hipGraphExecDestroy(ms.graph_exec);
             ms.graph_exec = nullptr;
@@ -1129,6 +1326,7 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
             ms.graph_bu = int(a.bottom_up);
             ms.graph_alpha = int(a.alpha);
             ms.graph_low = int(a.low_degree);
+            ms.graph_deep = int(deep);
             GG_HIP(hipEventRecord(ev_start, stream));
         }
         GG_HIP(hipGraphLaunch(ms.graph_exec, stream));
@@ -1186,6 +1384,12 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     ms.prev_k = k;
     ms.tail_clean = true;
     ms.last_depth = static_cast<int>(fin[MS_MAXLEV]) + 1;
+    // Fused levels next when this batch was deep and thin: no bottom-up or
+    // dense level (or, fused, no list longer than a sparse level's edges / 2).
+    ms.last_deep = deep;
+    ms.deep_learned = deep ? fin[MS_MAXF] * 2 <= static_cast<unsigned long long>(a.sparse_edges)
+                           : ms.last_depth >= kMsDeepDepth && fin[MS_BULEVELS] == 0 &&
+                                 fin[MS_MAXE] <= static_cast<unsigned long long>(a.sparse_edges);
     ms.last_bu_levels = static_cast<int>(fin[MS_BULEVELS]);
     // Levels queued for the next batch: this depth + 1 (an empty level ends a
     // batch), raised at once and lowered only after 16 shallower batches in a

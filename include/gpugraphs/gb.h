@@ -175,9 +175,10 @@ int  gb_bfs_get_config(const gb_dgraph* dg, int* block, int* edges_per_thread,
 int  gb_bfs_direction_config(gb_dgraph* dg, int enable, int alpha, int beta);
 int  gb_bfs_get_direction_config(const gb_dgraph* dg, int* enable, int* alpha, int* beta, int* symmetric);
 /* Deep mode of gb_bfs (no reference counterpart; the reference pays a host
- * round trip per level, gpuBFS.cu:66-100): one kernel launch per level
- * instead of two -- every level is expanded top-down straight into the next
- * queue, and the level's counters are kept by the expansion itself.  GB_DEEP_AUTO
+ * round trip per level, gpuBFS.cu:66-100): one launch per level of a kernel
+ * built for thin frontiers -- a thread per frontier vertex, claims straight
+ * into the visited bitmap, one queue reservation per wave, top-down only --
+ * and the level's counters are kept by that kernel.  GB_DEEP_AUTO
  * (default) uses it once searches of the graph have been deep (>= 64 levels)
  * with thin frontiers only, such as road networks; GB_DEEP_ON always;
  * GB_DEEP_OFF never.  *used (nullable) tells whether the last gb_bfs ran in
@@ -228,6 +229,13 @@ int  gb_batch_direction_config(gb_dgraph* dg, int enable, int alpha);
 int  gb_batch_get_direction_config(const gb_dgraph* dg, int* enable, int* alpha);
 /* Number of levels the last gb_bfs_batch on this handle ran bottom-up. */
 int  gb_batch_last_bottom_up(const gb_dgraph* dg, int* levels);
+/* Deep mode of gb_bfs_batch: one fused launch per level (each vertex of the
+ * level's list takes its new bits and pushes them along its row), top-down
+ * only.  GB_DEEP_AUTO (default) uses it once a batch on the graph has been
+ * deep (>= 64 levels) with thin frontiers only; GB_DEEP_ON always;
+ * GB_DEEP_OFF never.  *used (nullable): whether the last batch used it. */
+int  gb_batch_deep_config(gb_dgraph* dg, int mode);
+int  gb_batch_get_deep_config(const gb_dgraph* dg, int* mode, int* used);
 
 /* ---- autotuner (no reference counterpart: the reference's kernel shape is
  *      the compile-time BLOCK_SIZE 1024, 1D Partition/include/

@@ -322,6 +322,57 @@ def test_edgeless_sources_across_batches(gpu, use_graph):
         check_batch(G, dg, sources)
 
 
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_fused_deep_batches(gpu, use_graph):
+    # Fused levels (one launch per level, top-down only), forced on: a deep
+    # grid (levels far past the 254 kept in bytes), an R-MAT graph (wide
+    # levels, hub rows pushed by whole waves), a directed path, and graphs
+    # with edgeless and repeated sources, batch after batch on one handle.
+    cases = []
+    Gg = gg.Graph.grid(150, 120, 0.05, seed=95)
+    cases.append(("grid", Gg, [np.array([0, 7, Gg.n - 1, 9000], dtype=np.int32),
+                               np.arange(0, Gg.n, Gg.n // 64, dtype=np.int32)[:64],
+                               np.array([5, 5, 17000, 5], dtype=np.int32)]))
+    R = gg.Graph.rmat(13, seed=7)
+    deg = np.diff(R.offsets)
+    iso = np.nonzero(deg == 0)[0][:2].astype(np.int32)
+    hub = np.array([np.argmax(deg)], dtype=np.int32)
+    nz = np.nonzero(deg)[0].astype(np.int32)
+    cases.append(("rmat", R, [np.concatenate([nz[:30], iso, hub]), nz[30:94], np.concatenate([iso, iso, nz[:3]])]))
+    n = 3000
+    src = np.arange(n - 1, dtype=np.int32)
+    D = gg.Graph.from_edges(n, src, src + 1)
+    cases.append(("directed path", D, [np.array([0, 1500, n - 1], dtype=np.int32), np.arange(0, n, 50, dtype=np.int32)]))
+    for name, G, batches in cases:
+        dg = gg.DeviceGraph(G, gpu)
+        dg.batch_config(use_graph=use_graph)
+        dg.batch_deep_config(gg.GB_DEEP_ON)
+        for i, sources in enumerate(batches):
+            log(f"fused batches: {name} batch {i} ({len(sources)} sources)")
+            check_batch(G, dg, sources)
+            assert dg.get_batch_deep_config() == {"mode": gg.GB_DEEP_ON, "used": True}
+        dg.batch_deep_config(gg.GB_DEEP_OFF)  # back to the two-launch levels on the same workspace
+        check_batch(G, dg, batches[0])
+        assert not dg.get_batch_deep_config()["used"]
+
+
+def test_fused_batches_learned_on_deep_graphs_only(gpu):
+    G = gg.Graph.grid(200, 150, 0.05, seed=96)
+    dg = gg.DeviceGraph(G, gpu)
+    used = []
+    for i in range(3):
+        sources = np.arange(i, G.n, G.n // 16, dtype=np.int32)[:16]
+        check_batch(G, dg, sources)
+        used.append(dg.get_batch_deep_config()["used"])
+    assert used == [False, True, True], used
+    R = gg.Graph.rmat(14, seed=97)
+    dr = gg.DeviceGraph(R, gpu)
+    nz = np.nonzero(np.diff(R.offsets))[0].astype(np.int32)
+    for i in range(3):
+        check_batch(R, dr, nz[i * 20:(i + 1) * 20])
+        assert not dr.get_batch_deep_config()["used"]
+
+
 def test_levels_past_the_byte_range(gpu):
     # A 700-vertex path: levels go past 254, where the batch stores levels
     # into the rows directly instead of the byte lines.
