@@ -422,7 +422,7 @@ def batched(gg, args):
     edges = sum(st["m_reached"][:len(sources)]) / 2
     bcfg, bdir = dg.get_batch_config(), dg.get_batch_direction_config()
     run_cfg = {"block": bcfg["block"], "edges_per_thread": bcfg["edges_per_thread"],
-               "sparse_edges": bcfg["sparse_edges"], "alpha": bdir["alpha"]}
+               "sparse_edges": bcfg["sparse_edges"], "alpha": bdir["alpha"], "beta": bdir["beta"]}
     prof, prof_note = traffic_profile("msbfs_rmat24_traffic.json", run_cfg)
     out = {
         "workload": "rmat-s24-ef16, 64 seeded sources in one multi-source BFS",

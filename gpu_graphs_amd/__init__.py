@@ -307,13 +307,14 @@ class DeviceGraph:
         return {"block": b.value, "edges_per_thread": k.value, "use_graph": bool(g.value),
                 "sparse_edges": s.value}
 
-    def batch_direction_config(self, enable=True, alpha=0):
-        check(lib().gb_batch_direction_config(self._h, int(enable), int(alpha)))
+    def batch_direction_config(self, enable=True, alpha=0, beta=-1):
+        """Bottom-up levels of bfs_batch; alpha 0 and beta -1 keep the current values."""
+        check(lib().gb_batch_direction_config(self._h, int(enable), int(alpha), int(beta)))
 
     def get_batch_direction_config(self):
-        e, a = C.c_int(), C.c_int()
-        check(lib().gb_batch_get_direction_config(self._h, C.byref(e), C.byref(a)))
-        return {"enable": bool(e.value), "alpha": a.value}
+        e, a, b = C.c_int(), C.c_int(), C.c_int()
+        check(lib().gb_batch_get_direction_config(self._h, C.byref(e), C.byref(a), C.byref(b)))
+        return {"enable": bool(e.value), "alpha": a.value, "beta": b.value}
 
     def batch_deep_config(self, mode=GB_DEEP_AUTO):
         """Fused launches per level for deep batches with thin frontiers (GB_DEEP_*)."""

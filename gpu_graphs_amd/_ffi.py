@@ -162,8 +162,8 @@ SIGNATURES = {
     "gb_batch_config": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int64]),
     "gb_batch_get_config": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int64)]),
-    "gb_batch_direction_config": (C.c_int, [P, C.c_int, C.c_int]),
-    "gb_batch_get_direction_config": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "gb_batch_direction_config": (C.c_int, [P, C.c_int, C.c_int, C.c_int]),
+    "gb_batch_get_direction_config": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "gb_batch_last_bottom_up": (C.c_int, [P, C.POINTER(C.c_int)]),
     "gb_bfs_deep_config": (C.c_int, [P, C.c_int]),
     "gb_batch_deep_config": (C.c_int, [P, C.c_int]),

@@ -150,6 +150,7 @@ void autotune_batch(DeviceGraph& g, const int32_t* sources, int k, int reps, gb_
                      [&]() { return shape(c.block, c.edges_per_thread); }});
     if (g.symmetric && c.direction_opt) {
         knobs.push_back({{1, 2, 4, 8}, [&](int64_t v) { c.alpha = int(v); }, [&]() { return int64_t(c.alpha); }});
+        knobs.push_back({{0, 8, 32, 128}, [&](int64_t v) { c.beta = int(v); }, [&]() { return int64_t(c.beta); }});
         knobs.push_back({{16, 32, 64}, [&](int64_t v) { c.low_degree = int(v); },
                          [&]() { return int64_t(c.low_degree); }});
     }
@@ -164,7 +165,7 @@ void autotune_batch(DeviceGraph& g, const int32_t* sources, int k, int reps, gb_
     res.block = c.block;
     res.edges_per_thread = c.edges_per_thread;
     res.alpha = c.alpha;
-    res.beta = 0;
+    res.beta = c.beta;
     res.low_degree = c.low_degree;
     res.frontier_edges = c.sparse_edges;
     if (out) *out = res;

@@ -374,20 +374,22 @@ int gb_batch_get_config(const gb_dgraph* dg, int* block, int* edges_per_thread, 
     });
 }
 
-int gb_batch_direction_config(gb_dgraph* dg, int enable, int alpha) {
+int gb_batch_direction_config(gb_dgraph* dg, int enable, int alpha, int beta) {
     return guarded([&] {
         need(dg, "null device graph");
-        need(alpha >= 0, "alpha must be >= 0");
+        need(alpha >= 0 && beta >= -1, "alpha must be >= 0 and beta >= -1");
         dg->d->ms_cfg.direction_opt = enable != 0;
         if (alpha) dg->d->ms_cfg.alpha = alpha;
+        if (beta >= 0) dg->d->ms_cfg.beta = beta;
     });
 }
 
-int gb_batch_get_direction_config(const gb_dgraph* dg, int* enable, int* alpha) {
+int gb_batch_get_direction_config(const gb_dgraph* dg, int* enable, int* alpha, int* beta) {
     return guarded([&] {
         need(dg, "null device graph");
         if (enable) *enable = dg->d->ms_cfg.direction_opt ? 1 : 0;
         if (alpha) *alpha = dg->d->ms_cfg.alpha;
+        if (beta) *beta = dg->d->ms_cfg.beta;
     });
 }
 
@@ -468,6 +470,7 @@ int gb_batch_apply_tuning(gb_dgraph* dg, const gb_tune_result* t) {
         c.block = t->block;
         c.edges_per_thread = t->edges_per_thread;
         c.alpha = t->alpha;
+        c.beta = t->beta;
         c.low_degree = t->low_degree;
         c.sparse_edges = t->frontier_edges;
     });

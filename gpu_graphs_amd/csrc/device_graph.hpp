@@ -94,6 +94,7 @@ struct MsConfig {
     int64_t sparse_edges = 0;   // levels with at most this many frontier edges pass a touched list
     bool direction_opt = true;  // bottom-up levels on symmetric graphs
     int alpha = 2;              // bottom-up when frontier edges * alpha > edges not yet fully seen
+    int beta = 0;               // ... and the frontier holds >= (vertices with edges) / beta vertices (0: any)
     int low_degree = 32;        // bottom-up: rows up to this long are scanned by one lane
     int deep = GB_DEEP_AUTO;    // one fused launch per level for deep batches with thin frontiers
 };
@@ -146,7 +147,7 @@ struct MsWorkspace {
     size_t bytes = 0;
     hipGraphExec_t graph_exec = nullptr;  // captured batch sequence
     int graph_depth = 0, graph_block = 0, graph_kpt = 0, graph_bu = -1, graph_alpha = 0, graph_low = 0,
-        graph_deep = -1;
+        graph_deep = -1, graph_beta = -1;
     int64_t graph_sparse = -1;
 };
 

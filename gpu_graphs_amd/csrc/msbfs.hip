@@ -125,6 +125,7 @@ struct MsArgs {
     uint32_t bottom_up;          // bottom-up levels allowed (symmetric graph)
     uint32_t alpha;
     uint32_t low_degree;         // bottom-up: longer rows are scanned by the whole wave
+    uint32_t beta;               // bottom-up only while F * beta >= nz (0: no such limit)
 };
 
 template <int BLOCK, int K>
@@ -602,7 +603,12 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(MsArgs a, int L) {
     // update_compact(L).  MS_FULLDEG is final (update_compact(L) has ended).
     const bool scanned = a.ctl[MS_TVALID0 + (L & 1)] == 0;
     const int64_t open_edges = a.m - static_cast<int64_t>(a.ctl[MS_FULLDEG]);
-    const bool bottom_up = a.bottom_up && scanned && static_cast<int64_t>(E) * a.alpha > open_edges;
+    // Beamer's second rule as well: once the frontier holds fewer than
+    // nz / beta vertices, a top-down level (which hands a list to the next
+    // update) is cheaper than a bottom-up one (after which the next update
+    // scans every vertex).
+    const bool bottom_up = a.bottom_up && scanned && static_cast<int64_t>(E) * a.alpha > open_edges &&
+                           (a.beta == 0 || static_cast<int64_t>(F) * a.beta >= a.nz);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.ctl[MS_TVALID0 + npar] = (sparse && !bottom_up) ? 1 : 0;
         if (bottom_up) atomicAdd(&a.ctl[MS_BULEVELS], 1ull);
@@ -1221,6 +1227,7 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     a.bottom_up = (g.symmetric && g.ms_cfg.direction_opt) ? 1u : 0u;
     a.alpha = static_cast<uint32_t>(g.ms_cfg.alpha);
     a.low_degree = static_cast<uint32_t>(g.ms_cfg.low_degree);
+    a.beta = static_cast<uint32_t>(g.ms_cfg.beta);
 
     int occ = 0;
     GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(ks->expand),
@@ -1301,7 +1308,8 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
         const bool reuse = ms.graph_exec && ms.graph_depth == depth_hint && ms.graph_block == g.ms_cfg.block &&
                            ms.graph_kpt == g.ms_cfg.edges_per_thread && ms.graph_sparse == a.sparse_edges &&
                            ms.graph_bu == int(a.bottom_up) && ms.graph_alpha == int(a.alpha) &&
-                           ms.graph_low == int(a.low_degree) && ms.graph_deep == int(deep);
+                           ms.graph_low == int(a.low_degree) && ms.graph_deep == int(deep) &&
+                           ms.graph_beta == int(a.beta);
         if (!reuse) {
             if (ms.graph_exec) (void)hipGraphExecDestroy(ms.graph_exec);
             ms.graph_exec = nullptr;
@@ -1327,6 +1335,7 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
             ms.graph_alpha = int(a.alpha);
             ms.graph_low = int(a.low_degree);
             ms.graph_deep = int(deep);
+            ms.graph_beta = int(a.beta);
             GG_HIP(hipEventRecord(ev_start, stream));
         }
         GG_HIP(hipGraphLaunch(ms.graph_exec, stream));

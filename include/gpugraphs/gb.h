@@ -221,12 +221,14 @@ int  gb_bfs_batch_device(gb_dgraph* dg, const int32_t* sources, int k, int32_t* 
 int  gb_batch_config(gb_dgraph* dg, int block, int edges_per_thread, int use_graph, int64_t sparse_edges);
 int  gb_batch_get_config(const gb_dgraph* dg, int* block, int* edges_per_thread, int* use_graph,
                          int64_t* sparse_edges);
-/* Bottom-up levels of gb_bfs_batch (symmetric graphs only): enable (1/0) and
+/* Bottom-up levels of gb_bfs_batch (symmetric graphs only): enable (1/0);
  * alpha: a level goes bottom-up when its frontier edges * alpha exceed the
  * edges of the vertices not yet reached by every source (default 2; 0 keeps
- * the current value). */
-int  gb_batch_direction_config(gb_dgraph* dg, int enable, int alpha);
-int  gb_batch_get_direction_config(const gb_dgraph* dg, int* enable, int* alpha);
+ * the current value); beta: and only while the frontier holds at least
+ * (vertices with edges) / beta vertices (0: no such limit, the default; -1
+ * keeps the current value). */
+int  gb_batch_direction_config(gb_dgraph* dg, int enable, int alpha, int beta);
+int  gb_batch_get_direction_config(const gb_dgraph* dg, int* enable, int* alpha, int* beta);
 /* Number of levels the last gb_bfs_batch on this handle ran bottom-up. */
 int  gb_batch_last_bottom_up(const gb_dgraph* dg, int* levels);
 /* Deep mode of gb_bfs_batch: one fused launch per level (each vertex of the

@@ -183,7 +183,7 @@ def test_bottom_up_batches(gpu, enable, alpha):
     G = gg.Graph.rmat(16, seed=81)
     dg = gg.DeviceGraph(G, gpu)
     dg.batch_direction_config(enable, alpha)
-    assert dg.get_batch_direction_config() == {"enable": enable, "alpha": alpha}
+    assert dg.get_batch_direction_config() == {"enable": enable, "alpha": alpha, "beta": 0}
     deg = np.diff(G.offsets)
     rng = np.random.default_rng(alpha)
     for k in (64, 7):
