@@ -308,6 +308,8 @@ def single_source(gg, args):
                              "min": round(min(per_gteps), 4), "median": round(float(np.median(per_gteps)), 4)},
         "roofline_bfs_fraction": round(t_roof / dev_s, 4),
         "last_bottom_up_levels": list(dg.last_bottom_up()),
+        "launch_hints": dict(zip(("depth", "bu_lo", "bu_hi"), dg.launch_hints())),
+        "tail": dg.get_tail_config(),
         "levels_to_host": {"d2h_ms": round(d2h_ms, 4), "bytes": 4 * n,
                            "gbs": round(4 * n / (d2h_ms / 1e3) / 1e9, 2),
                            "gteps_incl_d2h": round(edges / (dev_s + nb * d2h_ms / 1e3) / 1e9, 4)},

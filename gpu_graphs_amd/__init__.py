@@ -23,7 +23,8 @@ import numpy as np
 from . import _ffi
 from ._ffi import (GB_AUTO_N, GB_DROP_SELF_LOOPS, GB_EAGAIN, GB_EBUSY, GB_EDEVICE, GB_EINVAL, GB_EIO, GB_ENOMEM,
                    GB_ERANGE, GB_KEEP_ORDER, GB_MODE_ASYNC, GB_MODE_GRAPH, GB_MODE_LEVEL_LOOP, GB_OK, GB_ONE_BASED, GB_SYMMETRIZE,
-                   GB_DEEP_AUTO, GB_DEEP_OFF, GB_DEEP_ON, BatchStats, GBError, Stats, check, lib)
+                   GB_DEEP_AUTO, GB_DEEP_OFF, GB_DEEP_ON, GB_TAIL_AUTO, GB_TAIL_DECLINED, GB_TAIL_FINISHED,
+                   GB_TAIL_HANDED_BACK, GB_TAIL_OFF, BatchStats, GBError, Stats, check, lib)
 
 GB_DEGREE_BUCKETS = 34
 
@@ -31,7 +32,8 @@ __all__ = [
     "Graph", "DeviceGraph", "DeviceArray", "Server", "csr", "gpuBFS", "GBError", "Stats", "device_count",
     "exclusive_scan", "GB_AUTO_N", "GB_ONE_BASED", "GB_SYMMETRIZE", "GB_KEEP_ORDER", "GB_DROP_SELF_LOOPS",
     "GB_MODE_ASYNC", "GB_MODE_LEVEL_LOOP", "GB_MODE_GRAPH", "GB_OK", "GB_EINVAL", "GB_EIO", "GB_ENOMEM", "GB_EDEVICE", "GB_ERANGE",
-    "GB_EBUSY", "GB_EAGAIN", "GB_DEEP_AUTO", "GB_DEEP_ON", "GB_DEEP_OFF",
+    "GB_EBUSY", "GB_EAGAIN", "GB_DEEP_AUTO", "GB_DEEP_ON", "GB_DEEP_OFF", "GB_TAIL_AUTO", "GB_TAIL_OFF",
+    "GB_TAIL_FINISHED", "GB_TAIL_DECLINED", "GB_TAIL_HANDED_BACK",
 ]
 
 
@@ -264,6 +266,18 @@ class DeviceGraph:
         m, u = C.c_int(), C.c_int()
         check(lib().gb_bfs_get_deep_config(self._h, C.byref(m), C.byref(u)))
         return {"mode": m.value, "used": bool(u.value)}
+
+    def tail_config(self, level=GB_TAIL_AUTO):
+        """Level from which one workgroup runs the rest of a search: GB_TAIL_AUTO
+        (learned), GB_TAIL_OFF, or a level >= 0."""
+        check(lib().gb_bfs_tail_config(self._h, int(level)))
+
+    def get_tail_config(self):
+        """{"level": the setting, "used": level the last bfs started its tail
+        at (-1: none), "outcome": GB_TAIL_FINISHED / DECLINED / HANDED_BACK, 0 if none}."""
+        lv, u, o = C.c_int(), C.c_int(), C.c_int()
+        check(lib().gb_bfs_get_tail_config(self._h, C.byref(lv), C.byref(u), C.byref(o)))
+        return {"level": lv.value, "used": u.value, "outcome": o.value}
 
     def launch_hints(self):
         """(depth, bu_lo, bu_hi): levels queued before the first done check and

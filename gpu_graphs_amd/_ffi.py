@@ -31,6 +31,11 @@ GB_MODE_GRAPH = 2
 GB_DEEP_AUTO = 0
 GB_DEEP_ON = 1
 GB_DEEP_OFF = 2
+GB_TAIL_AUTO = -1
+GB_TAIL_OFF = -2
+GB_TAIL_FINISHED = 1
+GB_TAIL_DECLINED = 2
+GB_TAIL_HANDED_BACK = 3
 
 _NAMES = {
     GB_EINVAL: "GB_EINVAL", GB_EIO: "GB_EIO", GB_ENOMEM: "GB_ENOMEM", GB_EDEVICE: "GB_EDEVICE",
@@ -169,6 +174,8 @@ SIGNATURES = {
     "gb_batch_deep_config": (C.c_int, [P, C.c_int]),
     "gb_batch_get_deep_config": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "gb_bfs_get_deep_config": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "gb_bfs_tail_config": (C.c_int, [P, C.c_int]),
+    "gb_bfs_get_tail_config": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "gb_bfs_autotune": (C.c_int, [P, P, C.c_int, C.c_int, C.POINTER(TuneResult)]),
     "gb_batch_autotune": (C.c_int, [P, P, C.c_int, C.c_int, C.POINTER(TuneResult)]),
     "gb_bfs_apply_tuning": (C.c_int, [P, C.POINTER(TuneResult)]),

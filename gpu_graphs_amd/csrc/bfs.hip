@@ -94,6 +94,8 @@ enum : int {
     CTL_FCNT0 = 22,    // 2 words, one-launch levels: (count << 32 | edges) of level L at fcnt[L & 1]
     CTL_RCNT0 = 24,    // 2 words, one-launch levels: queued vertices of levels <= L at rcnt[L & 1]
     CTL_DEEPE0 = 26,   // 2 words, deep levels: CSR entries of level L's frontier at deepe[L & 1]
+    CTL_TAIL = 28,     // what the tail kernel did: GB_TAIL_* outcome | (level << 8)
+    CTL_TAILFROM = 29, // first level of the current run of small queued levels (~0: none)
     CTL_WORDS = 32
 };
 enum : unsigned long long { ERR_TILE = 1, ERR_RANGE = 2 };
@@ -108,6 +110,12 @@ constexpr int kBuFirst = 4;                                 // row entries each 
 static_assert(kBuFirst == 4, "the head array holds one int4 per vertex");
 constexpr int kMaxCapturedLevels = 32;                      // deeper searches continue with direct launches
 constexpr int kDeepDepth = 64;                              // learned depth from which GB_DEEP_AUTO applies
+constexpr int kTailBlock = 1024;                            // the tail kernel's one workgroup
+constexpr int kTailCap = 1536;                              // frontier entries of a tail level (LDS)
+constexpr uint32_t kTailMaxEdges = 1u << 15;                // CSR entries of a level the tail expands
+constexpr uint32_t kTailLearnEdges = 4096;                  // levels this small count toward the tail start
+constexpr int kTailK = 4;                                   // edges per thread per pass
+constexpr int kNoTail = 1 << 20;                            // learned tail start when the last level was large
 
 struct BfsArgs {
     const int64_t* __restrict__ offsets;
@@ -691,6 +699,11 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(BfsArgs a, int L, int
         if (a.fused) {
             a.ctl[CTL_FCNT0 + (L & 1)] = cnt;
             a.ctl[CTL_RCNT0 + (L & 1)] = through;
+            // Where the search's tail starts (the host learns it): the first
+            // level of the last run of small levels that have a queue.
+            const bool small = E <= kTailLearnEdges && F <= kTailCap && (a.ctl[CTL_REP0 + L % 3] & REP_QUEUE);
+            if (!small) a.ctl[CTL_TAILFROM] = ~0ull;
+            else if (a.ctl[CTL_TAILFROM] == ~0ull) a.ctl[CTL_TAILFROM] = static_cast<unsigned long long>(L);
         }
     }
     if (a.fused) {
@@ -1050,6 +1063,198 @@ __global__ __launch_bounds__(kDeepBlock) void bfs_deep_kernel(BfsArgs a, int L) 
     if (bad) atomicOr(&a.ctl[CTL_ERR], ERR_RANGE);
 }
 
+// The tail of a search (one launch per level, symmetric graphs).  Once the
+// frontier is small, a launch per level costs more than the level's work
+// (about 4.5 us of dispatch and kernel boundary), and since searches of one
+// graph differ in depth, the levels queued for the deepest are no-ops for
+// the others.  From the level the host has learned (the first level of the
+// last run of small levels in recent searches), one workgroup runs the rest
+// of the search with workgroup barriers only; no workgroup waits on another.
+// A level's entries (vertex, first CSR entry, prefix of degrees) live in
+// LDS.  Each thread takes kTailK edges per pass (owner by binary search in
+// the prefix), claims their targets with atomicOr on the visited bitmap and
+// queues the winners with their row bounds (one LDS reservation per wave),
+// so the next level's size is known before it starts.
+//  * A level that does not fit (more than kTailCap entries or kTailMaxEdges
+//    CSR entries) is not expanded here.  At the first level nothing has
+//    changed yet: the host goes on with the usual launches from that level
+//    (GB_TAIL_DECLINED).  Later, the level is left as a bare vertex queue
+//    and the host goes on with bfs_deep_kernel from there
+//    (GB_TAIL_HANDED_BACK).
+//  * The counters the host reads (reached vertices, CSR entries, deepest
+//    level, largest frontier, done) are kept as expand() keeps them.
+static_assert(kTailCap <= 2 * kTailBlock, "the prefix pass takes two entries per thread");
+
+__global__ __launch_bounds__(kTailBlock) void bfs_tail_kernel(BfsArgs a, int T) {
+    __shared__ uint32_t s_pre[2][kTailCap + 1];  // exclusive prefix of degrees (+ the total)
+    __shared__ int64_t s_rb[2][kTailCap];
+    __shared__ int32_t s_v[2][kTailCap];
+    __shared__ unsigned long long s_wave[kTailBlock / 64];
+    __shared__ uint32_t s_count;
+    __shared__ unsigned long long s_edges;
+    __shared__ uint32_t s_bad;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    if (a.ctl[CTL_DONE] | a.ctl[CTL_ERR]) return;
+    const unsigned long long cnt = a.ctl[CTL_RING0 + T % 3];
+    uint32_t F = static_cast<uint32_t>(cnt >> 32), E = static_cast<uint32_t>(cnt);
+    if (F == 0) {
+        if (tid == 0) {
+            a.ctl[CTL_DONE] = 1;
+            a.ctl[CTL_TAIL] = GB_TAIL_FINISHED | (static_cast<unsigned long long>(T) << 8);
+        }
+        return;
+    }
+    if (!(a.ctl[CTL_REP0 + T % 3] & REP_QUEUE) || F > static_cast<uint32_t>(kTailCap) || E > kTailMaxEdges) {
+        if (tid == 0) a.ctl[CTL_TAIL] = GB_TAIL_DECLINED | (static_cast<unsigned long long>(T) << 8);
+        return;
+    }
+    {
+        const int q = T & 1;
+        for (uint32_t i = tid; i < F; i += kTailBlock) {
+            s_v[0][i] = a.q_v[q][i];
+            s_rb[0][i] = a.q_rb[q][i];
+            s_pre[0][i] = a.q_eoff[q][i];
+        }
+        if (tid == 0) {
+            s_pre[0][F] = E;
+            s_bad = 0;
+            if (a.ctl[CTL_TAILFROM] == ~0ull) a.ctl[CTL_TAILFROM] = static_cast<unsigned long long>(T);
+        }
+    }
+    int32_t* const levels = a.p->levels;
+    int32_t* const parents = a.p->parents;
+    const uint64_t n = static_cast<uint64_t>(a.n);
+    for (int L = T;; ++L) {
+        const int cur = (L - T) & 1, nxt = cur ^ 1;
+        if (tid == 0) {
+            a.ctl[CTL_NREACH] += F;
+            a.ctl[CTL_MREACH] += E;
+            a.ctl[CTL_MAXE] = std::max<unsigned long long>(a.ctl[CTL_MAXE], E);
+            a.ctl[CTL_MAXLEV] = std::max<unsigned long long>(a.ctl[CTL_MAXLEV], static_cast<unsigned long long>(L));
+            s_count = 0;
+            s_edges = 0;
+        }
+        __syncthreads();
+        bool bad = false;
+        for (uint32_t e0 = 0; e0 < E; e0 += kTailBlock * kTailK) {
+            int32_t u[kTailK];
+            uint32_t own[kTailK];
+#pragma unroll
+            for (int j = 0; j < kTailK; ++j) {
+                const uint32_t e = e0 + j * kTailBlock + tid;
+                u[j] = -1;
+                own[j] = 0;
+                if (e < E) {
+                    own[j] = owner_of(s_pre[cur], F, e);
+                    const int64_t i = s_rb[cur][own[j]] + (e - s_pre[cur][own[j]]);
+                    if (i < 0 || i >= a.m) {
+                        bad = true;
+                    } else {
+                        const int32_t c = a.cols[i];
+                        if (static_cast<uint64_t>(static_cast<uint32_t>(c)) >= n) bad = true;
+                        else u[j] = c;
+                    }
+                }
+            }
+            uint32_t old[kTailK];
+#pragma unroll
+            for (int j = 0; j < kTailK; ++j)
+                old[j] = u[j] >= 0 ? atomicOr(&a.vis[static_cast<uint32_t>(u[j]) >> 5], 1u << (u[j] & 31)) : ~0u;
+            uint32_t won = 0;
+#pragma unroll
+            for (int j = 0; j < kTailK; ++j) {
+                if (u[j] >= 0 && !((old[j] >> (u[j] & 31)) & 1u)) {
+                    won |= 1u << j;
+                    levels[u[j]] = L + 1;
+                    if (parents) parents[u[j]] = s_v[cur][own[j]];
+                }
+            }
+            // The winners' rows make the next level's entries.
+            int64_t wrb[kTailK];
+            uint32_t wdeg[kTailK];
+            unsigned long long my_edges = 0;
+#pragma unroll
+            for (int j = 0; j < kTailK; ++j) {
+                wrb[j] = 0;
+                wdeg[j] = 0;
+                if ((won >> j) & 1u) {
+                    wrb[j] = a.offsets[u[j]];
+                    wdeg[j] = static_cast<uint32_t>(a.offsets[u[j] + 1] - wrb[j]);
+                    my_edges += wdeg[j];
+                }
+            }
+            const uint32_t c = static_cast<uint32_t>(__popc(won));
+            const uint32_t incl = wave_inclusive_scan(c);
+            const uint32_t tot = __shfl(incl, 63, 64);
+            if (tot == 0) continue;  // wave-uniform
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(&s_count, tot);
+            base = __shfl(base, 0, 64);
+            const unsigned long long we = wave_sum(my_edges);
+            if (lane == 0) atomicAdd(&s_edges, we);
+            uint32_t slot = base + incl - c;
+#pragma unroll
+            for (int j = 0; j < kTailK; ++j) {
+                if ((won >> j) & 1u) {
+                    if (slot < static_cast<uint32_t>(kTailCap)) {
+                        s_v[nxt][slot] = u[j];
+                        s_rb[nxt][slot] = wrb[j];
+                        s_pre[nxt][slot] = wdeg[j];  // the degree until the prefix pass
+                    } else if (slot < n) {
+                        a.q_v[(L + 1) & 1][slot] = u[j];  // beyond the LDS: kept for the deep kernel
+                    } else {
+                        bad = true;
+                    }
+                    ++slot;
+                }
+            }
+        }
+        if (bad) s_bad = 1;
+        __syncthreads();
+        const uint32_t Fn = s_count;
+        const unsigned long long En = s_edges;
+        if (s_bad) {
+            if (tid == 0) atomicOr(&a.ctl[CTL_ERR], ERR_RANGE);
+            return;
+        }
+        if (Fn == 0) {
+            if (tid == 0) {
+                a.ctl[CTL_DONE] = 1;
+                a.ctl[CTL_TAIL] = GB_TAIL_FINISHED | (static_cast<unsigned long long>(L + 1) << 8);
+            }
+            return;
+        }
+        if (Fn > static_cast<uint32_t>(kTailCap) || En > kTailMaxEdges) {
+            // Level L+1 goes on in bfs_deep_kernel, which reads a bare
+            // vertex queue and counts the level itself.
+            for (uint32_t i = tid; i < std::min<uint32_t>(Fn, kTailCap); i += kTailBlock)
+                a.q_v[(L + 1) & 1][i] = s_v[nxt][i];
+            if (tid == 0) {
+                a.ctl[CTL_RING0 + (L + 1) % 3] = static_cast<unsigned long long>(Fn) << 32 |
+                                                 std::min<unsigned long long>(En, 0xffffffffull);
+                a.ctl[CTL_RING0 + (L + 2) % 3] = 0;
+                a.ctl[CTL_DEEPE0] = 0;
+                a.ctl[CTL_DEEPE0 + 1] = 0;
+                a.ctl[CTL_TAILFROM] = ~0ull;
+                a.ctl[CTL_TAIL] = GB_TAIL_HANDED_BACK | (static_cast<unsigned long long>(L + 1) << 8);
+            }
+            return;
+        }
+        // The next level's prefix of degrees, in place (two entries per thread).
+        const uint32_t i0 = 2 * tid, i1 = 2 * tid + 1;
+        const unsigned long long d0 = i0 < Fn ? s_pre[nxt][i0] : 0u;
+        const unsigned long long d1 = i1 < Fn ? s_pre[nxt][i1] : 0u;
+        unsigned long long total;
+        const unsigned long long ex = block_exclusive_scan<kTailBlock>(d0 + d1, s_wave, total);
+        if (i0 < Fn) s_pre[nxt][i0] = static_cast<uint32_t>(ex);
+        if (i1 < Fn) s_pre[nxt][i1] = static_cast<uint32_t>(ex + d0);
+        if (tid == 0) s_pre[nxt][Fn] = static_cast<uint32_t>(total);
+        F = Fn;
+        E = static_cast<uint32_t>(En);
+        __syncthreads();
+    }
+}
+
 // Upload of a symmetric graph: each row's neighbours ordered by descending
 // degree bucket (file order inside a bucket).  A bottom-up level probes a row
 // from its start and stops at the first frontier neighbour; the hubs, which
@@ -1179,6 +1384,7 @@ __global__ void bfs_clear_kernel(BfsArgs a) {
         a.ctl[CTL_NREACH] = d ? 0 : 1;  // expand(0) counts a source with edges
         a.ctl[CTL_REP0] = REP_QUEUE;
         a.ctl[CTL_BUFIRST] = ~0ull;
+        a.ctl[CTL_TAILFROM] = ~0ull;
     }
     for (uint64_t tt = threadIdx.x; tt * a.tile_edges < d; tt += blockDim.x) a.tile_start[0][tt] = 0;
 }
@@ -1457,8 +1663,13 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     a.fused = (a.bottom_up && !deep) ? 1u : 0u;
     a.n_edged = g.n_edged;
     last_fused = a.fused != 0;
+    bool force_deep = false;  // the levels after a tail that handed back a large level
     auto launch_level = [&](int L) {
         const int bu = bu_at(L) ? 1 : 0, bu1 = bu_at(L + 1) ? 1 : 0, bu2 = bu_at(L + 2) ? 1 : 0;
+        if (force_deep) {
+            hipLaunchKernelGGL(bfs_deep_kernel, dim3(deep_grid), dim3(kDeepBlock), 0, stream, a, L);
+            return;
+        }
         // One launch per level in deep mode, and for level 0 (the source
         // alone, top-down; the clears have zeroed both frontier bitmaps):
         // expand(L) builds level L+1's queue and keeps compact(L)'s counters.
@@ -1490,8 +1701,20 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
 
     int L = 0;
     int last_slot = 0;  // h_ctl slot holding the final control block
-    auto enqueue = [&](int count, int slot) {
-        for (int i = 0; i < count; ++i) launch_level(L++);
+    // The first batch of levels ends with the tail kernel at level tail_at
+    // (>= 0), which runs the rest of the search in one workgroup.
+    auto launch_levels = [&](int count, int tail_at) {
+        for (int i = 0; i < count; ++i) {
+            if (L == tail_at) {
+                hipLaunchKernelGGL(bfs_tail_kernel, dim3(1), dim3(kTailBlock), 0, stream, a, L);
+                ++L;
+                return;
+            }
+            launch_level(L++);
+        }
+    };
+    auto enqueue = [&](int count, int slot, int tail_at = -1) {
+        launch_levels(count, tail_at);
         GG_HIP(hipGetLastError());
         GG_HIP(hipEventRecord(ev_stop, stream));
         GG_HIP(hipMemcpyAsync(h_ctl + slot * kCtlWords, ws.ctl, sizeof(unsigned long long) * kCtlWords,
@@ -1521,6 +1744,7 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     };
 
     bool timed_traversal = true;  // ev_kernel marks the end of the clears
+    last_tail_at = -1;
     if (g.cfg.mode == GB_MODE_LEVEL_LOOP) {
         // One level at a time; the host reads the next frontier size.
         GG_HIP(hipEventRecord(ev_start, stream));
@@ -1552,16 +1776,23 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
         // the search).  Each batch ends with a snapshot of the control block;
         // the last batch's snapshot is the final state, so the common case
         // (the first batch reaches the end) needs one synchronisation.
-        const int first = std::max(depth_hint, 2);
+        int first = std::max(depth_hint, 2);
+        // One workgroup finishes the search from the tail level (learned, or
+        // set), when that comes before the end of the first batch.
+        const int tail_want = g.cfg.tail >= 0 ? g.cfg.tail : g.cfg.tail == GB_TAIL_AUTO ? tail_hint : -1;
+        if (a.fused && g.cfg.tail >= 0) first = std::max(first, g.cfg.tail + 1);
+        const int D = g.cfg.mode == GB_MODE_GRAPH ? std::min(first, kMaxCapturedLevels) : first;
+        const int tail_at = a.fused && tail_want >= 0 && tail_want < D ? tail_want : -1;
+        last_tail_at = tail_at;
         if (g.cfg.mode == GB_MODE_GRAPH) {
             // The parameter upload, the clears and the first levels are one
             // captured hipGraph, replayed while the depth, the bottom-up
             // window and the kernel shape stay the same.
-            const int D = std::min(first, kMaxCapturedLevels);
             const bool reuse = ws.graph_exec && ws.g_depth == D && ws.g_lo == bu_lo && ws.g_hi == bu_hi &&
                                ws.g_block == g.cfg.block && ws.g_k == g.cfg.edges_per_thread &&
                                ws.g_dense == int64_t(a.dense_edges) && ws.g_bu == int(a.bottom_up) &&
-                               ws.g_alpha == int(a.alpha) && ws.g_beta == int(a.beta) && ws.g_deep == int(deep);
+                               ws.g_alpha == int(a.alpha) && ws.g_beta == int(a.beta) && ws.g_deep == int(deep) &&
+                               ws.g_tail == tail_at;
             if (!reuse) {
                 if (ws.graph_exec) (void)hipGraphExecDestroy(ws.graph_exec);
                 ws.graph_exec = nullptr;
@@ -1569,7 +1800,8 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
                 GG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
                 try {
                     launch_prologue();
-                    for (int i = 0; i < D; ++i) launch_level(i);
+                    launch_levels(D, tail_at);
+                    GG_HIP(hipGetLastError());
                     GG_HIP(hipMemcpyAsync(h_ctl, ws.ctl, sizeof(unsigned long long) * kCtlWords,
                                           hipMemcpyDeviceToHost, stream));
                 } catch (...) {
@@ -1592,12 +1824,13 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
                 ws.g_alpha = int(a.alpha);
                 ws.g_beta = int(a.beta);
                 ws.g_deep = int(deep);
+                ws.g_tail = tail_at;
             }
             GG_HIP(hipEventRecord(ev_start, stream));
             GG_HIP(hipGraphLaunch(ws.graph_exec, stream));
             GG_HIP(hipEventRecord(ev_stop, stream));
             GG_HIP(hipEventRecord(ev_batch[0], stream));
-            L = D;
+            L = tail_at >= 0 ? tail_at + 1 : D;  // the level after the captured launches
             last_slot = 0;
             timed_traversal = false;
         } else {
@@ -1605,7 +1838,19 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
             launch_prologue();
             GG_HIP(hipGetLastError());
             GG_HIP(hipEventRecord(ev_kernel, stream));
-            enqueue(first, 0);
+            enqueue(D, 0, tail_at);
+        }
+        if (tail_at >= 0 && !done(0)) {
+            // The tail declined its first level (state untouched: the usual
+            // launches go on from there) or handed back a large level as a
+            // bare queue (the deep kernel goes on from there).
+            const unsigned long long tw = h_ctl[CTL_TAIL];
+            if ((tw & 0xffu) == GB_TAIL_HANDED_BACK) {
+                L = static_cast<int>(tw >> 8);
+                force_deep = true;
+            } else {
+                L = tail_at;
+            }
         }
         continue_levels();
         GG_HIP(hipEventSynchronize(ev_batch[last_slot]));
@@ -1615,9 +1860,10 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     if (fin[CTL_ERR] & ERR_RANGE) fail(GB_EDEVICE, "internal error: frontier entry or column index out of range");
     if (!fin[CTL_DONE]) fail(GB_EDEVICE, "internal error: BFS did not reach an empty frontier");
     last_num_levels = static_cast<int>(fin[CTL_MAXLEV]) + 1;
+    last_tail_outcome = last_tail_at >= 0 ? static_cast<int>(fin[CTL_TAIL] & 0xffu) : 0;
     last_bu_first = fin[CTL_BULAST] >= fin[CTL_BUFIRST] && fin[CTL_BUFIRST] != ~0ull ? int(fin[CTL_BUFIRST]) : -1;
     last_bu_last = last_bu_first >= 0 ? int(fin[CTL_BULAST]) : -1;
-    learn_from_search(static_cast<int64_t>(fin[CTL_MAXE]) > static_cast<int64_t>(a.dense_edges));
+    learn_from_search(static_cast<int64_t>(fin[CTL_MAXE]) > static_cast<int64_t>(a.dense_edges), fin[CTL_TAILFROM]);
     float ms = 0.f;
     GG_HIP(hipEventElapsedTime(&ms, ev_start, ev_stop));
     last_ms = ms;
@@ -1643,8 +1889,25 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
 //    (the first learned window replaces the default); after two searches in
 //    a row with only small frontiers (deep, thin graphs) no bottom-up
 //    launches at all, until a search has a dense level again.
-// Both only shape launches; any level without a bottom-up launch runs top-down.
-void Runner::learn_from_search(bool had_dense_level) {
+//  * tail_hint (one launch per level only): the level from which one
+//    workgroup runs the rest of the search -- the latest start of a final
+//    run of small levels among recent searches (raised at once, lowered
+//    after 256 searches in a row whose run started earlier).
+// All only shape launches; any level without a bottom-up launch runs
+// top-down, and a tail that meets a large level hands it back.
+void Runner::learn_from_search(bool had_dense_level, unsigned long long tail_from) {
+    if (last_fused) {
+        const int t = tail_from >= static_cast<unsigned long long>(kNoTail) ? kNoTail : static_cast<int>(tail_from);
+        if (tail_hint < 0 || t > tail_hint) {
+            tail_hint = t;
+            tail_streak = 0;
+        } else if (t < tail_hint && ++tail_streak >= 256) {
+            tail_hint = t;
+            tail_streak = 0;
+        } else if (t == tail_hint) {
+            tail_streak = 0;
+        }
+    }
     // compact() of the last level sets the done flag; with one launch per
     // level, expand() of the empty level after it does.
     const int want = last_num_levels + (last_fused ? 1 : 0);

@@ -432,6 +432,23 @@ int gb_bfs_get_deep_config(const gb_dgraph* dg, int* mode, int* used) {
     });
 }
 
+int gb_bfs_tail_config(gb_dgraph* dg, int level) {
+    return guarded([&] {
+        need(dg, "null device graph");
+        need(level == GB_TAIL_AUTO || level == GB_TAIL_OFF || level >= 0, "unknown tail level");
+        dg->d->cfg.tail = level;
+    });
+}
+
+int gb_bfs_get_tail_config(const gb_dgraph* dg, int* level, int* used, int* outcome) {
+    return guarded([&] {
+        need(dg, "null device graph");
+        if (level) *level = dg->d->cfg.tail;
+        if (used) *used = dg->d->runner->last_tail_at;
+        if (outcome) *outcome = dg->d->runner->last_tail_outcome;
+    });
+}
+
 int gb_bfs_autotune(gb_dgraph* dg, const int32_t* sources, int count, int reps, gb_tune_result* out) {
     return guarded([&] {
         need(dg && sources, "null argument");

@@ -51,6 +51,7 @@ struct BfsConfig {
     int alpha = 14;               // bottom-up when the frontier grows and its edges * alpha > unexplored edges
     int beta = 24;                // back top-down when the frontier shrinks below n / beta vertices
     int deep = GB_DEEP_AUTO;      // one launch per level for deep searches with thin frontiers
+    int tail = GB_TAIL_AUTO;      // level from which one workgroup finishes the search (GB_TAIL_*, or >= 0)
 };
 
 // Per-call parameters of a single-source BFS, read by the kernels from
@@ -82,7 +83,8 @@ struct BfsWorkspace {
     size_t bytes = 0;
     // Captured launch sequence (GB_MODE_GRAPH) and what it was captured for.
     hipGraphExec_t graph_exec = nullptr;
-    int g_depth = 0, g_lo = 0, g_hi = 0, g_block = 0, g_k = 0, g_bu = 0, g_alpha = 0, g_beta = 0, g_deep = 0;
+    int g_depth = 0, g_lo = 0, g_hi = 0, g_block = 0, g_k = 0, g_bu = 0, g_alpha = 0, g_beta = 0, g_deep = 0,
+        g_tail = -1;
     int64_t g_dense = -1;
 };
 
@@ -176,6 +178,10 @@ struct Runner {
     int short_streak = 0;                   // searches in a row shallower than depth_hint
     bool last_deep = false;                 // the last search ran in deep mode
     bool last_fused = false;                // the last search ran one launch per level
+    int tail_hint = -1;                     // level from which one workgroup finishes the search (-1: none yet)
+    int tail_streak = 0;                    // searches in a row whose small tail started earlier
+    int last_tail_at = -1;                  // level the last search started its tail at (-1: none)
+    int last_tail_outcome = 0;              // GB_TAIL_* outcome of that tail (0: none)
     const void* occ_kernel = nullptr;       // expand kernel whose occupancy occ_blocks holds
     int occ_blocks = 0;
     BfsWorkspace ws;
@@ -212,7 +218,7 @@ struct Runner {
    private:
     // Frees everything the runner holds (destructor, failed constructor).
     void release();
-    void learn_from_search(bool had_dense_level);
+    void learn_from_search(bool had_dense_level, unsigned long long tail_from);
     void ensure_workspace();
     void free_workspace();
     void ensure_ms_workspace();

@@ -188,6 +188,24 @@ int  gb_bfs_get_direction_config(const gb_dgraph* dg, int* enable, int* alpha, i
 #define GB_DEEP_OFF  2
 int  gb_bfs_deep_config(gb_dgraph* dg, int mode);
 int  gb_bfs_get_deep_config(const gb_dgraph* dg, int* mode, int* used);
+/* The tail of a search (no reference counterpart): on symmetric graphs
+ * outside deep mode, the launches stop at a level learned per handle (the
+ * latest start of a final run of small levels in recent searches), from
+ * which one workgroup runs the rest of the search with workgroup barriers.
+ * A level too large for it goes back to the per-level kernels.  level:
+ * GB_TAIL_AUTO (default, learned), GB_TAIL_OFF, or a level >= 0 to start
+ * the tail there.  Results do not depend on it.  *used (nullable): level the
+ * last gb_bfs started its tail at, -1 if none; *outcome (nullable):
+ * GB_TAIL_FINISHED, GB_TAIL_DECLINED (its first level was too large; the
+ * per-level kernels went on from there), GB_TAIL_HANDED_BACK (a later level
+ * was too large; the deep-mode kernel went on from there), 0 if no tail. */
+#define GB_TAIL_AUTO (-1)
+#define GB_TAIL_OFF  (-2)
+#define GB_TAIL_FINISHED    1
+#define GB_TAIL_DECLINED    2
+#define GB_TAIL_HANDED_BACK 3
+int  gb_bfs_tail_config(gb_dgraph* dg, int level);
+int  gb_bfs_get_tail_config(const gb_dgraph* dg, int* level, int* used, int* outcome);
 /* Levels the last gb_bfs on this handle searched bottom-up (-1, -1 if none). */
 int  gb_bfs_last_bottom_up(const gb_dgraph* dg, int* first_level, int* last_level);
 /* Launch hints learned from the searches so far: levels launched before the
