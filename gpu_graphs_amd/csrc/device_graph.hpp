@@ -149,7 +149,7 @@ struct MsWorkspace {
     size_t bytes = 0;
     hipGraphExec_t graph_exec = nullptr;  // captured batch sequence
     int graph_depth = 0, graph_block = 0, graph_kpt = 0, graph_bu = -1, graph_alpha = 0, graph_low = 0,
-        graph_deep = -1, graph_beta = -1;
+        graph_deep = -1, graph_beta = -1, graph_nib = -1;
     int64_t graph_sparse = -1;
 };
 
@@ -179,7 +179,10 @@ struct Runner {
     bool last_deep = false;                 // the last search ran in deep mode
     bool last_fused = false;                // the last search ran one launch per level
     int tail_hint = -1;                     // level from which one workgroup finishes the search (-1: none yet)
-    int tail_streak = 0;                    // searches in a row whose small tail started earlier
+    int tail_streak = 0;                    // searches in a row that would take an earlier tail level
+    static constexpr int kTailWindow = 64;  // searches the tail level is chosen over
+    int tail_seen[kTailWindow] = {};        // where their final runs of small levels began (ring)
+    int tail_seen_n = 0;                    // searches recorded so far
     int last_tail_at = -1;                  // level the last search started its tail at (-1: none)
     int last_tail_outcome = 0;              // GB_TAIL_* outcome of that tail (0: none)
     const void* occ_kernel = nullptr;       // expand kernel whose occupancy occ_blocks holds

@@ -84,6 +84,7 @@ constexpr int kUcRounds = 4;
 constexpr int kUcChunk = kUcBlock * kUcRounds;
 constexpr int kMaxCapturedLevels = 32;  // deeper searches continue with direct launches
 constexpr int kByteLevels = 254;         // levels kept in the byte lines; 254 = stored directly, 255 = none
+constexpr int kNibLevels = 14;           // levels kept in the nibble lines; 14 = stored directly, 15 = none
 constexpr int kMaxLevelBatch = 128;     // levels enqueued between done checks
 
 // Per vertex, side by side: a top-down probe reads both in one 16-byte load
@@ -98,7 +99,7 @@ struct MsArgs {
     const int32_t* __restrict__ cols;
     SeenNext* sn;
     unsigned long long* fmask;   // per vertex: sources that reached it in the last scanned level
-    unsigned long long* lv8;     // per vertex: 8 words = one level byte per source
+    unsigned long long* lv8;     // per vertex: its level line (8 words of bytes, or 4 of nibbles when nib)
     unsigned long long* reached; // bit per vertex: its lv8 line has been written in this batch
     const MsParams* p;           // sources, k, output rows (device memory)
     const int32_t* perm;         // original id -> id of the degree-ordered copy
@@ -126,6 +127,7 @@ struct MsArgs {
     uint32_t alpha;
     uint32_t low_degree;         // bottom-up: longer rows are scanned by the whole wave
     uint32_t beta;               // bottom-up only while F * beta >= nz (0: no such limit)
+    uint32_t nib;                // level lines of nibbles (shallow searches), else of bytes
 };
 
 template <int BLOCK, int K>
@@ -159,9 +161,32 @@ __device__ __forceinline__ unsigned long long spread_bits_to_bytes(unsigned long
     return (nz >> 7) * 0xFFull;
 }
 
-// After the last level: level rows [k][n] from the byte lines, a wave per 64
-// vertices, one full 256-byte store per source (byte 255 -> -1; byte 254 was
-// stored directly and is left alone).
+// Nibble i of the result is 0xF when bit i of b (16 bits) is set.
+__device__ __forceinline__ unsigned long long spread_bits_to_nibbles(unsigned long long b) {
+    unsigned long long x = b & 0xFFFFull;
+    x = (x | (x << 24)) & 0x000000FF000000FFull;
+    x = (x | (x << 12)) & 0x000F000F000F000Full;
+    x = (x | (x << 6)) & 0x0303030303030303ull;
+    x = (x | (x << 3)) & 0x1111111111111111ull;
+    return x * 0xFull;
+}
+
+// Level lines: per vertex, the level at which each source reached it.  Byte
+// lines (8 words) hold byte s for source s; nibble lines (4 words, for
+// searches known to stay shallow) nibble s, which halves the bytes every
+// level writes and the finish reads.  A format's largest value means "not
+// reached", the one below "stored in the output rows directly" (levels
+// from kByteLevels or kNibLevels on).
+__device__ __forceinline__ int line_pairs(const MsArgs& a) { return a.nib ? 2 : 4; }  // 16-byte pairs
+__device__ __forceinline__ int line_levels(const MsArgs& a) { return a.nib ? kNibLevels : kByteLevels; }
+__device__ __forceinline__ const ulonglong2* line_of(const MsArgs& a, int64_t v) {
+    return reinterpret_cast<const ulonglong2*>(a.lv8 + v * (a.nib ? 4 : 8));
+}
+
+// After the last level: level rows [k][n] from the level lines, a wave per 64
+// vertices, one full 256-byte store per source (a line's "not reached" value
+// -> -1; levels stored directly are left alone).
+template <bool NIB>
 __global__ __launch_bounds__(256) void ms_finish_kernel(MsArgs a, int only_if_done) {
     // Replayed right after the first levels: a no-op unless the search has ended.
     if (only_if_done && (!a.ctl[MS_DONE] || a.ctl[MS_ERR])) return;
@@ -177,20 +202,26 @@ __global__ __launch_bounds__(256) void ms_finish_kernel(MsArgs a, int only_if_do
         unsigned long long w[8];
         const int64_t vn = valid ? a.perm[v] : 0;
         const bool reached = valid && ((a.reached[vn >> 6] >> (vn & 63)) & 1ull);
-        const ulonglong2* line = reinterpret_cast<const ulonglong2*>(a.lv8 + vn * 8);
+        constexpr int kPairs = NIB ? 2 : 4;
+        const ulonglong2* line = reinterpret_cast<const ulonglong2*>(a.lv8 + vn * 2 * kPairs);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < kPairs; ++j) {
             const ulonglong2 x = reached ? line[j] : make_ulonglong2(~0ull, ~0ull);
             w[2 * j] = x.x;
             w[2 * j + 1] = x.y;
         }
+        constexpr uint32_t kBits = NIB ? 4 : 8, kNone = (1u << kBits) - 1;
+        constexpr uint32_t kDirect = NIB ? kNibLevels : kByteLevels;
+        // The rows are global memory: say so, so that the stores are global
+        // (not flat) stores.
+        __attribute__((address_space(1))) int32_t* const rows =
+            (__attribute__((address_space(1))) int32_t*)levels;
 #pragma unroll
         for (int s = 0; s < 64; ++s) {
             if (s < k) {
-                const uint32_t byte = static_cast<uint32_t>(w[s >> 3] >> (8 * (s & 7))) & 0xFFu;
-                if (valid && byte != static_cast<uint32_t>(kByteLevels))  // output only: streamed past the caches
-                    __builtin_nontemporal_store(byte == 0xFFu ? -1 : static_cast<int32_t>(byte),
-                                                &levels[int64_t(s) * a.n + v]);
+                const uint32_t x = static_cast<uint32_t>(w[s / (64 / kBits)] >> (kBits * (s % (64 / kBits)))) & kNone;
+                if (valid && x != kDirect)  // output only: streamed past the caches
+                    __builtin_nontemporal_store(x == kNone ? -1 : static_cast<int32_t>(x), &rows[int64_t(s) * a.n + v]);
             }
         }
     }
@@ -240,6 +271,21 @@ __device__ __forceinline__ unsigned long long merge_level_word(unsigned long lon
                                                                unsigned long long vb) {
     const unsigned long long bm = spread_bits_to_bytes(nv8);
     return (old & ~bm) | (vb & bm);
+}
+
+// Pair j of a level line with the entries of the sources in nv set to level
+// value lv (already clamped to the format), other entries kept from `old`.
+__device__ __forceinline__ ulonglong2 merge_line_pair(const MsArgs& a, ulonglong2 old, unsigned long long nv, int j,
+                                                      unsigned long long lv) {
+    if (a.nib) {
+        const unsigned long long vn = lv * 0x1111111111111111ull;
+        const unsigned long long m0 = spread_bits_to_nibbles(nv >> (32 * j));
+        const unsigned long long m1 = spread_bits_to_nibbles(nv >> (32 * j + 16));
+        return make_ulonglong2((old.x & ~m0) | (vn & m0), (old.y & ~m1) | (vn & m1));
+    }
+    const unsigned long long vb = lv * 0x0101010101010101ull;
+    return make_ulonglong2(merge_level_word(old.x, (nv >> (16 * j)) & 0xFFull, vb),
+                           merge_level_word(old.y, (nv >> (16 * j + 8)) & 0xFFull, vb));
 }
 
 // Chunk c of update_compact(L): thread t takes slots c*span + r*kUcBlock + t
@@ -299,16 +345,15 @@ __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, int64_t span, bo
         const unsigned long long sv = st[r].x;
         newbits[r] = st[r].y & ~sv;
         first[r] = newbits[r] && !sv;  // the first level that reaches the vertex
-        const ulonglong2* src = reinterpret_cast<const ulonglong2*>(a.lv8 + (vs[r] >= 0 ? vs[r] : 0) * 8);
+        const ulonglong2* src = line_of(a, vs[r] >= 0 ? vs[r] : 0);
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-            line[r][j] = (newbits[r] && !first[r]) ? src[j] : make_ulonglong2(~0ull, ~0ull);
+            line[r][j] = (newbits[r] && !first[r] && j < line_pairs(a)) ? src[j] : make_ulonglong2(~0ull, ~0ull);
     }
     // Step 4: stores.
     uint32_t active = 0;  // bit r: the vertex of slot r goes to the queue
     uint32_t kept = 0, edges = 0;
-    const unsigned long long lv = static_cast<unsigned long long>(L < kByteLevels ? L : kByteLevels);
-    const unsigned long long vb = lv * 0x0101010101010101ull;
+    const unsigned long long lv = static_cast<unsigned long long>(L < line_levels(a) ? L : line_levels(a));
 #pragma unroll
     for (int r = 0; r < kUcRounds; ++r) {
         const int64_t v = vs[r];
@@ -327,16 +372,15 @@ __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, int64_t span, bo
             const int64_t w = (ibase + r * kUcBlock + (tid & ~63u)) >> 6;
             if (fb && (tid & 63u) == 0) a.reached[w] |= fb;
         }
-        if (L >= kByteLevels) {
+        if (L >= line_levels(a)) {
             // every lane of the wave takes part in the shuffles of write_levels
             if (__ballot(nv != 0)) write_levels(levels, a.n, nv ? a.iperm[v] : 0, nv, L);
         }
         if (nv) {
-            ulonglong2* dst = reinterpret_cast<ulonglong2*>(a.lv8 + v * 8);
+            ulonglong2* dst = const_cast<ulonglong2*>(line_of(a, v));
 #pragma unroll
             for (int j = 0; j < 4; ++j)
-                dst[j] = make_ulonglong2(merge_level_word(line[r][j].x, (nv >> (16 * j)) & 0xFFull, vb),
-                                         merge_level_word(line[r][j].y, (nv >> (16 * j + 8)) & 0xFFull, vb));
+                if (j < line_pairs(a)) dst[j] = merge_line_pair(a, line[r][j], nv, j, lv);
             const uint32_t d = static_cast<uint32_t>(re[r] - rb[r]);
             if (d) {
                 active |= 1u << r;
@@ -779,8 +823,7 @@ __global__ __launch_bounds__(kMsDeepBlock) void ms_deep_kernel(MsArgs a, int L) 
     const uint32_t F = static_cast<uint32_t>(F64);
     int32_t* const levels = a.p->levels;
     unsigned long long* const cnt_next = &a.ctl[MS_DCNT0 + (L + 1) % 3];
-    const unsigned long long vb = static_cast<unsigned long long>(L < kByteLevels ? L : kByteLevels) *
-                                  0x0101010101010101ull;
+    const unsigned long long lv = static_cast<unsigned long long>(L < line_levels(a) ? L : line_levels(a));
     bool bad = false, anynew = false;
     // next of level L+1 (the words this level ORs into)
     auto next_out = [&](int64_t w) -> unsigned long long* { return odd ? &a.sn[w].next : &a.next1[w]; };
@@ -825,9 +868,10 @@ __global__ __launch_bounds__(kMsDeepBlock) void ms_deep_kernel(MsArgs a, int L) 
         // Its line (when an earlier level wrote it) and, for a short row, its
         // columns and their targets' state.
         ulonglong2 line[4];
-        const ulonglong2* src = reinterpret_cast<const ulonglong2*>(a.lv8 + (v >= 0 ? v : 0) * 8);
+        const ulonglong2* src = line_of(a, v >= 0 ? v : 0);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) line[j] = (nv && !first) ? src[j] : make_ulonglong2(~0ull, ~0ull);
+        for (int j = 0; j < 4; ++j)
+            line[j] = (nv && !first && j < line_pairs(a)) ? src[j] : make_ulonglong2(~0ull, ~0ull);
         int32_t u[kMsDeepRow];
 #pragma unroll
         for (int j = 0; j < kMsDeepRow; ++j) {
@@ -857,17 +901,16 @@ __global__ __launch_bounds__(kMsDeepBlock) void ms_deep_kernel(MsArgs a, int L) 
             }
         }
         if (first) atomicOr(&a.reached[v >> 6], 1ull << (v & 63));
-        if (L >= kByteLevels) {
+        if (L >= line_levels(a)) {
             // every lane of the wave takes part in the shuffles of write_levels
             if (__ballot(nv != 0)) write_levels(levels, a.n, nv ? a.iperm[v] : 0, nv, L);
         }
         if (nv) {
             anynew = true;
-            ulonglong2* dst = reinterpret_cast<ulonglong2*>(a.lv8 + v * 8);
+            ulonglong2* dst = const_cast<ulonglong2*>(line_of(a, v));
 #pragma unroll
             for (int j = 0; j < 4; ++j)
-                dst[j] = make_ulonglong2(merge_level_word(line[j].x, (nv >> (16 * j)) & 0xFFull, vb),
-                                         merge_level_word(line[j].y, (nv >> (16 * j + 8)) & 0xFFull, vb));
+                if (j < line_pairs(a)) dst[j] = merge_line_pair(a, line[j], nv, j, lv);
         }
         // Pushes along a short row; targets it turns from untouched to
         // touched join the next list.
@@ -1270,6 +1313,10 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     const bool deep = g.ms_cfg.deep == GB_DEEP_ON || (g.ms_cfg.deep == GB_DEEP_AUTO && ms.deep_learned);
     const uint32_t dgrid = static_cast<uint32_t>(
         std::max<int64_t>(1, std::min<int64_t>((n + kMsDeepBlock - 1) / kMsDeepBlock, int64_t(g.num_cus) * 2)));
+    // Nibble level lines when the last batch of this runner was shallow
+    // enough for them (a deeper batch stays exact: its deeper levels are
+    // stored in the output rows directly, and the next batch uses bytes).
+    a.nib = (!deep && ms.depth_hint > 0 && ms.depth_hint <= kNibLevels) ? 1u : 0u;
     int L = 0;
     auto launch_level = [&]() {
         if (deep) {
@@ -1287,7 +1334,8 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     // block is snapshotted behind it: a batch that fits the hint costs the
     // host one wait.
     auto launch_finish = [&](int only_if_done) {
-        hipLaunchKernelGGL(ms_finish_kernel, dim3(fgrid), dim3(256), 0, stream, a, only_if_done);
+        if (a.nib) hipLaunchKernelGGL(ms_finish_kernel<true>, dim3(fgrid), dim3(256), 0, stream, a, only_if_done);
+        else hipLaunchKernelGGL(ms_finish_kernel<false>, dim3(fgrid), dim3(256), 0, stream, a, only_if_done);
     };
     auto snapshot_to = [&](int slot) {
         GG_HIP(hipMemcpyAsync(ms.h_ctl + slot * kCtlWords, ms.ctl, sizeof(unsigned long long) * kCtlWords,
@@ -1309,7 +1357,7 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
                            ms.graph_kpt == g.ms_cfg.edges_per_thread && ms.graph_sparse == a.sparse_edges &&
                            ms.graph_bu == int(a.bottom_up) && ms.graph_alpha == int(a.alpha) &&
                            ms.graph_low == int(a.low_degree) && ms.graph_deep == int(deep) &&
-                           ms.graph_beta == int(a.beta);
+                           ms.graph_beta == int(a.beta) && ms.graph_nib == int(a.nib);
         if (!reuse) {
             if (ms.graph_exec) (void)hipGraphExecDestroy(ms.graph_exec);
             ms.graph_exec = nullptr;
@@ -1336,6 +1384,7 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
             ms.graph_low = int(a.low_degree);
             ms.graph_deep = int(deep);
             ms.graph_beta = int(a.beta);
+            ms.graph_nib = int(a.nib);
             GG_HIP(hipEventRecord(ev_start, stream));
         }
         GG_HIP(hipGraphLaunch(ms.graph_exec, stream));

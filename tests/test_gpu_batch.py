@@ -383,3 +383,28 @@ def test_levels_past_the_byte_range(gpu):
     sources = np.array([0, n - 1, 350, 253, 254, 255, 0], dtype=np.int32)
     levels = check_batch(G, dg, sources)
     assert levels[0, n - 1] == n - 1 and levels[2, 0] == 350
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_shallow_lines_then_a_deeper_batch(gpu, use_graph):
+    # After shallow batches a runner keeps levels in 4-bit lines (levels up
+    # to 13; deeper ones go to the output rows directly).  A batch that turns
+    # out deeper than that (sources on a separate 40-vertex path beside an
+    # R-MAT core) stays exact, and so do the batches after it, which go back
+    # to byte lines and then to 4-bit lines again.
+    core = gg.Graph.rmat(13, seed=98)
+    nc, plen = core.n, 40
+    cs = np.repeat(np.arange(nc), np.diff(core.offsets))
+    path = np.arange(nc, nc + plen)
+    src = np.concatenate([cs, path[:-1], path[1:]]).astype(np.int32)
+    dst = np.concatenate([core.cols, path[1:], path[:-1]]).astype(np.int32)
+    G = gg.Graph.from_edges(nc + plen, src, dst)
+    dg = gg.DeviceGraph(G, gpu)
+    dg.batch_config(use_graph=use_graph)
+    nz = np.nonzero(np.diff(G.offsets)[:nc])[0].astype(np.int32)
+    shallow = [nz[i * 40:(i + 1) * 40] for i in range(3)]
+    deep = np.concatenate([nz[200:230], path[-10:].astype(np.int32), [nc + plen - 1, nc + plen - 1]])
+    for i, sources in enumerate(shallow + [deep, shallow[0], shallow[1], deep[::-1].copy(), shallow[2]]):
+        log(f"lines batch {i}: {len(sources)} sources")
+        levels = check_batch(G, dg, sources)
+        assert (levels.max() >= 30) == (i in (3, 6)), i
