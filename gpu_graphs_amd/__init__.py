@@ -237,7 +237,7 @@ class DeviceGraph:
         self.n = graph.n
         self.m = graph.m
 
-    def config(self, block=256, edges_per_thread=4, blocks_per_cu=0, mode=GB_MODE_GRAPH, dense_edges=0):
+    def config(self, block=256, edges_per_thread=4, blocks_per_cu=0, mode=GB_MODE_ASYNC, dense_edges=0):
         """Kernel tunables; dense_edges=0 selects the default threshold."""
         check(lib().gb_bfs_config(self._h, block, edges_per_thread, blocks_per_cu, mode, dense_edges))
 

@@ -130,7 +130,8 @@ struct BfsArgs {
     int32_t* q_v[2];
     uint32_t* tile_start[2];
     unsigned long long* ctl;
-    const BfsParams* p;         // source, levels, parents of this call (device memory)
+    BfsParams* p;               // source, levels, parents of this call (device memory)
+    const BfsParams* hp;        // the same, host-visible (written by the host before the launch)
     int64_t n;
     int64_t m;
     uint32_t nwords;
@@ -1085,7 +1086,7 @@ __global__ __launch_bounds__(kDeepBlock) void bfs_deep_kernel(BfsArgs a, int L) 
 //    level, largest frontier, done) are kept as expand() keeps them.
 static_assert(kTailCap <= 2 * kTailBlock, "the prefix pass takes two entries per thread");
 
-__global__ __launch_bounds__(kTailBlock) void bfs_tail_kernel(BfsArgs a, int T) {
+__device__ void tail_levels(const BfsArgs& a, int T) {
     __shared__ uint32_t s_pre[2][kTailCap + 1];  // exclusive prefix of degrees (+ the total)
     __shared__ int64_t s_rb[2][kTailCap];
     __shared__ int32_t s_v[2][kTailCap];
@@ -1255,6 +1256,30 @@ __global__ __launch_bounds__(kTailBlock) void bfs_tail_kernel(BfsArgs a, int T) 
     }
 }
 
+// The control block as the host reads it after the search: copied into
+// host-visible memory by the kernel that ends the launched sequence, so the
+// sequence holds no copy node (a copy is a separate blit dispatch, and in a
+// replayed graph it also delays the submission of the kernels behind it).
+// Every thread of the workgroup calls this after its last update of ctl.
+__device__ __forceinline__ void snapshot_ctl(unsigned long long* ctl, unsigned long long* out) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x < CTL_WORDS)
+        out[threadIdx.x] = __hip_atomic_load(ctl + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// `snap` (host-visible, may be null): where the final control block goes.
+__global__ __launch_bounds__(kTailBlock) void bfs_tail_kernel(BfsArgs a, int T, unsigned long long* snap) {
+    tail_levels(a, T);
+    if (snap) snapshot_ctl(a.ctl, snap);
+}
+
+// The control block to host-visible memory when the sequence does not end
+// with the tail kernel (earlier kernels' updates are complete at its start).
+__global__ __launch_bounds__(64) void bfs_snapshot_kernel(const unsigned long long* ctl, unsigned long long* out) {
+    if (threadIdx.x < CTL_WORDS) out[threadIdx.x] = ctl[threadIdx.x];
+}
+
 // Upload of a symmetric graph: each row's neighbours ordered by descending
 // degree bucket (file order inside a bucket).  A bottom-up level probes a row
 // from its start and stops at the first frontier neighbour; the hubs, which
@@ -1332,9 +1357,13 @@ __global__ void row_head_kernel(const int64_t* __restrict__ offsets, const int32
 //    queue entry, and the control block reset.  The thread that clears an
 //    entry of the source also makes the source's store to it.
 __global__ void bfs_clear_kernel(BfsArgs a) {
-    const int32_t s = a.p->source;
-    int32_t* const levels = a.p->levels;
-    int32_t* const parents = a.p->parents;
+    // The call's parameters come from the host-visible block (one read per
+    // workgroup); the later kernels read the device copy made here.
+    const BfsParams prm = *a.hp;
+    const int32_t s = prm.source;
+    int32_t* const levels = prm.levels;
+    int32_t* const parents = prm.parents;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.p = prm;
     const int64_t sw = static_cast<uint32_t>(s) >> 5;
     const uint32_t sbit = 1u << (static_cast<uint32_t>(s) & 31u);
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
@@ -1516,8 +1545,12 @@ Runner::Runner(DeviceGraph& graph) : g(graph), device(graph.device), n(graph.n),
         GG_HIP(hipEventCreate(&ev_kernel));
         GG_HIP(hipEventCreateWithFlags(&ev_batch[0], hipEventDisableTiming));
         GG_HIP(hipEventCreateWithFlags(&ev_batch[1], hipEventDisableTiming));
+        // Coherent host memory: the kernels that end a launched sequence
+        // store the control block here, and the host reads it after the
+        // sequence's event.
         GG_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_ctl), sizeof(unsigned long long) * 2 * kCtlWords,
-                             hipHostMallocDefault));
+                             hipHostMallocCoherent | hipHostMallocMapped));
+        GG_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&h_ctl_dev), h_ctl, 0));
     } catch (...) {
         release();
         throw;
@@ -1533,6 +1566,7 @@ void Runner::release() {
     free_ms_workspace();
     if (h_ctl) (void)hipHostFree(h_ctl);
     h_ctl = nullptr;
+    h_ctl_dev = nullptr;
     for (hipEvent_t* e : {&ev_start, &ev_stop, &ev_kernel, &ev_batch[0], &ev_batch[1]}) {
         if (*e) (void)hipEventDestroy(*e);
         *e = nullptr;
@@ -1582,7 +1616,10 @@ void Runner::ensure_workspace() {
         alloc(&ws.levels, sizeof(int32_t) * n);
         alloc(&ws.parents, sizeof(int32_t) * n);
         alloc(&ws.params, sizeof(BfsParams));
-        GG_HIP(hipHostMalloc(reinterpret_cast<void**>(&ws.h_params), sizeof(BfsParams), hipHostMallocDefault));
+        // Coherent host memory, read by the clear kernel of every search.
+        GG_HIP(hipHostMalloc(reinterpret_cast<void**>(&ws.h_params), sizeof(BfsParams),
+                             hipHostMallocCoherent | hipHostMallocMapped));
+        GG_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&ws.h_params_dev), ws.h_params, 0));
     } catch (...) {
         free_workspace();
         throw;
@@ -1621,6 +1658,7 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     }
     a.ctl = ws.ctl;
     a.p = ws.params;
+    a.hp = ws.h_params_dev;
     a.n = n;
     a.m = m;
     a.nwords = static_cast<uint32_t>((n + 31) / 32);
@@ -1688,42 +1726,55 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
         hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(g.cfg.block), 0, stream, a, L, bu, bu1, 0);
         hipLaunchKernelGGL(bfs_compact_kernel, dim3(cgrid), dim3(kCompactBlock), 0, stream, a, L, bu1, bu2);
     };
-    // The call's parameters go to device memory first (pinned source buffer:
-    // the copy reads it when the stream reaches it, also inside a graph).
+    // The call's parameters: the host-visible block the clear kernel reads
+    // when the stream reaches it (also inside a replayed graph; the previous
+    // call has ended, so nothing reads the block meanwhile).
     ws.h_params->source = source;
     ws.h_params->pad = 0;
     ws.h_params->levels = d_levels;
     ws.h_params->parents = d_parents;
     auto launch_prologue = [&]() {
-        GG_HIP(hipMemcpyAsync(ws.params, ws.h_params, sizeof(BfsParams), hipMemcpyHostToDevice, stream));
         hipLaunchKernelGGL(bfs_clear_kernel, dim3(clear_grid), dim3(256), 0, stream, a);
     };
 
     int L = 0;
     int last_slot = 0;  // h_ctl slot holding the final control block
     // The first batch of levels ends with the tail kernel at level tail_at
-    // (>= 0), which runs the rest of the search in one workgroup.
-    auto launch_levels = [&](int count, int tail_at) {
+    // (>= 0), which runs the rest of the search in one workgroup.  The
+    // launches end with the control block stored to h_ctl slot `slot`: by the
+    // tail kernel, or by the snapshot kernel.
+    auto launch_levels = [&](int count, int tail_at, int slot) {
+        unsigned long long* snap = h_ctl_dev + slot * kCtlWords;
         for (int i = 0; i < count; ++i) {
             if (L == tail_at) {
-                hipLaunchKernelGGL(bfs_tail_kernel, dim3(1), dim3(kTailBlock), 0, stream, a, L);
+                hipLaunchKernelGGL(bfs_tail_kernel, dim3(1), dim3(kTailBlock), 0, stream, a, L, snap);
                 ++L;
                 return;
             }
             launch_level(L++);
         }
+        hipLaunchKernelGGL(bfs_snapshot_kernel, dim3(1), dim3(64), 0, stream,
+                           static_cast<const unsigned long long*>(ws.ctl), snap);
     };
-    auto enqueue = [&](int count, int slot, int tail_at = -1) {
-        launch_levels(count, tail_at);
+    // The event that follows the batch of levels whose snapshot is in slot i.
+    // The first batch is followed by ev_stop alone (an event record costs a
+    // few microseconds of the call's wall time); later batches by
+    // ev_batch[slot] as well, which stays valid while ev_stop moves on.
+    hipEvent_t batch_ev[2] = {ev_batch[0], ev_batch[1]};
+    auto enqueue = [&](int count, int slot, int tail_at = -1, bool first = false) {
+        launch_levels(count, tail_at, slot);
         GG_HIP(hipGetLastError());
         GG_HIP(hipEventRecord(ev_stop, stream));
-        GG_HIP(hipMemcpyAsync(h_ctl + slot * kCtlWords, ws.ctl, sizeof(unsigned long long) * kCtlWords,
-                              hipMemcpyDeviceToHost, stream));
-        GG_HIP(hipEventRecord(ev_batch[slot], stream));
+        if (first) {
+            batch_ev[slot] = ev_stop;
+        } else {
+            GG_HIP(hipEventRecord(ev_batch[slot], stream));
+            batch_ev[slot] = ev_batch[slot];
+        }
         last_slot = slot;
     };
     auto done = [&](int slot) {
-        GG_HIP(hipEventSynchronize(ev_batch[slot]));
+        GG_HIP(hipEventSynchronize(batch_ev[slot]));
         return h_ctl[slot * kCtlWords + CTL_DONE] != 0 || h_ctl[slot * kCtlWords + CTL_ERR] != 0;
     };
     // Deeper than the first batch: batches of doubling size (at most 64
@@ -1800,10 +1851,8 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
                 GG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
                 try {
                     launch_prologue();
-                    launch_levels(D, tail_at);
+                    launch_levels(D, tail_at, 0);
                     GG_HIP(hipGetLastError());
-                    GG_HIP(hipMemcpyAsync(h_ctl, ws.ctl, sizeof(unsigned long long) * kCtlWords,
-                                          hipMemcpyDeviceToHost, stream));
                 } catch (...) {
                     hipGraph_t dummy = nullptr;
                     (void)hipStreamEndCapture(stream, &dummy);
@@ -1829,17 +1878,18 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
             GG_HIP(hipEventRecord(ev_start, stream));
             GG_HIP(hipGraphLaunch(ws.graph_exec, stream));
             GG_HIP(hipEventRecord(ev_stop, stream));
-            GG_HIP(hipEventRecord(ev_batch[0], stream));
+            batch_ev[0] = ev_stop;
             L = tail_at >= 0 ? tail_at + 1 : D;  // the level after the captured launches
             last_slot = 0;
-            timed_traversal = false;
         } else {
+            // Direct launches (the default): for a sequence of about ten
+            // kernels a replayed graph costs more device and host time here
+            // than the launches themselves.
             GG_HIP(hipEventRecord(ev_start, stream));
             launch_prologue();
-            GG_HIP(hipGetLastError());
-            GG_HIP(hipEventRecord(ev_kernel, stream));
-            enqueue(D, 0, tail_at);
+            enqueue(D, 0, tail_at, true);
         }
+        timed_traversal = false;
         if (tail_at >= 0 && !done(0)) {
             // The tail declined its first level (state untouched: the usual
             // launches go on from there) or handed back a large level as a
@@ -1853,7 +1903,7 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
             }
         }
         continue_levels();
-        GG_HIP(hipEventSynchronize(ev_batch[last_slot]));
+        GG_HIP(hipEventSynchronize(batch_ev[last_slot]));
     }
     const unsigned long long* fin = h_ctl + last_slot * kCtlWords;
     if (fin[CTL_ERR] & ERR_TILE) fail(GB_EDEVICE, "internal error: inconsistent frontier tile index");

@@ -45,7 +45,7 @@ struct BfsConfig {
     int block = 256;              // threads per workgroup of expand() (4 waves)
     int edges_per_thread = 4;     // K: a tile is block * K frontier edges
     int blocks_per_cu = 0;        // 0 = at most 16 waves per CU
-    int mode = GB_MODE_GRAPH;
+    int mode = GB_MODE_ASYNC;     // direct launches: cheaper than a replayed graph for ~10 kernels
     int64_t dense_edges = 0;      // levels with more frontier edges use bitmap compaction
     bool direction_opt = true;    // bottom-up levels on symmetric graphs
     int alpha = 14;               // bottom-up when the frontier grows and its edges * alpha > unexplored edges
@@ -76,7 +76,8 @@ struct BfsWorkspace {
     int32_t* levels = nullptr;                  // levels staging for host-pointer calls
     int32_t* parents = nullptr;
     BfsParams* params = nullptr;                // device
-    BfsParams* h_params = nullptr;              // pinned (read by the captured copy)
+    BfsParams* h_params = nullptr;              // pinned, coherent: written by the host, read by the clear kernel
+    BfsParams* h_params_dev = nullptr;          // its device-side address
     int64_t tile_cap = 0;                       // entries per tile_start buffer
     int64_t nchunks = 0;                        // compaction chunks
     int tile_edges = 0;                         // T the tile_start buffers were sized for
@@ -129,8 +130,10 @@ struct MsWorkspace {
     unsigned long long* next1 = nullptr;  // n: deep batches: next of odd levels (even: sn's next)
     unsigned long long* ctl = nullptr;
     MsParams* params = nullptr;           // device
-    MsParams* h_params = nullptr;         // pinned (read by the captured copy)
-    unsigned long long* h_ctl = nullptr;  // pinned: 2 snapshots of ctl
+    MsParams* h_params = nullptr;         // pinned, coherent: written by the host, read by the clear kernel
+    MsParams* h_params_dev = nullptr;     // its device-side address
+    unsigned long long* h_ctl = nullptr;  // pinned, coherent: 2 snapshots of ctl
+    unsigned long long* h_ctl_dev = nullptr;  // its device-side address (the kernels store the snapshots)
     unsigned long long* partial = nullptr;  // per-workgroup statistics
     int32_t* levels = nullptr;            // staging for host-pointer calls
     size_t levels_bytes = 0;
@@ -168,7 +171,8 @@ struct Runner {
     hipEvent_t ev_batch[2] = {nullptr, nullptr};
     static constexpr int kCtlWords = 32;
     static constexpr int kDefaultBuHigh = 15;
-    unsigned long long* h_ctl = nullptr;  // pinned: 2 snapshots of the control block
+    unsigned long long* h_ctl = nullptr;  // pinned, coherent: 2 snapshots of the control block
+    unsigned long long* h_ctl_dev = nullptr;  // its device-side address (the kernels store the snapshots)
     int last_num_levels = 0;              // depth of the previous BFS: first batch size
     int bu_lo = 1, bu_hi = kDefaultBuHigh;  // levels that get a bottom-up launch
     bool bu_learned = false;                // window learned from a search (else the default)

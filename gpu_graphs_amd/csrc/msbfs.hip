@@ -101,7 +101,8 @@ struct MsArgs {
     unsigned long long* fmask;   // per vertex: sources that reached it in the last scanned level
     unsigned long long* lv8;     // per vertex: its level line (8 words of bytes, or 4 of nibbles when nib)
     unsigned long long* reached; // bit per vertex: its lv8 line has been written in this batch
-    const MsParams* p;           // sources, k, output rows (device memory)
+    MsParams* p;                 // sources, k, output rows (device memory)
+    const MsParams* hp;          // the same, host-visible (written by the host before the launch)
     const int32_t* perm;         // original id -> id of the degree-ordered copy
     const int32_t* iperm;        // id of the degree-ordered copy -> original id
     const uint32_t* heavy_tiles; // rounds + 1: first tile of each round of the heavy rows
@@ -145,7 +146,13 @@ struct MsTileLds {
 __global__ void ms_clear_kernel(MsArgs a) {
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
     const int64_t tid0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t end = a.p->clear_all ? a.n : a.nz;
+    // The batch's parameters come from the host-visible block; workgroup 0
+    // makes the device copy the later kernels read.
+    static_assert(sizeof(MsParams) % 4 == 0, "parameter block copied in words");
+    if (blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < sizeof(MsParams) / 4; i += blockDim.x)
+            reinterpret_cast<uint32_t*>(a.p)[i] = reinterpret_cast<const uint32_t*>(a.hp)[i];
+    const int64_t end = a.hp->clear_all ? a.n : a.nz;
     for (int64_t i = tid0; i < end; i += stride) {
         a.sn[i].seen = 0;
         a.sn[i].next = 0;
@@ -186,8 +193,11 @@ __device__ __forceinline__ const ulonglong2* line_of(const MsArgs& a, int64_t v)
 // After the last level: level rows [k][n] from the level lines, a wave per 64
 // vertices, one full 256-byte store per source (a line's "not reached" value
 // -> -1; levels stored directly are left alone).
+// `snap` (host-visible, may be null): where workgroup 0 stores the control
+// block, which no kernel changes from here on.
 template <bool NIB>
-__global__ __launch_bounds__(256) void ms_finish_kernel(MsArgs a, int only_if_done) {
+__global__ __launch_bounds__(256) void ms_finish_kernel(MsArgs a, int only_if_done, unsigned long long* snap) {
+    if (snap && blockIdx.x == 0 && threadIdx.x < Runner::kCtlWords) snap[threadIdx.x] = a.ctl[threadIdx.x];
     // Replayed right after the first levels: a no-op unless the search has ended.
     if (only_if_done && (!a.ctl[MS_DONE] || a.ctl[MS_ERR])) return;
     const int k = a.p->k;
@@ -225,6 +235,12 @@ __global__ __launch_bounds__(256) void ms_finish_kernel(MsArgs a, int only_if_do
             }
         }
     }
+}
+
+// The control block to host-visible memory between batches of levels (the
+// earlier kernels' updates are complete at its start).
+__global__ __launch_bounds__(64) void ms_snapshot_kernel(const unsigned long long* ctl, unsigned long long* out) {
+    if (threadIdx.x < Runner::kCtlWords) out[threadIdx.x] = ctl[threadIdx.x];
 }
 
 // Level 0: next(source i) |= bit i; each distinct source once on the touched
@@ -1199,9 +1215,14 @@ void Runner::ensure_ms_workspace() {
         alloc(&ms.ctl, sizeof(unsigned long long) * kCtlWords);
         alloc(&ms.params, sizeof(MsParams));
         alloc(&ms.partial, sizeof(unsigned long long) * kStatsBlocks * 64 * 2);
-        GG_HIP(hipHostMalloc(reinterpret_cast<void**>(&ms.h_params), sizeof(MsParams), hipHostMallocDefault));
+        // Coherent host memory: the clear kernel reads the parameters from
+        // it, and the finish or snapshot kernel stores the control block.
+        GG_HIP(hipHostMalloc(reinterpret_cast<void**>(&ms.h_params), sizeof(MsParams),
+                             hipHostMallocCoherent | hipHostMallocMapped));
+        GG_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&ms.h_params_dev), ms.h_params, 0));
         GG_HIP(hipHostMalloc(reinterpret_cast<void**>(&ms.h_ctl), sizeof(unsigned long long) * 2 * kCtlWords,
-                             hipHostMallocDefault));
+                             hipHostMallocCoherent | hipHostMallocMapped));
+        GG_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&ms.h_ctl_dev), ms.h_ctl, 0));
     } catch (...) {
         free_ms_workspace();
         throw;
@@ -1253,6 +1274,7 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     a.lv8 = ms.lv8;
     a.reached = ms.reached;
     a.p = ms.params;
+    a.hp = ms.h_params_dev;
     a.q_rb = ms.q_rb;
     a.q_eoff = ms.q_eoff;
     a.q_v = ms.q_v;
@@ -1304,7 +1326,6 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     ms.h_params->prev_k = ms.tail_clean ? ms.prev_k : 0;
     ms.tail_clean = false;
     auto launch_prologue = [&]() {
-        GG_HIP(hipMemcpyAsync(ms.params, ms.h_params, sizeof(MsParams), hipMemcpyHostToDevice, stream));
         hipLaunchKernelGGL(ms_clear_kernel, dim3(cgrid), dim3(256), 0, stream, a);
         hipLaunchKernelGGL(ms_seed_kernel, dim3(1), dim3(64), 0, stream, a);
     };
@@ -1333,19 +1354,22 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     // ended the search (the kernel checks the done flag), and the control
     // block is snapshotted behind it: a batch that fits the hint costs the
     // host one wait.
-    auto launch_finish = [&](int only_if_done) {
-        if (a.nib) hipLaunchKernelGGL(ms_finish_kernel<true>, dim3(fgrid), dim3(256), 0, stream, a, only_if_done);
-        else hipLaunchKernelGGL(ms_finish_kernel<false>, dim3(fgrid), dim3(256), 0, stream, a, only_if_done);
+    // The finish also stores the control block to h_ctl slot `slot`.
+    auto launch_finish = [&](int only_if_done, int slot) {
+        unsigned long long* snap = ms.h_ctl_dev + slot * kCtlWords;
+        if (a.nib)
+            hipLaunchKernelGGL(ms_finish_kernel<true>, dim3(fgrid), dim3(256), 0, stream, a, only_if_done, snap);
+        else
+            hipLaunchKernelGGL(ms_finish_kernel<false>, dim3(fgrid), dim3(256), 0, stream, a, only_if_done, snap);
     };
     auto snapshot_to = [&](int slot) {
-        GG_HIP(hipMemcpyAsync(ms.h_ctl + slot * kCtlWords, ms.ctl, sizeof(unsigned long long) * kCtlWords,
-                              hipMemcpyDeviceToHost, stream));
+        hipLaunchKernelGGL(ms_snapshot_kernel, dim3(1), dim3(64), 0, stream,
+                           static_cast<const unsigned long long*>(ms.ctl), ms.h_ctl_dev + slot * kCtlWords);
     };
     auto first_part = [&]() {
         launch_prologue();
         for (int i = 0; i < depth_hint; ++i) launch_level();
-        launch_finish(1);
-        snapshot_to(0);
+        launch_finish(1, 0);
     };
     GG_HIP(hipEventRecord(ev_start, stream));
     if (g.ms_cfg.use_graph) {
@@ -1427,10 +1451,10 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
         // All levels have run (later ones were no-ops): level rows from the
         // byte lines, then the final control block (behind the snapshot
         // still in flight to the same slot).
-        launch_finish(0);
+        launch_finish(0, last);
         GG_HIP(hipGetLastError());
         GG_HIP(hipEventRecord(ev_stop, stream));
-        snapshot(last);
+        GG_HIP(hipEventRecord(ev_batch[last], stream));
         GG_HIP(hipEventSynchronize(ev_batch[last]));
     }
     const unsigned long long* fin = ms.h_ctl + last * kCtlWords;

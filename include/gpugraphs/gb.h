@@ -144,10 +144,11 @@ void gb_device_free(gb_dgraph* dg);
 /* ---- single-source BFS (replaces gpuBFS::gpuBFS, gpuBFS.cu:13-116) ----- */
 
 /* Execution modes for gb_bfs_config. */
-#define GB_MODE_ASYNC       0   /* levels queued ahead in batches, no per-level host sync */
+#define GB_MODE_ASYNC       0   /* levels queued ahead in batches, no per-level host sync
+                                   (default)                                               */
 #define GB_MODE_LEVEL_LOOP  1   /* host reads the frontier size after every level        */
 #define GB_MODE_GRAPH       2   /* as ASYNC, the first levels replayed from one captured
-                                   hipGraph (default)                                      */
+                                   hipGraph                                                */
 
 /* levels: n int32 (host), -1 = unreached.  parents: nullable, n int32,
  * parent[source] = source, -1 = unreached (any valid BFS tree). */
