@@ -117,7 +117,7 @@ struct MsParams {
 // Device workspace of the batched multi-source BFS.
 struct MsWorkspace {
     unsigned long long* sn = nullptr;     // 2n: per vertex, sources that reached it and sources reaching it now
-    unsigned long long* fmask = nullptr;  // n: sources that reached it in the last scanned level
+    unsigned long long* fmask = nullptr;  // n: copy of seen, stored when it changes (gathered bottom-up)
     unsigned long long* lv8 = nullptr;    // 8n: one level byte per (vertex, source)
     unsigned long long* reached = nullptr;  // ceil(n/64): vertices whose lv8 line the batch has written
     int64_t* q_rb = nullptr;              // queue of active vertices

@@ -22,9 +22,12 @@
 // update_compact visits only the listed vertices -- a level then costs
 // O(frontier), which is what deep, thin graphs (road networks) need.
 // On symmetric graphs a big level runs bottom-up instead: every vertex with
-// sources still unseen ORs the frontier masks of its neighbours (fmask,
-// stored by the scanning update_compact) and stops as soon as all its unseen
-// sources are covered; next(v) is then a plain store.  A level goes
+// sources still unseen ORs the seen masks of its neighbours (fmask, a copy
+// of seen that update_compact stores when a vertex gains bits) and stops as
+// soon as all its unseen sources are covered; next(v) is then a plain
+// store.  A neighbour's seen bits are as good as its level-L bits there: a
+// source s unseen at v after level L has dist_s(v) > L, so every neighbour
+// u has dist_s(u) >= L, and s in seen(u) means dist_s(u) = L.  A level goes
 // bottom-up when its frontier edges * alpha exceed the edges of the vertices
 // not yet reached by every source.
 // Level writes are staged as one byte per (vertex, source): each vertex owns
@@ -98,7 +101,7 @@ struct MsArgs {
     const int64_t* __restrict__ offsets;
     const int32_t* __restrict__ cols;
     SeenNext* sn;
-    unsigned long long* fmask;   // per vertex: sources that reached it in the last scanned level
+    unsigned long long* fmask;   // per vertex: seen, stored when it changes (the mask bottom-up levels gather)
     unsigned long long* lv8;     // per vertex: its level line (8 words of bytes, or 4 of nibbles when nib)
     unsigned long long* reached; // bit per vertex: its lv8 line has been written in this batch
     MsParams* p;                 // sources, k, output rows (device memory)
@@ -157,6 +160,7 @@ __global__ void ms_clear_kernel(MsArgs a) {
         a.sn[i].seen = 0;
         a.sn[i].next = 0;
         a.next1[i] = 0;
+        a.fmask[i] = 0;
     }
     for (int64_t i = tid0; i < (a.n + 63) / 64; i += stride) a.reached[i] = 0;
 }
@@ -376,7 +380,7 @@ __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, int64_t span, bo
         const unsigned long long nv = newbits[r];
         if (v >= 0) {
             if (st[r].y) *reinterpret_cast<ulonglong2*>(a.sn + v) = make_ulonglong2(st[r].x | nv, 0ull);
-            a.fmask[v] = nv;  // a scan covers every vertex: the next level may go bottom-up
+            if (nv) a.fmask[v] = st[r].x | nv;  // seen only changes here
         }
         act |= nv;
         if (list) {
@@ -659,8 +663,9 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(MsArgs a, int L) {
     }
     const bool sparse = E <= a.sparse_edges;
     const int npar = (L + 1) & 1;
-    // Bottom-up needs every vertex's frontier mask: only after a scanning
-    // update_compact(L).  MS_FULLDEG is final (update_compact(L) has ended).
+    // Bottom-up only after a scanning update_compact(L): a level handed a
+    // touched list is sparse, and top-down is cheaper there.  MS_FULLDEG is
+    // final (update_compact(L) has ended).
     const bool scanned = a.ctl[MS_TVALID0 + (L & 1)] == 0;
     const int64_t open_edges = a.m - static_cast<int64_t>(a.ctl[MS_FULLDEG]);
     // Beamer's second rule as well: once the frontier holds fewer than

@@ -408,3 +408,38 @@ def test_shallow_lines_then_a_deeper_batch(gpu, use_graph):
         log(f"lines batch {i}: {len(sources)} sources")
         levels = check_batch(G, dg, sources)
         assert (levels.max() >= 30) == (i in (3, 6)), i
+
+
+class _View:
+    """A device pointer into another allocation (at an element offset)."""
+
+    def __init__(self, base, offset):
+        import ctypes
+        self.ptr = ctypes.c_void_p(base.ptr.value + offset * base.dtype.itemsize)
+
+
+def test_rows_at_any_alignment(gpu):
+    # Output rows that start 16-byte aligned or not (a device buffer at an
+    # offset of 0..3 entries), with levels kept in the lines and levels stored
+    # into the rows directly (a 700-vertex path, levels past 254); n is a
+    # multiple of 4 in both, the entries around the rows stay untouched.
+    n = 700
+    p = np.arange(n - 1, dtype=np.int32)
+    path = gg.Graph.from_edges(n, np.concatenate([p, p + 1]), np.concatenate([p + 1, p]))
+    cases = [(gg.Graph.rmat(12, seed=64), None), (path, np.array([0, n - 1, 350, 3], dtype=np.int32))]
+    for G, sources in cases:
+        assert G.n % 4 == 0
+        if sources is None:
+            cand = np.nonzero(np.diff(G.offsets))[0]
+            sources = np.random.default_rng(64).choice(cand, 64, replace=False).astype(np.int32)
+        k = sources.size
+        want = O.bfs_multi(G.offsets, G.cols, sources)
+        dg = gg.DeviceGraph(G, gpu)
+        base = gg.DeviceArray(gpu, np.int32, k * G.n + 8)
+        for off in range(4):
+            log(f"alignment: n={G.n} k={k} offset {off}")
+            base.upload(np.full(base.count, 0x5A5A5A5A, dtype=np.int32))
+            dg.bfs_batch_device(sources, _View(base, off))
+            got = base.numpy()
+            np.testing.assert_array_equal(got[off:off + k * G.n].reshape(k, G.n), want, err_msg=f"offset {off}")
+            assert (got[:off] == 0x5A5A5A5A).all() and (got[off + k * G.n:] == 0x5A5A5A5A).all()
