@@ -576,53 +576,65 @@ __device__ void bottom_up_level(const BfsArgs& a, int L, BuWaveLds* s_wave, bool
     if (enqueue) {
         // Queue level L+1 (a top-down level may follow) from the bitmap words
         // this workgroup wrote: in each pass its waves covered WAVES *
-        // kBuGroups consecutive groups, 2 words each.  One packed (count <<
-        // 32 | edges) reservation per workgroup and pass, as in compact(); a
-        // winner has an edge (to its parent).  The barrier makes the waves'
-        // word stores visible.
+        // kBuGroups consecutive groups, 2 words each.  The words of all its
+        // passes are taken at once, spread over the workgroup's threads, and
+        // one packed (count << 32 | edges) reservation covers them all, as in
+        // compact(); a winner has an edge (to its parent).  The barrier makes
+        // the waves' word stores visible.
         __syncthreads();
-        constexpr uint32_t kWords = WAVES * kBuGroups * 2;
+        constexpr uint32_t kBlock = WAVES * 64;
+        constexpr uint32_t kWords = WAVES * kBuGroups * 2;  // words per pass
         __shared__ unsigned long long s_wave[WAVES];
         __shared__ unsigned long long s_base;
         const int q = (L + 1) & 1;
-        for (uint64_t p0 = static_cast<uint64_t>(blockIdx.x) * WAVES * kBuGroups; p0 < ngroups;
-             p0 += static_cast<uint64_t>(gridDim.x) * WAVES * kBuGroups) {
-            const uint64_t w = p0 * 2 + threadIdx.x;
-            const uint32_t bits = (threadIdx.x < kWords && w < a.nwords) ? fout[w] : 0u;
-            uint32_t cnt = 0, edges = 0;
+        const uint64_t first = static_cast<uint64_t>(blockIdx.x) * WAVES * kBuGroups;
+        const uint64_t pstride = static_cast<uint64_t>(gridDim.x) * WAVES * kBuGroups;
+        const uint64_t npass = first < ngroups ? (ngroups - first + pstride - 1) / pstride : 0;
+        const uint64_t nslots = npass * kWords;
+        // Word slot j: word j % kWords of pass j / kWords.
+        auto word_of = [&](uint64_t j) { return (first + (j / kWords) * pstride) * 2 + j % kWords; };
+        uint32_t cnt = 0, edges = 0;
+        for (uint64_t j = threadIdx.x; j < nslots; j += kBlock) {
+            const uint64_t w = word_of(j);
+            const uint32_t bits = w < a.nwords ? fout[w] : 0u;
             for (uint32_t b = bits; b; b &= b - 1) {
                 const int64_t v = static_cast<int64_t>(w * 32 + __ffs(b) - 1);
                 ++cnt;
                 edges += static_cast<uint32_t>(a.offsets[v + 1] - a.offsets[v]);
             }
-            const unsigned long long mine = (static_cast<unsigned long long>(cnt) << 32) | edges;
-            unsigned long long total;
-            const unsigned long long excl = block_exclusive_scan<WAVES * 64>(mine, s_wave, total);
-            if (threadIdx.x == 0) {
-                s_base = total ? atomicAdd(&a.ctl[CTL_RING0 + (L + 1) % 3], total) : 0ull;
-                if (total && !reservation_fits(s_base, total, a.n, a.m)) {
-                    atomicOr(&a.ctl[CTL_ERR], ERR_RANGE);
-                    s_base = ~0ull;
-                }
-            }
-            __syncthreads();
-            const unsigned long long base = s_base;
-            __syncthreads();  // s_base is rewritten by the next pass
-            if (!bits || base == ~0ull) continue;
-            uint32_t slot = static_cast<uint32_t>((base + excl) >> 32);
-            uint32_t eo = static_cast<uint32_t>(base + excl);
-            for (uint32_t b = bits; b; b &= b - 1) {
-                const int64_t v = static_cast<int64_t>(w * 32 + __ffs(b) - 1);
-                const int64_t rb = a.offsets[v];
-                const uint32_t d = static_cast<uint32_t>(a.offsets[v + 1] - rb);
-                a.q_rb[q][slot] = rb;
-                a.q_eoff[q][slot] = eo;
-                a.q_v[q][slot] = static_cast<int32_t>(v);
-                mark_tiles(a.tile_start[q], a.tile_edges, eo, d, slot);
-                ++slot;
-                eo += d;
+        }
+        const unsigned long long mine = (static_cast<unsigned long long>(cnt) << 32) | edges;
+        unsigned long long total;
+        const unsigned long long excl = block_exclusive_scan<kBlock>(mine, s_wave, total);
+        if (threadIdx.x == 0) {
+            s_base = total ? atomicAdd(&a.ctl[CTL_RING0 + (L + 1) % 3], total) : 0ull;
+            if (total && !reservation_fits(s_base, total, a.n, a.m)) {
+                atomicOr(&a.ctl[CTL_ERR], ERR_RANGE);
+                s_base = ~0ull;
             }
         }
+        __syncthreads();
+        const unsigned long long base = s_base;
+        if (cnt && base != ~0ull) {
+            uint32_t slot = static_cast<uint32_t>((base + excl) >> 32);
+            uint32_t eo = static_cast<uint32_t>(base + excl);
+            for (uint64_t j = threadIdx.x; j < nslots; j += kBlock) {
+                const uint64_t w = word_of(j);
+                const uint32_t bits = w < a.nwords ? fout[w] : 0u;
+                for (uint32_t b = bits; b; b &= b - 1) {
+                    const int64_t v = static_cast<int64_t>(w * 32 + __ffs(b) - 1);
+                    const int64_t rb = a.offsets[v];
+                    const uint32_t d = static_cast<uint32_t>(a.offsets[v + 1] - rb);
+                    a.q_rb[q][slot] = rb;
+                    a.q_eoff[q][slot] = eo;
+                    a.q_v[q][slot] = static_cast<int32_t>(v);
+                    mark_tiles(a.tile_start[q], a.tile_edges, eo, d, slot);
+                    ++slot;
+                    eo += d;
+                }
+            }
+        }
+        __syncthreads();  // s_wave is reused below
     }
     if (bad) atomicOr(&a.ctl[CTL_ERR], ERR_RANGE);
     const unsigned long long mine = (static_cast<unsigned long long>(wave_sum(my_cnt)) << 32) | wave_sum(my_edges);
