@@ -407,25 +407,35 @@ __device__ void expand_tile(const BfsArgs& a, int32_t* levels, int32_t* parents,
 //
 // The step is bound by memory latency, not bandwidth (a search level
 // touches few bytes per vertex), so it is organised for loads in flight:
-//   * a wave takes kBuGroups groups of 64 vertices per pass and issues the
-//     visited-word, offset, first-entries and frontier-probe loads of all of
-//     them together (four dependent steps per pass, not per group);
-//   * every unvisited lane probes the first kBuFirst neighbours of its
-//     vertex, kept side by side per vertex (`head`, built at upload), so a
-//     wave reads them for its 64 vertices in one coalesced load; the first
-//     hit usually comes within them;
+//   * a wave owns chunks of kBuChunkGroups groups of 64 vertices (their
+//     visited words, one load per lane) and lists the chunk's unvisited
+//     vertices in LDS first: a late level, where most vertices are visited,
+//     then costs one round per chunk instead of one pass per 4 groups;
+//   * a round takes kBuGroups list entries per lane and issues the
+//     first-entries, offset and frontier-probe loads of all of them
+//     together (three dependent steps per round);
+//   * every listed lane probes the first kBuFirst neighbours of its vertex,
+//     kept side by side per vertex (`head`, built at upload); the first hit
+//     usually comes within them;
 //   * rows not settled by that go to a per-wave list in LDS, which 8-lane
 //     sub-waves drain: a sub-wave scans 8 consecutive entries of its row per
 //     step, stops at the first hit, and takes the next row from the list as
 //     soon as it is done, so long and short rows share the wave's steps.
-// The wave owns the bitmap words of its groups and writes them with plain
-// stores; the output bitmap arrives zeroed (compact(L-1)), so only words
-// with new vertices are written.
+// The vertices found are collected as bits in LDS; the wave owns its
+// chunk's bitmap words and writes them with plain stores.  The output
+// bitmap arrives zeroed (two levels ahead), so only words with new vertices
+// are written.
+constexpr int kBuChunkGroups = 16;                 // groups of 64 vertices per wave chunk
+constexpr int kBuChunk = kBuChunkGroups * 64;      // vertices per wave chunk
+constexpr int kBuChunkWords = kBuChunkGroups * 2;  // their visited words (lanes 0..31)
+static_assert(kBuChunkWords <= 64, "one visited word per lane");
 struct BuWaveLds {
-    int64_t rb[kBuGroups * 64];    // list: next entry of the row
-    uint32_t len[kBuGroups * 64];  // list: entries left
-    uint16_t vid[kBuGroups * 64];  // list: vertex (group * 64 + lane) within the pass
-    int32_t par[kBuGroups * 64];   // by vertex within the pass: parent found by the list, or -1
+    int64_t rb[kBuGroups * 64];    // long rows: next entry of the row
+    uint32_t len[kBuGroups * 64];  // long rows: entries left
+    uint16_t vid[kBuGroups * 64];  // long rows: slot (j * 64 + lane) within the round
+    int32_t par[kBuGroups * 64];   // by slot within the round: parent found by the long-row list, or -1
+    uint16_t unv[kBuChunk];        // the chunk's unvisited vertices (offsets in the chunk)
+    uint32_t found[kBuChunkWords]; // the chunk's vertices found at this level
 };
 
 template <int WAVES>
@@ -439,7 +449,7 @@ __device__ void bottom_up_level(const BfsArgs& a, int L, BuWaveLds* s_wave, bool
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t sg = lane >> 3, sl = lane & 7u;  // sub-wave and lane within it
     const uint64_t n = static_cast<uint64_t>(a.n);
-    const uint64_t ngroups = (n + 63) / 64;
+    const uint64_t nchunks = (n + kBuChunk - 1) / kBuChunk;
     const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * WAVES;
     uint32_t my_cnt = 0;
     uint64_t my_edges = 0;
@@ -451,148 +461,165 @@ __device__ void bottom_up_level(const BfsArgs& a, int L, BuWaveLds* s_wave, bool
         }
         return (fin[static_cast<uint32_t>(u) >> 5] >> (u & 31)) & 1u;
     };
-    for (uint64_t g0 = (static_cast<uint64_t>(blockIdx.x) * WAVES + (threadIdx.x >> 6)) * kBuGroups;
-         g0 < ngroups; g0 += nwaves * kBuGroups) {
-        uint32_t vw[kBuGroups];
-        bool unv[kBuGroups];
-        int64_t rb[kBuGroups], re[kBuGroups];
-        int32_t par[kBuGroups];
-        // Step 1: visited words.
-#pragma unroll
-        for (int j = 0; j < kBuGroups; ++j) {
-            const uint64_t v = (g0 + j) * 64 + lane;
-            vw[j] = v < n ? a.vis[(g0 + j) * 2 + (lane >> 5)] : ~0u;
-            unv[j] = !((vw[j] >> (lane & 31u)) & 1u);
+    for (uint64_t c = static_cast<uint64_t>(blockIdx.x) * WAVES + (threadIdx.x >> 6); c < nchunks; c += nwaves) {
+        // The chunk's visited words (lanes 0..31) and its unvisited vertices,
+        // listed in LDS in vertex order.
+        const uint64_t w = c * kBuChunkWords + lane;
+        uint32_t vw = ~0u;
+        if (lane < static_cast<uint32_t>(kBuChunkWords) && w < a.nwords) {
+            vw = a.vis[w];
+            const uint64_t v0 = w * 32;  // bits of vertices >= n count as visited
+            if (v0 + 32 > n) vw |= ~0u << static_cast<uint32_t>(n - v0);
         }
-        // Step 2: first neighbours (side by side for consecutive vertices)
-        // and row bounds of the unvisited vertices.
-        int4 hd[kBuGroups];
-#pragma unroll
-        for (int j = 0; j < kBuGroups; ++j) {
-            rb[j] = re[j] = 0;
-            hd[j] = make_int4(-1, -1, -1, -1);
-            if (unv[j]) {
-                const uint64_t v = (g0 + j) * 64 + lane;
-                hd[j] = a.head[v];
-                rb[j] = a.offsets[v];
-                re[j] = a.offsets[v + 1];
-            }
+        const uint32_t mine = static_cast<uint32_t>(__popc(~vw));
+        const uint32_t incl = wave_inclusive_scan(mine);
+        const uint32_t U = __shfl(incl, 63, 64);
+        if (U == 0) continue;  // wave-uniform
+        {
+            uint32_t pos = incl - mine;
+            for (uint32_t b = ~vw; b; b &= b - 1) sw.unv[pos++] = static_cast<uint16_t>(lane * 32 + __ffs(b) - 1);
         }
-        // Step 3: probes of the first neighbours.
-        uint32_t cont = 0;  // bit j: the row goes to the list
+        if (lane < static_cast<uint32_t>(kBuChunkWords)) sw.found[lane] = 0u;
+        wave_lds_sync();
+        for (uint32_t r0 = 0; r0 < U; r0 += kBuGroups * 64) {
+            int64_t vtx[kBuGroups];
+            int64_t rb[kBuGroups], re[kBuGroups];
+            int32_t par[kBuGroups];
+            // Step 1: first neighbours (side by side per vertex) and row
+            // bounds of the listed vertices.
+            int4 hd[kBuGroups];
 #pragma unroll
-        for (int j = 0; j < kBuGroups; ++j) {
-            const int32_t u[kBuFirst] = {hd[j].x, hd[j].y, hd[j].z, hd[j].w};
-            par[j] = -1;
-#pragma unroll
-            for (int k = 0; k < kBuFirst; ++k)
-                if (par[j] < 0 && u[k] >= 0 && in_frontier(u[k])) par[j] = u[k];
-            if (unv[j] && par[j] < 0 && re[j] - rb[j] > kBuFirst) cont |= 1u << j;
-        }
-        // Step 4: the rows still open, drained by the 8 sub-waves.
-        uint32_t nitems = 0;
-#pragma unroll
-        for (int j = 0; j < kBuGroups; ++j) {
-            const bool c = (cont >> j) & 1u;
-            const uint64_t m = wave_ballot(c);
-            if (c) {
-                const uint32_t slot = nitems + rank_below(m);
-                sw.rb[slot] = rb[j] + kBuFirst;
-                sw.len[slot] = static_cast<uint32_t>(re[j] - rb[j] - kBuFirst);
-                sw.vid[slot] = static_cast<uint16_t>(j * 64 + lane);
-                sw.par[j * 64 + lane] = -1;
-            }
-            nitems += static_cast<uint32_t>(__popcll(m));
-        }
-        if (nitems) {
-            wave_lds_sync();
-            int32_t item = sg < nitems ? static_cast<int32_t>(sg) : -1;
-            int64_t pos = 0, end = 0;
-            if (item >= 0) {
-                pos = sw.rb[item];
-                end = pos + sw.len[item];
-            }
-            uint32_t next = 8;
-            while (wave_ballot(item >= 0)) {
-                const int64_t i = pos + sl;
-                int32_t cu = -1;
-                bool hit = false;
-                if (item >= 0 && i < end) {
-                    cu = a.cols[i];
-                    hit = in_frontier(cu);
+            for (int j = 0; j < kBuGroups; ++j) {
+                const uint32_t i = r0 + j * 64 + lane;
+                vtx[j] = i < U ? static_cast<int64_t>(c * kBuChunk + sw.unv[i]) : -1;
+                rb[j] = re[j] = 0;
+                hd[j] = make_int4(-1, -1, -1, -1);
+                if (vtx[j] >= 0) {
+                    hd[j] = a.head[vtx[j]];
+                    rb[j] = a.offsets[vtx[j]];
+                    re[j] = a.offsets[vtx[j] + 1];
                 }
-                const uint64_t hb = wave_ballot(hit);
-                const uint32_t mine = static_cast<uint32_t>(hb >> (sg * 8)) & 0xffu;
-                const int src = mine ? static_cast<int>(sg * 8 + __ffs(mine) - 1) : static_cast<int>(lane);
-                const int32_t pu = __shfl(cu, src, 64);
-                bool done = false;
+            }
+            // Step 2: probes of the first neighbours.
+            uint32_t cont = 0;  // bit j: the row goes to the long-row list
+#pragma unroll
+            for (int j = 0; j < kBuGroups; ++j) {
+                const int32_t u[kBuFirst] = {hd[j].x, hd[j].y, hd[j].z, hd[j].w};
+                par[j] = -1;
+#pragma unroll
+                for (int k = 0; k < kBuFirst; ++k)
+                    if (par[j] < 0 && u[k] >= 0 && in_frontier(u[k])) par[j] = u[k];
+                if (vtx[j] >= 0 && par[j] < 0 && re[j] - rb[j] > kBuFirst) cont |= 1u << j;
+            }
+            // Step 3: the rows still open, drained by the 8 sub-waves.
+            uint32_t nitems = 0;
+#pragma unroll
+            for (int j = 0; j < kBuGroups; ++j) {
+                const bool cc = (cont >> j) & 1u;
+                const uint64_t m = wave_ballot(cc);
+                if (cc) {
+                    const uint32_t slot = nitems + rank_below(m);
+                    sw.rb[slot] = rb[j] + kBuFirst;
+                    sw.len[slot] = static_cast<uint32_t>(re[j] - rb[j] - kBuFirst);
+                    sw.vid[slot] = static_cast<uint16_t>(j * 64 + lane);
+                    sw.par[j * 64 + lane] = -1;
+                }
+                nitems += static_cast<uint32_t>(__popcll(m));
+            }
+            if (nitems) {
+                wave_lds_sync();
+                int32_t item = sg < nitems ? static_cast<int32_t>(sg) : -1;
+                int64_t pos = 0, end = 0;
                 if (item >= 0) {
-                    if (mine) {
-                        if (sl == 0) sw.par[sw.vid[item]] = pu;
-                        done = true;
-                    } else {
-                        pos += 8;
-                        done = pos >= end;
-                    }
+                    pos = sw.rb[item];
+                    end = pos + sw.len[item];
                 }
-                const uint64_t need = wave_ballot(done && sl == 0);
-                if (done) {
-                    const uint32_t t = next + static_cast<uint32_t>(__popcll(need & ((1ull << (sg * 8)) - 1)));
-                    item = t < nitems ? static_cast<int32_t>(t) : -1;
+                uint32_t next = 8;
+                while (wave_ballot(item >= 0)) {
+                    const int64_t i = pos + sl;
+                    int32_t cu = -1;
+                    bool hit = false;
+                    if (item >= 0 && i < end) {
+                        cu = a.cols[i];
+                        hit = in_frontier(cu);
+                    }
+                    const uint64_t hb = wave_ballot(hit);
+                    const uint32_t mine8 = static_cast<uint32_t>(hb >> (sg * 8)) & 0xffu;
+                    const int src = mine8 ? static_cast<int>(sg * 8 + __ffs(mine8) - 1) : static_cast<int>(lane);
+                    const int32_t pu = __shfl(cu, src, 64);
+                    bool done = false;
                     if (item >= 0) {
-                        pos = sw.rb[item];
-                        end = pos + sw.len[item];
+                        if (mine8) {
+                            if (sl == 0) sw.par[sw.vid[item]] = pu;
+                            done = true;
+                        } else {
+                            pos += 8;
+                            done = pos >= end;
+                        }
+                    }
+                    const uint64_t need = wave_ballot(done && sl == 0);
+                    if (done) {
+                        const uint32_t t = next + static_cast<uint32_t>(__popcll(need & ((1ull << (sg * 8)) - 1)));
+                        item = t < nitems ? static_cast<int32_t>(t) : -1;
+                        if (item >= 0) {
+                            pos = sw.rb[item];
+                            end = pos + sw.len[item];
+                        }
+                    }
+                    next += static_cast<uint32_t>(__popcll(need));
+                }
+                wave_lds_sync();
+            }
+            // Step 4: levels, parents, and the bits of the vertices found.
+#pragma unroll
+            for (int j = 0; j < kBuGroups; ++j) {
+                if ((cont >> j) & 1u) par[j] = sw.par[j * 64 + lane];
+                const bool found = vtx[j] >= 0 && par[j] >= 0;
+                if (found) {
+                    const int64_t v = vtx[j];
+                    levels[v] = L + 1;
+                    if (parents) parents[v] = par[j];
+                    const uint32_t off = static_cast<uint32_t>(v - static_cast<int64_t>(c * kBuChunk));
+                    atomicOr(&sw.found[off >> 5], 1u << (off & 31u));
+                    if (!enqueue) {  // with `enqueue` the block counts them below
+                        ++my_cnt;
+                        my_edges += static_cast<uint64_t>(re[j] - rb[j]);
                     }
                 }
-                next += static_cast<uint32_t>(__popcll(need));
             }
-            wave_lds_sync();
+            if (nitems) wave_lds_sync();  // the long-row list of the next round reuses the LDS
         }
-        // Step 5: levels, parents, bitmap words.
-#pragma unroll
-        for (int j = 0; j < kBuGroups; ++j) {
-            if ((cont >> j) & 1u) par[j] = sw.par[j * 64 + lane];
-            const bool found = unv[j] && par[j] >= 0;
-            const uint64_t nb = wave_ballot(found);
-            const uint64_t v = (g0 + j) * 64 + lane;
-            if (found) {
-                levels[v] = L + 1;
-                if (parents) parents[v] = par[j];
-            }
-            if (found && !enqueue) {  // with `enqueue` the block counts them below
-                ++my_cnt;
-                my_edges += static_cast<uint64_t>(re[j] - rb[j]);
-            }
-            const uint64_t w = (g0 + j) * 2 + (lane >> 5);
-            const uint32_t bits = static_cast<uint32_t>(nb >> (lane & 32u));
-            if ((lane & 31u) == 0 && bits && w < a.nwords) {  // fout was zeroed two levels ahead
+        wave_lds_sync();
+        if (lane < static_cast<uint32_t>(kBuChunkWords) && w < a.nwords) {
+            const uint32_t bits = sw.found[lane];
+            if (bits) {  // fout was zeroed two levels ahead
                 fout[w] = bits;
-                a.vis[w] = vw[j] | bits;
-                a.prev[w] = vw[j] | bits;
+                a.vis[w] = vw | bits;
+                a.prev[w] = vw | bits;
             }
         }
-        if (nitems) wave_lds_sync();  // the list of the next pass reuses the LDS
+        wave_lds_sync();  // the next chunk's list reuses the LDS
     }
     if (enqueue) {
         // Queue level L+1 (a top-down level may follow) from the bitmap words
-        // this workgroup wrote: in each pass its waves covered WAVES *
-        // kBuGroups consecutive groups, 2 words each.  The words of all its
+        // this workgroup wrote: in each pass its waves covered WAVES
+        // consecutive chunks (kBuChunkWords words each).  The words of all its
         // passes are taken at once, spread over the workgroup's threads, and
         // one packed (count << 32 | edges) reservation covers them all, as in
         // compact(); a winner has an edge (to its parent).  The barrier makes
         // the waves' word stores visible.
         __syncthreads();
         constexpr uint32_t kBlock = WAVES * 64;
-        constexpr uint32_t kWords = WAVES * kBuGroups * 2;  // words per pass
+        constexpr uint32_t kWords = WAVES * kBuChunkWords;  // words per pass
         __shared__ unsigned long long s_wave[WAVES];
         __shared__ unsigned long long s_base;
         const int q = (L + 1) & 1;
-        const uint64_t first = static_cast<uint64_t>(blockIdx.x) * WAVES * kBuGroups;
-        const uint64_t pstride = static_cast<uint64_t>(gridDim.x) * WAVES * kBuGroups;
-        const uint64_t npass = first < ngroups ? (ngroups - first + pstride - 1) / pstride : 0;
+        const uint64_t first = static_cast<uint64_t>(blockIdx.x) * WAVES;  // first chunk of each pass
+        const uint64_t pstride = static_cast<uint64_t>(gridDim.x) * WAVES;
+        const uint64_t npass = first < nchunks ? (nchunks - first + pstride - 1) / pstride : 0;
         const uint64_t nslots = npass * kWords;
         // Word slot j: word j % kWords of pass j / kWords.
-        auto word_of = [&](uint64_t j) { return (first + (j / kWords) * pstride) * 2 + j % kWords; };
+        auto word_of = [&](uint64_t j) { return (first + (j / kWords) * pstride) * kBuChunkWords + j % kWords; };
         uint32_t cnt = 0, edges = 0;
         for (uint64_t j = threadIdx.x; j < nslots; j += kBlock) {
             const uint64_t w = word_of(j);
