@@ -505,6 +505,9 @@ __global__ __launch_bounds__(kUcBlock) void ms_update_compact_kernel(MsArgs a, i
 // wave owns 64 consecutive vertices; rows of at most `low_degree` entries
 // are scanned by their lane, longer rows by the whole wave 64 entries at a
 // time; a scan stops once all of the vertex's unseen sources are covered.
+constexpr int kSubStep = 4;     // bottom-up long rows: entries per lane in a sub-wave's later steps
+constexpr int kHeavyBatch = 4;  // bottom-up heavy rows: tiles a wave scans together
+
 __device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask) {
     const uint32_t lane = threadIdx.x & 63u;
     bool bad = false;
@@ -538,30 +541,47 @@ __device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask) {
             rb = a.offsets[v] + int64_t(lo) * kMsHeavyRow;
             re = std::min<int64_t>(a.offsets[v + 1], rb + kMsHeavyRow);
         }
+        // The open tiles, kHeavyBatch at a time: the loads of all of them
+        // are issued before any is used.
         uint64_t om = wave_ballot(un != 0);
         while (om) {
-            const int hl = __ffsll(static_cast<long long>(om)) - 1;
-            om &= om - 1;
-            const int64_t hv = __shfl(v, hl, 64);
-            const int64_t hrb = __shfl(rb, hl, 64);
-            const int64_t hre = __shfl(re, hl, 64);
-            const unsigned long long hun = __shfl(un, hl, 64);
-            int32_t u[kMsHeavyRow / 64];
+            int64_t hv[kHeavyBatch], hrb[kHeavyBatch], hre[kHeavyBatch];
+            unsigned long long hun[kHeavyBatch];
 #pragma unroll
-            for (int j = 0; j < kMsHeavyRow / 64; ++j) {
-                const int64_t i = hrb + j * 64 + lane;
-                u[j] = i < hre ? a.cols[i] : -1;
+            for (int g = 0; g < kHeavyBatch; ++g) {
+                const bool use = om != 0;  // wave-uniform
+                const int hl = use ? __ffsll(static_cast<long long>(om)) - 1 : 0;
+                om &= om - 1;
+                hv[g] = __shfl(v, hl, 64);
+                hrb[g] = __shfl(rb, hl, 64);
+                hre[g] = use ? __shfl(re, hl, 64) : hrb[g];
+                hun[g] = use ? __shfl(un, hl, 64) : 0ull;
             }
-            unsigned long long acc = 0;
+            int32_t u[kHeavyBatch][kMsHeavyRow / 64];
 #pragma unroll
-            for (int j = 0; j < kMsHeavyRow / 64; ++j) {
-                if (u[j] < 0) continue;
-                if (u[j] >= a.n) bad = true;
-                else acc |= a.fmask[u[j]];
+            for (int g = 0; g < kHeavyBatch; ++g)
+#pragma unroll
+                for (int j = 0; j < kMsHeavyRow / 64; ++j) {
+                    const int64_t i = hrb[g] + j * 64 + lane;
+                    u[g][j] = i < hre[g] ? a.cols[i] : -1;
+                }
+            unsigned long long acc[kHeavyBatch];
+#pragma unroll
+            for (int g = 0; g < kHeavyBatch; ++g) {
+                acc[g] = 0;
+#pragma unroll
+                for (int j = 0; j < kMsHeavyRow / 64; ++j) {
+                    if (u[g][j] < 0) continue;
+                    if (u[g][j] >= a.n) bad = true;
+                    else acc[g] |= a.fmask[u[g][j]];
+                }
             }
 #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) acc |= __shfl_xor(acc, o, 64);
-            if (lane == 0 && (acc & hun)) atomicOr(&a.sn[hv].next, acc & hun);
+            for (int g = 0; g < kHeavyBatch; ++g) {
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) acc[g] |= __shfl_xor(acc[g], o, 64);
+                if (lane == 0 && (acc[g] & hun[g])) atomicOr(&a.sn[hv[g]].next, acc[g] & hun[g]);
+            }
         }
     }
     // Light rows: a wave owns 64 consecutive vertices (vertices without edges,
@@ -616,19 +636,32 @@ __device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask) {
             const unsigned long long hun = __shfl(un, hl, 64);
             unsigned long long hacc = 0;
             bool open = hrb < hre;
-            for (int64_t b0 = hrb; wave_ballot(open); b0 += 8) {
+            // The first step takes 8 entries (a row often closes on its first
+            // hubs); a row still open after it is usually scanned to its end,
+            // so later steps take kSubStep entries per lane, all loads issued
+            // before any is used.
+            int per = 1;
+            for (int64_t b0 = hrb; wave_ballot(open); b0 += 8 * per, per = kSubStep) {
+                int32_t u[kSubStep];
+                bool ok[kSubStep];
+#pragma unroll
+                for (int j = 0; j < kSubStep; ++j) {
+                    const int64_t i = b0 + j * 8 + sl;
+                    ok[j] = j < per && open && i < hre;
+                    u[j] = ok[j] ? a.cols[i] : 0;
+                }
                 unsigned long long x = 0;
-                const int64_t i = b0 + sl;
-                if (open && i < hre) {
-                    const int32_t u = a.cols[i];
-                    if (u < 0 || u >= a.n) bad = true;
-                    else x = a.fmask[u];
+#pragma unroll
+                for (int j = 0; j < kSubStep; ++j) {
+                    if (!ok[j]) continue;
+                    if (u[j] < 0 || u[j] >= a.n) bad = true;
+                    else x |= a.fmask[u[j]];
                 }
 #pragma unroll
                 for (int o = 4; o > 0; o >>= 1) x |= __shfl_xor(x, o, 8);
                 if (open) {
                     hacc |= x;
-                    open = (hacc & hun) != hun && b0 + 8 < hre;  // same in the 8 lanes of the sub-wave
+                    open = (hacc & hun) != hun && b0 + 8 * per < hre;  // same in the 8 lanes of the sub-wave
                 }
             }
             // The owner of each row takes its sub-wave's result.
