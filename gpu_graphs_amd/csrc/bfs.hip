@@ -124,7 +124,8 @@ struct BfsArgs {
     uint32_t* prev;   // vertices already placed in (or handled by) a frontier queue
     uint32_t* fb[3];  // frontier bitmaps: level L's at fb[L % 3]
     const uint32_t* iso;  // vertices of degree 0 (symmetric graphs), or null
-    const int4* head;     // first kBuFirst neighbours of each vertex, -1 padded (symmetric graphs)
+    const int4* head;     // first kBuFirst neighbours of each vertex, -1 padded, the 4th u stored as -2 - u
+                          // when the row is longer (symmetric graphs)
     int64_t* q_rb[2];
     uint32_t* q_eoff[2];
     int32_t* q_v[2];
@@ -504,7 +505,7 @@ __device__ void bottom_up_level(const BfsArgs& a, int L, BuWaveLds* s_wave, bool
             uint32_t cont = 0;  // bit j: the row goes to the long-row list
 #pragma unroll
             for (int j = 0; j < kBuGroups; ++j) {
-                const int32_t u[kBuFirst] = {hd[j].x, hd[j].y, hd[j].z, hd[j].w};
+                const int32_t u[kBuFirst] = {hd[j].x, hd[j].y, hd[j].z, hd[j].w < -1 ? -2 - hd[j].w : hd[j].w};
                 par[j] = -1;
 #pragma unroll
                 for (int k = 0; k < kBuFirst; ++k)
@@ -1037,18 +1038,25 @@ __global__ __launch_bounds__(kDeepBlock) void bfs_deep_kernel(BfsArgs a, int L) 
                 v = -1;
             }
         }
+        // Rows of at most 4 entries come whole from `head` (no offsets, no
+        // column loads); longer rows, or graphs without `head`, from the CSR.
+        int4 hd = make_int4(-1, -1, -1, -1);
+        if (v >= 0 && a.head) hd = a.head[v];
+        const bool from_csr = v >= 0 && (!a.head || hd.w < -1);
         int64_t rb = 0, re = 0;
-        if (v >= 0) {
+        if (from_csr) {
             rb = a.offsets[v];
             re = a.offsets[v + 1];
         }
-        const uint32_t deg = static_cast<uint32_t>(re - rb);
+        const uint32_t deg = from_csr ? static_cast<uint32_t>(re - rb)
+                                      : static_cast<uint32_t>((hd.x >= 0) + (hd.y >= 0) + (hd.z >= 0) + (hd.w >= 0));
         my_edges += deg;
         const bool short_row = v >= 0 && deg <= kDeepRow;
         int32_t u[kDeepRow];
 #pragma unroll
         for (int j = 0; j < kDeepRow; ++j) {
-            u[j] = (short_row && static_cast<uint32_t>(j) < deg) ? a.cols[rb + j] : -1;
+            if (from_csr) u[j] = (short_row && static_cast<uint32_t>(j) < deg) ? a.cols[rb + j] : -1;
+            else u[j] = j == 0 ? hd.x : j == 1 ? hd.y : j == 2 ? hd.z : j == 3 ? hd.w : -1;
             if (u[j] >= 0 && static_cast<uint64_t>(u[j]) >= n) {
                 bad = true;
                 u[j] = -1;
@@ -1377,14 +1385,16 @@ __global__ __launch_bounds__(kRowSortBlock) void row_sort_kernel(const int64_t* 
     }
 }
 
-// The first 4 entries of every row side by side (-1 past the row's end).
+// The first 4 entries of every row side by side (-1 past the row's end); the
+// fourth, u, is stored as -2 - u (below -1) when the row goes on, so a row of
+// at most 4 entries is complete without its offsets.
 __global__ void row_head_kernel(const int64_t* __restrict__ offsets, const int32_t* __restrict__ cols,
                                 int4* __restrict__ head, int64_t n) {
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
     for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride) {
         const int64_t rb = offsets[v], d = offsets[v + 1] - rb;
         head[v] = make_int4(d > 0 ? cols[rb] : -1, d > 1 ? cols[rb + 1] : -1, d > 2 ? cols[rb + 2] : -1,
-                            d > 3 ? cols[rb + 3] : -1);
+                            d > 4 ? -2 - cols[rb + 3] : d > 3 ? cols[rb + 3] : -1);
     }
 }
 
