@@ -1881,7 +1881,11 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
         // set), when that comes before the end of the first batch.
         const int tail_want = g.cfg.tail >= 0 ? g.cfg.tail : g.cfg.tail == GB_TAIL_AUTO ? tail_hint : -1;
         if (a.fused && g.cfg.tail >= 0) first = std::max(first, g.cfg.tail + 1);
-        const int D = g.cfg.mode == GB_MODE_GRAPH ? std::min(first, kMaxCapturedLevels) : first;
+        // At most kMaxCapturedLevels levels before the first done check, in
+        // either mode: deeper searches go on in pipelined batches, so a
+        // search shallower than the depth hint of a deep graph (thousands of
+        // levels) runs at most one batch of no-op levels past its end.
+        const int D = std::min(first, kMaxCapturedLevels);
         const int tail_at = a.fused && tail_want >= 0 && tail_want < D ? tail_want : -1;
         last_tail_at = tail_at;
         if (g.cfg.mode == GB_MODE_GRAPH) {
