@@ -183,7 +183,6 @@ struct Runner {
     bool last_deep = false;                 // the last search ran in deep mode
     bool last_fused = false;                // the last search ran one launch per level
     int tail_hint = -1;                     // level from which one workgroup finishes the search (-1: none yet)
-    int tail_streak = 0;                    // searches in a row that would take an earlier tail level
     static constexpr int kTailWindow = 64;  // searches the tail level is chosen over
     int tail_seen[kTailWindow] = {};        // where their final runs of small levels began (ring)
     int tail_seen_n = 0;                    // searches recorded so far
