@@ -472,6 +472,7 @@ def served(gg, args, g24=None, dg24=None):
         tickets = [None] * nq
         failures = []
         kept = []  # (graph, source, served levels) of the sampled queries
+        lat_by_graph = [[] for _ in gs]  # total latency (ms) of each completed query, by graph
 
         def releaser():
             # Results are consumed and released as they complete, in any order.
@@ -485,14 +486,16 @@ def served(gg, args, g24=None, dg24=None):
                     out = None
                     if sample_every and i % sample_every == 0:
                         out = np.empty(gs[gi[i]].n, dtype=np.int32)
-                    rc, _ = srv.poll(t, out)
+                    rc, lat = srv.poll(t, out)
                     if rc == gg.GB_EAGAIN:
                         still.append((i, t))
                         continue
                     if rc != 0:
                         failures.append(rc)
-                    elif out is not None:
-                        kept.append((int(gi[i]), src[i], out))
+                    else:
+                        lat_by_graph[gi[i]].append(lat["total_ms"])
+                        if out is not None:
+                            kept.append((int(gi[i]), src[i], out))
                     srv.release(t)
                     finished += 1
                 if len(still) == len(pending):
@@ -514,10 +517,13 @@ def served(gg, args, g24=None, dg24=None):
             checked["compared"] += 1
             if not np.array_equal(lv, dgs[g].bfs(s)[0]):
                 checked["mismatches"] += 1
+        per_graph = {specs[g][0]: {"queries": len(v), "p50_ms": round(float(np.percentile(v, 50)), 2),
+                                   "p99_ms": round(float(np.percentile(v, 99)), 2)}
+                     for g, v in enumerate(lat_by_graph) if v}
         return {"offered_qps": round(rate, 1) if rate else "saturated", "queries": nq,
                 "qps": round(st["qps"], 1), "p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2),
                 "mean_batch": round(st["mean_batch"], 2), "failed": len(failures),
-                "memory_used": st["memory_used"]}
+                "memory_used": st["memory_used"], "per_graph": per_graph}
 
     out = {"graphs": [{"name": s[0], "n": g.n, "m_directed": g.m} for s, g in zip(specs, gs)],
            "seed": SEED_SERVE, "memory_budget": args.serve_budget}
