@@ -414,14 +414,15 @@ __device__ void expand_tile(const BfsArgs& a, int32_t* levels, int32_t* parents,
 //     then costs one round per chunk instead of one pass per 4 groups;
 //   * a round takes kBuGroups list entries per lane and issues the
 //     first-entries, offset and frontier-probe loads of all of them
-//     together (three dependent steps per round);
+//     together (two dependent steps per round);
 //   * every listed lane probes the first kBuFirst neighbours of its vertex,
 //     kept side by side per vertex (`head`, built at upload); the first hit
 //     usually comes within them;
 //   * rows not settled by that go to a per-wave list in LDS, which 8-lane
-//     sub-waves drain: a sub-wave scans 8 consecutive entries of its row per
-//     step, stops at the first hit, and takes the next row from the list as
-//     soon as it is done, so long and short rows share the wave's steps.
+//     sub-waves drain once per chunk (or when it fills up), not once per
+//     round: a sub-wave scans 8 consecutive entries of its row per step,
+//     stops at the first hit, and takes the next row from the list as soon
+//     as it is done, so long and short rows share the wave's steps.
 // The vertices found are collected as bits in LDS; the wave owns its
 // chunk's bitmap words and writes them with plain stores.  The output
 // bitmap arrives zeroed (two levels ahead), so only words with new vertices
@@ -429,14 +430,15 @@ __device__ void expand_tile(const BfsArgs& a, int32_t* levels, int32_t* parents,
 constexpr int kBuChunkGroups = 16;                 // groups of 64 vertices per wave chunk
 constexpr int kBuChunk = kBuChunkGroups * 64;      // vertices per wave chunk
 constexpr int kBuChunkWords = kBuChunkGroups * 2;  // their visited words (lanes 0..31)
+constexpr int kBuList = kBuGroups * 64 + 128;      // long rows listed before the wave drains them
 static_assert(kBuChunkWords <= 64, "one visited word per lane");
+static_assert(kBuList >= kBuGroups * 64, "a round adds at most kBuGroups * 64 long rows");
 struct BuWaveLds {
-    int64_t rb[kBuGroups * 64];    // long rows: next entry of the row
-    uint32_t len[kBuGroups * 64];  // long rows: entries left
-    uint16_t vid[kBuGroups * 64];  // long rows: slot (j * 64 + lane) within the round
-    int32_t par[kBuGroups * 64];   // by slot within the round: parent found by the long-row list, or -1
-    uint16_t unv[kBuChunk];        // the chunk's unvisited vertices (offsets in the chunk)
-    uint32_t found[kBuChunkWords]; // the chunk's vertices found at this level
+    int64_t rb[kBuList];            // long rows: next entry of the row
+    uint32_t len[kBuList];          // long rows: entries left
+    uint16_t vid[kBuList];          // long rows: the vertex (offset in the chunk)
+    uint16_t unv[kBuChunk];         // the chunk's unvisited vertices (offsets in the chunk)
+    uint32_t found[kBuChunkWords];  // the chunk's vertices found at this level
 };
 
 template <int WAVES>
@@ -482,7 +484,69 @@ __device__ void bottom_up_level(const BfsArgs& a, int L, BuWaveLds* s_wave, bool
         }
         if (lane < static_cast<uint32_t>(kBuChunkWords)) sw.found[lane] = 0u;
         wave_lds_sync();
+        // Rows not settled by their first entries wait in the wave's list and
+        // are drained together (once per chunk, unless the list fills up):
+        // 8 rows at a time, one per 8-lane sub-wave, 8 entries per step; a
+        // sub-wave stops at its row's first hit and takes the next row at
+        // once.  The drain stores the found vertices' levels itself.
+        uint32_t nlist = 0;
+        auto drain = [&]() {  // wave-uniform
+            wave_lds_sync();
+            int32_t item = sg < nlist ? static_cast<int32_t>(sg) : -1;
+            int64_t pos = 0, end = 0;
+            if (item >= 0) {
+                pos = sw.rb[item];
+                end = pos + sw.len[item];
+            }
+            uint32_t next = 8;
+            while (wave_ballot(item >= 0)) {
+                const int64_t i = pos + sl;
+                int32_t cu = -1;
+                bool hit = false;
+                if (item >= 0 && i < end) {
+                    cu = a.cols[i];
+                    hit = in_frontier(cu);
+                }
+                const uint64_t hb = wave_ballot(hit);
+                const uint32_t mine8 = static_cast<uint32_t>(hb >> (sg * 8)) & 0xffu;
+                const int src = mine8 ? static_cast<int>(sg * 8 + __ffs(mine8) - 1) : static_cast<int>(lane);
+                const int32_t pu = __shfl(cu, src, 64);
+                bool done = false;
+                if (item >= 0) {
+                    if (mine8) {
+                        if (sl == 0) {
+                            const uint32_t off = sw.vid[item];
+                            const int64_t v = static_cast<int64_t>(c * kBuChunk + off);
+                            levels[v] = L + 1;
+                            if (parents) parents[v] = pu;
+                            atomicOr(&sw.found[off >> 5], 1u << (off & 31u));
+                            if (!enqueue) {  // with `enqueue` the block counts them below
+                                ++my_cnt;
+                                my_edges += static_cast<uint64_t>(sw.len[item]) + kBuFirst;
+                            }
+                        }
+                        done = true;
+                    } else {
+                        pos += 8;
+                        done = pos >= end;
+                    }
+                }
+                const uint64_t need = wave_ballot(done && sl == 0);
+                if (done) {
+                    const uint32_t t = next + static_cast<uint32_t>(__popcll(need & ((1ull << (sg * 8)) - 1)));
+                    item = t < nlist ? static_cast<int32_t>(t) : -1;
+                    if (item >= 0) {
+                        pos = sw.rb[item];
+                        end = pos + sw.len[item];
+                    }
+                }
+                next += static_cast<uint32_t>(__popcll(need));
+            }
+            wave_lds_sync();  // the list is refilled from slot 0
+            nlist = 0;
+        };
         for (uint32_t r0 = 0; r0 < U; r0 += kBuGroups * 64) {
+            if (nlist > static_cast<uint32_t>(kBuList - kBuGroups * 64)) drain();
             int64_t vtx[kBuGroups];
             int64_t rb[kBuGroups], re[kBuGroups];
             int32_t par[kBuGroups];
@@ -501,8 +565,8 @@ __device__ void bottom_up_level(const BfsArgs& a, int L, BuWaveLds* s_wave, bool
                     re[j] = a.offsets[vtx[j] + 1];
                 }
             }
-            // Step 2: probes of the first neighbours.
-            uint32_t cont = 0;  // bit j: the row goes to the long-row list
+            // Step 2: probes of the first neighbours; rows still open go to
+            // the long-row list, the others are settled here.
 #pragma unroll
             for (int j = 0; j < kBuGroups; ++j) {
                 const int32_t u[kBuFirst] = {hd[j].x, hd[j].y, hd[j].z, hd[j].w < -1 ? -2 - hd[j].w : hd[j].w};
@@ -510,73 +574,16 @@ __device__ void bottom_up_level(const BfsArgs& a, int L, BuWaveLds* s_wave, bool
 #pragma unroll
                 for (int k = 0; k < kBuFirst; ++k)
                     if (par[j] < 0 && u[k] >= 0 && in_frontier(u[k])) par[j] = u[k];
-                if (vtx[j] >= 0 && par[j] < 0 && re[j] - rb[j] > kBuFirst) cont |= 1u << j;
-            }
-            // Step 3: the rows still open, drained by the 8 sub-waves.
-            uint32_t nitems = 0;
-#pragma unroll
-            for (int j = 0; j < kBuGroups; ++j) {
-                const bool cc = (cont >> j) & 1u;
+                const bool cc = vtx[j] >= 0 && par[j] < 0 && re[j] - rb[j] > kBuFirst;
                 const uint64_t m = wave_ballot(cc);
                 if (cc) {
-                    const uint32_t slot = nitems + rank_below(m);
+                    const uint32_t slot = nlist + rank_below(m);
                     sw.rb[slot] = rb[j] + kBuFirst;
                     sw.len[slot] = static_cast<uint32_t>(re[j] - rb[j] - kBuFirst);
-                    sw.vid[slot] = static_cast<uint16_t>(j * 64 + lane);
-                    sw.par[j * 64 + lane] = -1;
+                    sw.vid[slot] = static_cast<uint16_t>(vtx[j] - static_cast<int64_t>(c * kBuChunk));
                 }
-                nitems += static_cast<uint32_t>(__popcll(m));
-            }
-            if (nitems) {
-                wave_lds_sync();
-                int32_t item = sg < nitems ? static_cast<int32_t>(sg) : -1;
-                int64_t pos = 0, end = 0;
-                if (item >= 0) {
-                    pos = sw.rb[item];
-                    end = pos + sw.len[item];
-                }
-                uint32_t next = 8;
-                while (wave_ballot(item >= 0)) {
-                    const int64_t i = pos + sl;
-                    int32_t cu = -1;
-                    bool hit = false;
-                    if (item >= 0 && i < end) {
-                        cu = a.cols[i];
-                        hit = in_frontier(cu);
-                    }
-                    const uint64_t hb = wave_ballot(hit);
-                    const uint32_t mine8 = static_cast<uint32_t>(hb >> (sg * 8)) & 0xffu;
-                    const int src = mine8 ? static_cast<int>(sg * 8 + __ffs(mine8) - 1) : static_cast<int>(lane);
-                    const int32_t pu = __shfl(cu, src, 64);
-                    bool done = false;
-                    if (item >= 0) {
-                        if (mine8) {
-                            if (sl == 0) sw.par[sw.vid[item]] = pu;
-                            done = true;
-                        } else {
-                            pos += 8;
-                            done = pos >= end;
-                        }
-                    }
-                    const uint64_t need = wave_ballot(done && sl == 0);
-                    if (done) {
-                        const uint32_t t = next + static_cast<uint32_t>(__popcll(need & ((1ull << (sg * 8)) - 1)));
-                        item = t < nitems ? static_cast<int32_t>(t) : -1;
-                        if (item >= 0) {
-                            pos = sw.rb[item];
-                            end = pos + sw.len[item];
-                        }
-                    }
-                    next += static_cast<uint32_t>(__popcll(need));
-                }
-                wave_lds_sync();
-            }
-            // Step 4: levels, parents, and the bits of the vertices found.
-#pragma unroll
-            for (int j = 0; j < kBuGroups; ++j) {
-                if ((cont >> j) & 1u) par[j] = sw.par[j * 64 + lane];
-                const bool found = vtx[j] >= 0 && par[j] >= 0;
-                if (found) {
+                nlist += static_cast<uint32_t>(__popcll(m));
+                if (vtx[j] >= 0 && par[j] >= 0) {
                     const int64_t v = vtx[j];
                     levels[v] = L + 1;
                     if (parents) parents[v] = par[j];
@@ -588,8 +595,8 @@ __device__ void bottom_up_level(const BfsArgs& a, int L, BuWaveLds* s_wave, bool
                     }
                 }
             }
-            if (nitems) wave_lds_sync();  // the long-row list of the next round reuses the LDS
         }
+        if (nlist) drain();
         wave_lds_sync();
         if (lane < static_cast<uint32_t>(kBuChunkWords) && w < a.nwords) {
             const uint32_t bits = sw.found[lane];
