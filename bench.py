@@ -493,7 +493,7 @@ def served(gg, args, g24=None, dg24=None):
                     if rc != 0:
                         failures.append(rc)
                     else:
-                        lat_by_graph[gi[i]].append(lat["total_ms"])
+                        lat_by_graph[gi[i]].append((lat["total_ms"], lat["queue_ms"], lat["run_ms"]))
                         if out is not None:
                             kept.append((int(gi[i]), src[i], out))
                     srv.release(t)
@@ -517,8 +517,11 @@ def served(gg, args, g24=None, dg24=None):
             checked["compared"] += 1
             if not np.array_equal(lv, dgs[g].bfs(s)[0]):
                 checked["mismatches"] += 1
-        per_graph = {specs[g][0]: {"queries": len(v), "p50_ms": round(float(np.percentile(v, 50)), 2),
-                                   "p99_ms": round(float(np.percentile(v, 99)), 2)}
+        def pct(v, k, q):
+            return round(float(np.percentile([x[k] for x in v], q)), 2)
+        # total = queue (arrival -> batch start) + run (batch start -> done)
+        per_graph = {specs[g][0]: {"queries": len(v), "p50_ms": pct(v, 0, 50), "p99_ms": pct(v, 0, 99),
+                                   "queue_p99_ms": pct(v, 1, 99), "run_p99_ms": pct(v, 2, 99)}
                      for g, v in enumerate(lat_by_graph) if v}
         return {"offered_qps": round(rate, 1) if rate else "saturated", "queries": nq,
                 "qps": round(st["qps"], 1), "p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2),

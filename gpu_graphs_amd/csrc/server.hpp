@@ -45,7 +45,7 @@ struct ServerConfig {
     bool plain = false;            // one single-source BFS per query, no batching
     int64_t memory_budget = 0;     // bytes of device memory for workspaces + results
     int64_t queue_capacity = 1 << 20;
-    double long_batch_ms = 20.0;   // batches expected to run longer hold at most lanes - 1 lanes
+    double long_batch_ms = 10.0;   // batches expected to run longer hold at most lanes - 1 lanes
 };
 
 class Executor {
