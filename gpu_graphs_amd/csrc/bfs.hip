@@ -1412,10 +1412,12 @@ __global__ void row_head_kernel(const int64_t* __restrict__ offsets, const int32
 //  * level 0 = the source alone: claimed, level 0, parent = itself, the only
 //    queue entry, and the control block reset.  The thread that clears an
 //    entry of the source also makes the source's store to it.
-__global__ void bfs_clear_kernel(BfsArgs a) {
-    // The call's parameters come from the host-visible block (one read per
-    // workgroup); the later kernels read the device copy made here.
-    const BfsParams prm = *a.hp;
+__global__ void bfs_clear_kernel(BfsArgs a, BfsParams arg) {
+    // The call's parameters: `arg` for a direct launch, else (a replayed
+    // sequence, whose arguments are fixed) the host-visible block, one
+    // scalar read per wave.  The later kernels read the device copy made
+    // here.
+    const BfsParams prm = arg.levels ? arg : *a.hp;
     const int32_t s = prm.source;
     int32_t* const levels = prm.levels;
     int32_t* const parents = prm.parents;
@@ -1782,15 +1784,19 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
         hipLaunchKernelGGL(ks->expand, dim3(grid), dim3(g.cfg.block), 0, stream, a, L, bu, bu1, 0);
         hipLaunchKernelGGL(bfs_compact_kernel, dim3(cgrid), dim3(kCompactBlock), 0, stream, a, L, bu1, bu2);
     };
-    // The call's parameters: the host-visible block the clear kernel reads
-    // when the stream reaches it (also inside a replayed graph; the previous
-    // call has ended, so nothing reads the block meanwhile).
+    // The call's parameters: kernel arguments of a direct launch of the
+    // clear kernel, or the host-visible block it reads when a replayed graph
+    // reaches it (the previous call has ended, so nothing reads the block
+    // meanwhile).
     ws.h_params->source = source;
     ws.h_params->pad = 0;
     ws.h_params->levels = d_levels;
     ws.h_params->parents = d_parents;
-    auto launch_prologue = [&]() {
-        hipLaunchKernelGGL(bfs_clear_kernel, dim3(clear_grid), dim3(256), 0, stream, a);
+    // A captured launch takes its parameters from ws.h_params when it runs.
+    auto launch_prologue = [&](bool captured) {
+        BfsParams arg = *ws.h_params;
+        if (captured) arg.levels = nullptr;
+        hipLaunchKernelGGL(bfs_clear_kernel, dim3(clear_grid), dim3(256), 0, stream, a, arg);
     };
 
     int L = 0;
@@ -1855,7 +1861,7 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     if (g.cfg.mode == GB_MODE_LEVEL_LOOP) {
         // One level at a time; the host reads the next frontier size.
         GG_HIP(hipEventRecord(ev_start, stream));
-        launch_prologue();
+        launch_prologue(false);
         GG_HIP(hipGetLastError());
         GG_HIP(hipEventRecord(ev_kernel, stream));
         for (;; ++L) {
@@ -1910,7 +1916,7 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
                 hipGraph_t captured = nullptr;
                 GG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
                 try {
-                    launch_prologue();
+                    launch_prologue(true);
                     launch_levels(D, tail_at, 0);
                     GG_HIP(hipGetLastError());
                 } catch (...) {
@@ -1946,7 +1952,7 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
             // kernels a replayed graph costs more device and host time here
             // than the launches themselves.
             GG_HIP(hipEventRecord(ev_start, stream));
-            launch_prologue();
+            launch_prologue(false);
             enqueue(D, 0, tail_at, true);
         }
         timed_traversal = false;
