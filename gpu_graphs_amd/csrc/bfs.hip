@@ -116,6 +116,7 @@ constexpr uint32_t kTailMaxEdges = 1u << 15;                // CSR entries of a 
 constexpr uint32_t kTailLearnEdges = 4096;                  // levels this small count toward the tail start
 constexpr int kTailK = 4;                                   // edges per thread per pass
 constexpr int kNoTail = 1 << 20;                            // learned tail start when the last level was large
+constexpr uint32_t kGrowDenseEdges = 4096;                  // dense threshold of the levels before the first bottom-up one
 
 struct BfsArgs {
     const int64_t* __restrict__ offsets;
@@ -143,6 +144,7 @@ struct BfsArgs {
     uint32_t fused;             // one launch per level (symmetric graphs, not deep mode)
     int64_t n_edged;            // vertices with at least one CSR entry
     uint32_t alpha, beta;
+    int32_t grow_until;         // levels L with L+1 <= grow_until count as dense from kGrowDenseEdges
 };
 
 // Beamer's test for a frontier of E CSR entries after `explored` entries of
@@ -718,8 +720,12 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(BfsArgs a, int L, int
     const unsigned long long dir = choose_dir(a, L, F, E, F_prev, E_prev, bu_launched != 0);
     // A dense level leaves level L+1 in the bitmaps only.  Without compact()
     // (one-launch levels) that is allowed only if level L+1 may go bottom-up,
-    // which it then must.
-    const bool dense = !deep && E > a.dense_edges && (!a.fused || bu_next);
+    // which it then must.  Up to the first level that searches of this graph
+    // took bottom-up (grow_until), levels grow fast enough that a smaller
+    // frontier already counts as dense: the next level goes bottom-up anyway,
+    // and skipping its queue shortens this level's chain.
+    const uint32_t dense_min = (a.fused && L + 1 <= a.grow_until) ? kGrowDenseEdges : a.dense_edges;
+    const bool dense = !deep && E > dense_min && (!a.fused || bu_next);
     // A one-launch bottom-up level also queues level L+1 when that level
     // cannot go bottom-up, or may go back top-down: the frontier shrinks, or
     // so few vertices are left that level L+1 is below n / beta whatever it
@@ -1722,6 +1728,7 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     a.nwords = static_cast<uint32_t>((n + 31) / 32);
     a.nchunks = static_cast<uint32_t>(ws.nchunks);
     a.dense_edges = static_cast<uint32_t>(std::min<int64_t>(g.cfg.dense_edges, 0xffffffffll));
+    a.grow_until = bu_learned ? bu_lo + 1 : 0;  // the first learned bottom-up level (the window has a margin of one)
     a.tile_edges = static_cast<uint32_t>(g.cfg.block * g.cfg.edges_per_thread);
     a.bottom_up = (g.symmetric && g.cfg.direction_opt) ? 1u : 0u;
     a.alpha = static_cast<uint32_t>(g.cfg.alpha);
