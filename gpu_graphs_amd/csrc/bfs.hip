@@ -1600,10 +1600,18 @@ void DeviceGraph::release() {
     head = nullptr;
 }
 
-Runner::Runner(DeviceGraph& graph) : g(graph), device(graph.device), n(graph.n), m(graph.m) {
+Runner::Runner(DeviceGraph& graph) : Runner(graph, nullptr) {}
+
+Runner::Runner(DeviceGraph& graph, hipStream_t lane_stream)
+    : g(graph), device(graph.device), n(graph.n), m(graph.m) {
     DeviceGuard guard(device);
     try {
-        GG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        if (lane_stream) {
+            stream = lane_stream;
+            own_stream = false;
+        } else {
+            GG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        }
         GG_HIP(hipEventCreate(&ev_start));
         GG_HIP(hipEventCreate(&ev_stop));
         GG_HIP(hipEventCreate(&ev_kernel));
@@ -1635,7 +1643,7 @@ void Runner::release() {
         if (*e) (void)hipEventDestroy(*e);
         *e = nullptr;
     }
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream && own_stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
 }
 

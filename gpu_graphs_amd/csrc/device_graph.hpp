@@ -167,6 +167,7 @@ struct Runner {
     const int device;
     const int64_t n, m;
     hipStream_t stream = nullptr;
+    bool own_stream = true;  // created (and destroyed) by the runner, else the caller's
     hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_kernel = nullptr;
     hipEvent_t ev_batch[2] = {nullptr, nullptr};
     static constexpr int kCtlWords = 32;
@@ -194,6 +195,9 @@ struct Runner {
     MsWorkspace ms;
 
     explicit Runner(DeviceGraph& graph);
+    // A runner that launches on the caller's stream (which must outlive it):
+    // the query server gives all runners of one lane the lane's stream.
+    Runner(DeviceGraph& graph, hipStream_t lane_stream);
     ~Runner();
     Runner(const Runner&) = delete;
     Runner& operator=(const Runner&) = delete;

@@ -1,9 +1,13 @@
 // HIP executor of the query server and its C ABI (gb_server_*).
 //
-// Lane i owns one Runner (stream + workspaces) per graph it has served, so
-// batches on different lanes -- on the same graph or on different graphs --
-// run concurrently on the device.  Result buffers are plain device
-// allocations; the server pools and budgets them.
+// Lane i owns one stream, created with the others when the server starts,
+// and one Runner (workspaces) per graph it has served, launching on the
+// lane's stream; so batches on different lanes -- on the same graph or on
+// different graphs -- run concurrently on the device.  (A stream per runner
+// spread the lanes over the device's few hardware queues at random: two
+// lanes that shared a queue ran one after the other, and a 0.2 ms R-MAT
+// search waited behind a 400 ms grid batch.)  Result buffers are plain
+// device allocations; the server pools and budgets them.
 //
 // A batch of k queries runs either as one multi-source BFS or as k
 // single-source BFS, whichever the measured per-graph costs say is cheaper:
@@ -26,14 +30,20 @@ namespace gg {
 
 class HipExecutor : public Executor {
    public:
-    HipExecutor(int device, int lanes) : device_(device), runners_(lanes) {
+    HipExecutor(int device, int lanes) : device_(device), runners_(lanes), lane_streams_(lanes, nullptr) {
         DeviceGuard guard(device_);
-        GG_HIP(hipStreamCreateWithFlags(&fetch_stream_, hipStreamNonBlocking));
+        try {
+            GG_HIP(hipStreamCreateWithFlags(&fetch_stream_, hipStreamNonBlocking));
+            for (hipStream_t& s : lane_streams_) GG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        } catch (...) {
+            release_streams();
+            throw;
+        }
     }
     ~HipExecutor() override {
         (void)hipSetDevice(device_);
-        runners_.clear();
-        if (fetch_stream_) (void)hipStreamDestroy(fetch_stream_);
+        runners_.clear();  // each runner synchronises its lane's stream first
+        release_streams();
     }
 
     int add(DeviceGraph* g) {
@@ -80,7 +90,7 @@ class HipExecutor : public Executor {
     void run(int lane, int gid, const int32_t* sources, int k, int32_t* result, bool plain) override {
         DeviceGraph* g = graph(gid);
         std::unique_ptr<Runner>& r = runners_[lane][gid];  // only this lane's thread touches runners_[lane]
-        if (!r) r = std::make_unique<Runner>(*g);
+        if (!r) r = std::make_unique<Runner>(*g, lane_streams_[lane]);
         if (plain || prefer_single(gid, k)) {
             double total = 0;
             for (int i = 0; i < k; ++i) {
@@ -164,11 +174,21 @@ class HipExecutor : public Executor {
         return graphs_[gid];
     }
 
+    void release_streams() noexcept {
+        for (hipStream_t& s : lane_streams_) {
+            if (s) (void)hipStreamDestroy(s);
+            s = nullptr;
+        }
+        if (fetch_stream_) (void)hipStreamDestroy(fetch_stream_);
+        fetch_stream_ = nullptr;
+    }
+
     int device_;
     std::mutex mu_, fetch_mu_, cost_mu_;
     std::map<int, Cost> cost_;
     std::vector<DeviceGraph*> graphs_;
     std::vector<std::map<int, std::unique_ptr<Runner>>> runners_;
+    std::vector<hipStream_t> lane_streams_;  // one per lane, shared by its runners
     hipStream_t fetch_stream_ = nullptr;
 };
 
