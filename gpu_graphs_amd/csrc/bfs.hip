@@ -96,6 +96,8 @@ enum : int {
     CTL_DEEPE0 = 26,   // 2 words, deep levels: CSR entries of level L's frontier at deepe[L & 1]
     CTL_TAIL = 28,     // what the tail kernel did: GB_TAIL_* outcome | (level << 8)
     CTL_TAILFROM = 29, // first level of the current run of small queued levels (~0: none)
+    CTL_T0 = 30,       // device clock (wall_clock64) when the clear kernel started
+    CTL_T1 = 31,       // in a stored snapshot only: device clock when it was taken
     CTL_WORDS = 32
 };
 enum : unsigned long long { ERR_TILE = 1, ERR_RANGE = 2 };
@@ -1321,11 +1323,15 @@ __device__ void tail_levels(const BfsArgs& a, int T) {
 // sequence holds no copy node (a copy is a separate blit dispatch, and in a
 // replayed graph it also delays the submission of the kernels behind it).
 // Every thread of the workgroup calls this after its last update of ctl.
+// The snapshot's CTL_T1 word is the device clock at the store: with CTL_T0
+// it times the sequence on the device without events.
 __device__ __forceinline__ void snapshot_ctl(unsigned long long* ctl, unsigned long long* out) {
     __threadfence();
     __syncthreads();
     if (threadIdx.x < CTL_WORDS)
-        out[threadIdx.x] = __hip_atomic_load(ctl + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        out[threadIdx.x] = threadIdx.x == CTL_T1
+                               ? static_cast<unsigned long long>(wall_clock64())
+                               : __hip_atomic_load(ctl + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // `snap` (host-visible, may be null): where the final control block goes.
@@ -1337,7 +1343,8 @@ __global__ __launch_bounds__(kTailBlock) void bfs_tail_kernel(BfsArgs a, int T, 
 // The control block to host-visible memory when the sequence does not end
 // with the tail kernel (earlier kernels' updates are complete at its start).
 __global__ __launch_bounds__(64) void bfs_snapshot_kernel(const unsigned long long* ctl, unsigned long long* out) {
-    if (threadIdx.x < CTL_WORDS) out[threadIdx.x] = ctl[threadIdx.x];
+    if (threadIdx.x < CTL_WORDS)
+        out[threadIdx.x] = threadIdx.x == CTL_T1 ? static_cast<unsigned long long>(wall_clock64()) : ctl[threadIdx.x];
 }
 
 // Upload of a symmetric graph: each row's neighbours ordered by descending
@@ -1423,6 +1430,7 @@ __global__ void bfs_clear_kernel(BfsArgs a, BfsParams arg) {
     // sequence, whose arguments are fixed) the host-visible block, one
     // scalar read per wave.  The later kernels read the device copy made
     // here.
+    const unsigned long long t0 = static_cast<unsigned long long>(wall_clock64());
     const BfsParams prm = arg.levels ? arg : *a.hp;
     const int32_t s = prm.source;
     int32_t* const levels = prm.levels;
@@ -1478,6 +1486,7 @@ __global__ void bfs_clear_kernel(BfsArgs a, BfsParams arg) {
         a.ctl[CTL_REP0] = REP_QUEUE;
         a.ctl[CTL_BUFIRST] = ~0ull;
         a.ctl[CTL_TAILFROM] = ~0ull;
+        a.ctl[CTL_T0] = t0;
     }
     for (uint64_t tt = threadIdx.x; tt * a.tile_edges < d; tt += blockDim.x) a.tile_start[0][tt] = 0;
 }
@@ -1623,6 +1632,10 @@ Runner::Runner(DeviceGraph& graph, hipStream_t lane_stream)
         GG_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_ctl), sizeof(unsigned long long) * 2 * kCtlWords,
                              hipHostMallocCoherent | hipHostMallocMapped));
         GG_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&h_ctl_dev), h_ctl, 0));
+        int khz = 0;
+        GG_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
+        if (khz <= 0) fail(GB_EDEVICE, "device reports no wall clock rate");
+        wall_khz = khz;
     } catch (...) {
         release();
         throw;
@@ -1872,6 +1885,7 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     };
 
     bool timed_traversal = true;  // ev_kernel marks the end of the clears
+    bool clock_timed = false;     // timed by the device clock stamps, not by ev_start / ev_stop
     last_tail_at = -1;
     if (g.cfg.mode == GB_MODE_LEVEL_LOOP) {
         // One level at a time; the host reads the next frontier size.
@@ -1965,10 +1979,13 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
         } else {
             // Direct launches (the default): for a sequence of about ten
             // kernels a replayed graph costs more device and host time here
-            // than the launches themselves.
-            GG_HIP(hipEventRecord(ev_start, stream));
+            // than the launches themselves.  No start event: recording one
+            // ahead of the clear kernel kept the GPU idle for its host time
+            // (about 4 us per search); the device time comes from the clock
+            // stamps of the first and the last kernel instead.
             launch_prologue(false);
             enqueue(D, 0, tail_at, true);
+            clock_timed = true;
         }
         timed_traversal = false;
         if (tail_at >= 0 && !done(0)) {
@@ -1996,7 +2013,14 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     last_bu_last = last_bu_first >= 0 ? int(fin[CTL_BULAST]) : -1;
     learn_from_search(static_cast<int64_t>(fin[CTL_MAXE]) > static_cast<int64_t>(a.dense_edges), fin[CTL_TAILFROM]);
     float ms = 0.f;
-    GG_HIP(hipEventElapsedTime(&ms, ev_start, ev_stop));
+    if (clock_timed) {
+        // The clear kernel's start to the final snapshot, at the device
+        // clock's rate (kHz: ticks per ms).
+        const int64_t ticks = static_cast<int64_t>(fin[CTL_T1] - fin[CTL_T0]);
+        ms = ticks > 0 ? static_cast<float>(static_cast<double>(ticks) / wall_khz) : 0.f;
+    } else {
+        GG_HIP(hipEventElapsedTime(&ms, ev_start, ev_stop));
+    }
     last_ms = ms;
     if (st) {
         float kms = ms;

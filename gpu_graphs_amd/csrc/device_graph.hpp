@@ -211,6 +211,7 @@ struct Runner {
     // device buffer of k rows of n int32.
     void bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_batch_stats* st);
     double last_ms = 0;  // device time of the last bfs / bfs_batch call
+    double wall_khz = 0; // rate of the device clock (wall_clock64) in kHz
     void bfs_batch_to_host(const int32_t* h_sources, int k, int32_t* h_levels, gb_batch_stats* st);
     // Allocates the batch workspace now (the server charges it to its budget).
     void prepare_batch();
