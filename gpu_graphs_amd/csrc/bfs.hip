@@ -1323,15 +1323,26 @@ __device__ void tail_levels(const BfsArgs& a, int T) {
 // sequence holds no copy node (a copy is a separate blit dispatch, and in a
 // replayed graph it also delays the submission of the kernels behind it).
 // Every thread of the workgroup calls this after its last update of ctl.
-// The snapshot's CTL_T1 word is the device clock at the store: with CTL_T0
-// it times the sequence on the device without events.
+// The snapshot's CTL_T1 word is the device clock at the store, stored last
+// (never 0): the host, which cleared it before the launch, polls it to learn
+// that the snapshot is complete, and with CTL_T0 it times the sequence on the
+// device without events.  Every thread of the workgroup calls this after its
+// stores to `out`.
+__device__ __forceinline__ void store_stamp(unsigned long long* out) {
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long t = static_cast<unsigned long long>(wall_clock64());
+        __hip_atomic_store(out + CTL_T1, t ? t : 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 __device__ __forceinline__ void snapshot_ctl(unsigned long long* ctl, unsigned long long* out) {
     __threadfence();
     __syncthreads();
-    if (threadIdx.x < CTL_WORDS)
-        out[threadIdx.x] = threadIdx.x == CTL_T1
-                               ? static_cast<unsigned long long>(wall_clock64())
-                               : __hip_atomic_load(ctl + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x < CTL_WORDS && threadIdx.x != CTL_T1)
+        out[threadIdx.x] = __hip_atomic_load(ctl + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    store_stamp(out);
 }
 
 // `snap` (host-visible, may be null): where the final control block goes.
@@ -1343,8 +1354,8 @@ __global__ __launch_bounds__(kTailBlock) void bfs_tail_kernel(BfsArgs a, int T, 
 // The control block to host-visible memory when the sequence does not end
 // with the tail kernel (earlier kernels' updates are complete at its start).
 __global__ __launch_bounds__(64) void bfs_snapshot_kernel(const unsigned long long* ctl, unsigned long long* out) {
-    if (threadIdx.x < CTL_WORDS)
-        out[threadIdx.x] = threadIdx.x == CTL_T1 ? static_cast<unsigned long long>(wall_clock64()) : ctl[threadIdx.x];
+    if (threadIdx.x < CTL_WORDS && threadIdx.x != CTL_T1) out[threadIdx.x] = ctl[threadIdx.x];
+    store_stamp(out);
 }
 
 // Upload of a symmetric graph: each row's neighbours ordered by descending
@@ -1846,25 +1857,33 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
         hipLaunchKernelGGL(bfs_snapshot_kernel, dim3(1), dim3(64), 0, stream,
                            static_cast<const unsigned long long*>(ws.ctl), snap);
     };
-    // The event that follows the batch of levels whose snapshot is in slot i.
-    // The first batch is followed by ev_stop alone (an event record costs a
-    // few microseconds of the call's wall time); later batches by
-    // ev_batch[slot] as well, which stays valid while ev_stop moves on.
-    hipEvent_t batch_ev[2] = {ev_batch[0], ev_batch[1]};
-    auto enqueue = [&](int count, int slot, int tail_at = -1, bool first = false) {
+    bool clock_timed = false;  // timed by the device clock stamps, not by ev_start / ev_stop
+    // The host waits for a batch of levels by polling the end stamp of its
+    // snapshot (cleared here before the batch is enqueued, stored last by the
+    // kernel): an event would add its record and signal latency, a few
+    // microseconds each.  The stream is queried now and then, so that a
+    // failed launch ends the wait with its error.
+    auto arm = [&](int slot) { __atomic_store_n(&h_ctl[slot * kCtlWords + CTL_T1], 0ull, __ATOMIC_RELEASE); };
+    auto wait_snapshot = [&](int slot) {
+        const unsigned long long* t1 = &h_ctl[slot * kCtlWords + CTL_T1];
+        for (uint32_t it = 1; __atomic_load_n(t1, __ATOMIC_ACQUIRE) == 0; ++it) {
+            if (it % 4096 != 0) continue;
+            const hipError_t r = hipStreamQuery(stream);
+            if (r == hipErrorNotReady) continue;
+            GG_HIP(r);
+            if (__atomic_load_n(t1, __ATOMIC_ACQUIRE) == 0)
+                fail(GB_EDEVICE, "internal error: the launched levels stored no control block");
+        }
+    };
+    auto enqueue = [&](int count, int slot, int tail_at = -1) {
+        arm(slot);
         launch_levels(count, tail_at, slot);
         GG_HIP(hipGetLastError());
-        GG_HIP(hipEventRecord(ev_stop, stream));
-        if (first) {
-            batch_ev[slot] = ev_stop;
-        } else {
-            GG_HIP(hipEventRecord(ev_batch[slot], stream));
-            batch_ev[slot] = ev_batch[slot];
-        }
+        if (!clock_timed) GG_HIP(hipEventRecord(ev_stop, stream));  // for the timing only
         last_slot = slot;
     };
     auto done = [&](int slot) {
-        GG_HIP(hipEventSynchronize(batch_ev[slot]));
+        wait_snapshot(slot);
         return h_ctl[slot * kCtlWords + CTL_DONE] != 0 || h_ctl[slot * kCtlWords + CTL_ERR] != 0;
     };
     // Deeper than the first batch: batches of doubling size (at most 64
@@ -1885,7 +1904,6 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     };
 
     bool timed_traversal = true;  // ev_kernel marks the end of the clears
-    bool clock_timed = false;     // timed by the device clock stamps, not by ev_start / ev_stop
     last_tail_at = -1;
     if (g.cfg.mode == GB_MODE_LEVEL_LOOP) {
         // One level at a time; the host reads the next frontier size.
@@ -1970,10 +1988,10 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
                 ws.g_deep = int(deep);
                 ws.g_tail = tail_at;
             }
+            arm(0);
             GG_HIP(hipEventRecord(ev_start, stream));
             GG_HIP(hipGraphLaunch(ws.graph_exec, stream));
             GG_HIP(hipEventRecord(ev_stop, stream));
-            batch_ev[0] = ev_stop;
             L = tail_at >= 0 ? tail_at + 1 : D;  // the level after the captured launches
             last_slot = 0;
         } else {
@@ -1983,9 +2001,9 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
             // ahead of the clear kernel kept the GPU idle for its host time
             // (about 4 us per search); the device time comes from the clock
             // stamps of the first and the last kernel instead.
-            launch_prologue(false);
-            enqueue(D, 0, tail_at, true);
             clock_timed = true;
+            launch_prologue(false);
+            enqueue(D, 0, tail_at);
         }
         timed_traversal = false;
         if (tail_at >= 0 && !done(0)) {
@@ -2001,7 +2019,8 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
             }
         }
         continue_levels();
-        GG_HIP(hipEventSynchronize(batch_ev[last_slot]));
+        wait_snapshot(last_slot);
+        if (!clock_timed) GG_HIP(hipEventSynchronize(ev_stop));
     }
     const unsigned long long* fin = h_ctl + last_slot * kCtlWords;
     if (fin[CTL_ERR] & ERR_TILE) fail(GB_EDEVICE, "internal error: inconsistent frontier tile index");
