@@ -89,6 +89,7 @@ constexpr int kMaxCapturedLevels = 32;  // deeper searches continue with direct 
 constexpr int kByteLevels = 254;         // levels kept in the byte lines; 254 = stored directly, 255 = none
 constexpr int kNibLevels = 14;           // levels kept in the nibble lines; 14 = stored directly, 15 = none
 constexpr int kMaxLevelBatch = 128;     // levels enqueued between done checks
+constexpr int64_t kHubLdsMinVertices = int64_t(1) << 21;  // bottom-up hub masks in LDS from this many vertices
 
 // Per vertex, side by side: a top-down probe reads both in one 16-byte load
 // (one cache line instead of two).
@@ -132,6 +133,7 @@ struct MsArgs {
     uint32_t low_degree;         // bottom-up: longer rows are scanned by the whole wave
     uint32_t beta;               // bottom-up only while F * beta >= nz (0: no such limit)
     uint32_t nib;                // level lines of nibbles (shallow searches), else of bytes
+    uint32_t hub_lds;            // bottom-up levels keep the masks of the first vertices in LDS
 };
 
 template <int BLOCK, int K>
@@ -140,6 +142,18 @@ struct MsTileLds {
     uint32_t eoff[T + 2];
     int64_t rb[T + 1];
     unsigned long long fm[T + 1];
+};
+
+// Bottom-up levels: the masks of the first kHubPerThread * BLOCK vertices
+// (the highest degrees: at R-MAT s24 the first 8,192 of 16.8 M vertices take
+// 20% of all row entries) are copied into LDS at the start of the level, so
+// that those gathers do not go to the L2.  8 KB per wave.
+constexpr int kHubPerThread = 16;
+
+template <int BLOCK, int K>
+union MsExpandLds {
+    MsTileLds<BLOCK, K> td;
+    unsigned long long hub[BLOCK * kHubPerThread];
 };
 
 // seen = next = 0; no vertex reached (the level lines are left as they are).
@@ -508,9 +522,11 @@ __global__ __launch_bounds__(kUcBlock) void ms_update_compact_kernel(MsArgs a, i
 constexpr int kSubStep = 4;     // bottom-up long rows: entries per lane in a sub-wave's later steps
 constexpr int kHeavyBatch = 4;  // bottom-up heavy rows: tiles a wave scans together
 
-__device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask) {
+__device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask, const unsigned long long* hub, uint32_t nhub) {
     const uint32_t lane = threadIdx.x & 63u;
     bool bad = false;
+    // Vertex u's mask (u in [0, n)): from the LDS copy for the first nhub.
+    auto mask_of = [&](int32_t u) { return static_cast<uint32_t>(u) < nhub ? hub[u] : a.fmask[u]; };
     // Heavy rows (a prefix of the degree-ordered ids): tiles of kMsHeavyRow
     // entries spread over every wave, so the longest rows do not hold up a
     // few waves.  Tiles are numbered round by round (tile r of every row
@@ -573,7 +589,7 @@ __device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask) {
                 for (int j = 0; j < kMsHeavyRow / 64; ++j) {
                     if (u[g][j] < 0) continue;
                     if (u[g][j] >= a.n) bad = true;
-                    else acc[g] |= a.fmask[u[g][j]];
+                    else acc[g] |= mask_of(u[g][j]);
                 }
             }
 #pragma unroll
@@ -614,7 +630,7 @@ __device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask) {
                         bad = true;
                         continue;
                     }
-                    acc |= a.fmask[u[j]];
+                    acc |= mask_of(u[j]);
                 }
                 if ((acc & un) == un) break;
             }
@@ -655,7 +671,7 @@ __device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask) {
                 for (int j = 0; j < kSubStep; ++j) {
                     if (!ok[j]) continue;
                     if (u[j] < 0 || u[j] >= a.n) bad = true;
-                    else x |= a.fmask[u[j]];
+                    else x |= mask_of(u[j]);
                 }
 #pragma unroll
                 for (int o = 4; o > 0; o >>= 1) x |= __shfl_xor(x, o, 8);
@@ -681,7 +697,8 @@ __device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask) {
 // Level L, step 2: push the queue's masks along every edge.
 template <int BLOCK, int K>
 __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(MsArgs a, int L) {
-    __shared__ MsTileLds<BLOCK, K> s;
+    __shared__ MsExpandLds<BLOCK, K> lds;
+    MsTileLds<BLOCK, K>& s = lds.td;
     constexpr uint32_t T = BLOCK * K;
     if (a.ctl[MS_DONE] | a.ctl[MS_ERR]) return;
     const unsigned long long cnt = a.ctl[MS_CNT0 + (L & 1)];
@@ -718,7 +735,11 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(MsArgs a, int L) {
         // component) would otherwise keep every row it never reaches open.
         const int k = a.p->k;
         const unsigned long long kmask = k >= 64 ? ~0ull : ((1ull << k) - 1);
-        ms_bottom_up(a, kmask & a.ctl[MS_ACT0 + (L & 1)]);
+        // fmask does not change during the level (update_compact stores it).
+        const uint32_t nhub = a.hub_lds ? static_cast<uint32_t>(std::min<int64_t>(BLOCK * kHubPerThread, a.n)) : 0u;
+        for (uint32_t i = threadIdx.x; i < nhub; i += BLOCK) lds.hub[i] = a.fmask[i];
+        __syncthreads();
+        ms_bottom_up(a, kmask & a.ctl[MS_ACT0 + (L & 1)], lds.hub, nhub);
         return;
     }
     const uint32_t ntiles = static_cast<uint32_t>((uint64_t(E) + T - 1) / T);
@@ -1331,6 +1352,9 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     a.alpha = static_cast<uint32_t>(g.ms_cfg.alpha);
     a.low_degree = static_cast<uint32_t>(g.ms_cfg.low_degree);
     a.beta = static_cast<uint32_t>(g.ms_cfg.beta);
+    // Copying the hub masks costs every workgroup 8 KB per wave of reads per
+    // bottom-up level: worth it on large graphs only.
+    a.hub_lds = n >= kHubLdsMinVertices ? 1u : 0u;
 
     int occ = 0;
     GG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(ks->expand),
