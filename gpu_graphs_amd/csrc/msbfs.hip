@@ -89,7 +89,7 @@ constexpr int kMaxCapturedLevels = 32;  // deeper searches continue with direct 
 constexpr int kByteLevels = 254;         // levels kept in the byte lines; 254 = stored directly, 255 = none
 constexpr int kNibLevels = 14;           // levels kept in the nibble lines; 14 = stored directly, 15 = none
 constexpr int kMaxLevelBatch = 128;     // levels enqueued between done checks
-constexpr int64_t kHubLdsMinVertices = int64_t(1) << 21;  // bottom-up hub masks in LDS from this many vertices
+constexpr int64_t kHubLdsMinVertices = int64_t(1) << 20;  // bottom-up hub masks in LDS from this many vertices
 
 // Per vertex, side by side: a top-down probe reads both in one 16-byte load
 // (one cache line instead of two).
@@ -145,10 +145,11 @@ struct MsTileLds {
 };
 
 // Bottom-up levels: the masks of the first kHubPerThread * BLOCK vertices
-// (the highest degrees: at R-MAT s24 the first 8,192 of 16.8 M vertices take
-// 20% of all row entries) are copied into LDS at the start of the level, so
-// that those gathers do not go to the L2.  8 KB per wave.
-constexpr int kHubPerThread = 16;
+// (the highest degrees: at R-MAT s24 the first 10,240 of 16.8 M vertices take
+// about 22% of all row entries) are copied into LDS at the start of the
+// level, so that those gathers do not go to the L2.  10 KB per wave: with 16
+// waves per CU that fills its 160 KB.
+constexpr int kHubPerThread = 20;
 
 template <int BLOCK, int K>
 union MsExpandLds {
@@ -1352,7 +1353,7 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     a.alpha = static_cast<uint32_t>(g.ms_cfg.alpha);
     a.low_degree = static_cast<uint32_t>(g.ms_cfg.low_degree);
     a.beta = static_cast<uint32_t>(g.ms_cfg.beta);
-    // Copying the hub masks costs every workgroup 8 KB per wave of reads per
+    // Copying the hub masks costs every workgroup 10 KB per wave of reads per
     // bottom-up level: worth it on large graphs only.
     a.hub_lds = n >= kHubLdsMinVertices ? 1u : 0u;
 
