@@ -111,6 +111,8 @@ struct MsParams {
     int32_t* levels;                // k rows of n int32
     int32_t prev_sources[64];       // the previous batch's sources (their state is cleared)
     int32_t prev_k;
+    int32_t bu_lo;                  // levels [bu_lo, bu_hi] go bottom-up, and their update builds no queue
+    int32_t bu_hi;
     int32_t pad;
 };
 
@@ -144,6 +146,8 @@ struct MsWorkspace {
     int depth_hint = 0;                   // levels queued before the first done check (see bfs_batch)
     int short_streak = 0;                 // batches in a row shallower than depth_hint
     int last_bu_levels = 0;               // levels the last batch ran bottom-up
+    int bu_lo = 1, bu_hi = 0;             // first / last level the last batch's direction rule chose bottom-up
+    int bu_alpha = -1, bu_beta = -1;      // the thresholds that batch ran with
     bool tail_clean = false;              // vertices without edges hold no state but prev_sources'
     bool deep_learned = false;            // the last batch was deep with thin frontiers: fused levels next
     bool last_deep = false;               // the last batch ran fused levels

@@ -73,6 +73,8 @@ enum : int {
     MS_DCNT0 = 13,     // 3 words, fused (deep) levels: length of level L's touched list at dcnt[L % 3]
     MS_MAXF = 16,      // fused levels: longest touched list
     MS_MAXE = 17,      // largest frontier (CSR entries) of a top-down level
+    MS_BUFIRST = 18,   // first / last level the direction rule chose bottom-up (~0 / 0: none)
+    MS_BULAST = 19,
     MS_WORDS = 20
 };
 enum : unsigned long long { MS_ERR_TILE = 1, MS_ERR_RANGE = 2 };
@@ -265,8 +267,10 @@ __global__ __launch_bounds__(64) void ms_snapshot_kernel(const unsigned long lon
 // Level 0: next(source i) |= bit i; each distinct source once on the touched
 // list of level 0.
 __global__ void ms_seed_kernel(MsArgs a) {
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         for (int i = 0; i < MS_WORDS; ++i) a.ctl[i] = 0;
+        a.ctl[MS_BUFIRST] = ~0ull;
+    }
     if (!a.p->clear_all && threadIdx.x < static_cast<uint32_t>(a.p->prev_k)) {
         const int32_t s = a.perm[a.p->prev_sources[threadIdx.x]];
         if (s >= a.nz) {
@@ -337,7 +341,8 @@ __device__ __forceinline__ ulonglong2 merge_line_pair(const MsArgs& a, ulonglong
 // they read.  Each vertex is in at most one slot of a launch (the touched
 // list holds distinct vertices), so no other thread writes its line or state.
 __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, int64_t span, bool list, uint32_t count,
-                            unsigned long long kmask, unsigned long long& fulldeg, unsigned long long& act) {
+                            unsigned long long kmask, bool no_queue, unsigned long long& fulldeg,
+                            unsigned long long& act) {
     __shared__ unsigned long long s_wave[kUcBlock / 64];
     __shared__ unsigned long long s_excl;
     __shared__ uint32_t s_ok;
@@ -446,7 +451,8 @@ __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, int64_t span, bo
         if (anynew) a.ctl[MS_MAXLEV] = static_cast<unsigned long long>(L);
     }
     __syncthreads();
-    if (!active || !s_ok) return;
+    // A level that goes bottom-up for sure reads no queue: only its counts.
+    if (!active || !s_ok || no_queue) return;
     const unsigned long long pre = s_excl + excl_local;
     uint32_t slot = static_cast<uint32_t>(pre >> 32);
     uint32_t eo = static_cast<uint32_t>(pre);
@@ -487,9 +493,12 @@ __global__ __launch_bounds__(kUcBlock) void ms_update_compact_kernel(MsArgs a, i
     const uint32_t nch = list ? static_cast<uint32_t>((count + span - 1) / span) : a.nz_chunks;
     const int k = a.p->k;
     const unsigned long long kmask = k >= 64 ? ~0ull : ((1ull << k) - 1);
+    // Levels in the learned bottom-up window go bottom-up after a scan
+    // (expand() forces it), so their queue is not built.
+    const bool no_queue = a.bottom_up && !list && L >= a.p->bu_lo && L <= a.p->bu_hi;
     unsigned long long fulldeg = 0, act = 0;
     for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
-        ms_uc_chunk(a, L, c, span, list, static_cast<uint32_t>(count), kmask, fulldeg, act);
+        ms_uc_chunk(a, L, c, span, list, static_cast<uint32_t>(count), kmask, no_queue, fulldeg, act);
         __syncthreads();
     }
     // One atomic per workgroup for the edges of the newly fully seen
@@ -723,12 +732,22 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(MsArgs a, int L) {
     // nz / beta vertices, a top-down level (which hands a list to the next
     // update) is cheaper than a bottom-up one (after which the next update
     // scans every vertex).
-    const bool bottom_up = a.bottom_up && scanned && static_cast<int64_t>(E) * a.alpha > open_edges &&
-                           (a.beta == 0 || static_cast<int64_t>(F) * a.beta >= a.nz);
+    const bool rule = a.bottom_up && scanned && static_cast<int64_t>(E) * a.alpha > open_edges &&
+                      (a.beta == 0 || static_cast<int64_t>(F) * a.beta >= a.nz);
+    // Levels in the window the rule chose in the runner's last batch go
+    // bottom-up whatever it says now: their update built no queue (bottom-up
+    // levels give the same levels as top-down ones, only the cost differs).
+    // The window learned for the next batch is the rule's own choice.
+    const bool forced = a.bottom_up && scanned && L >= a.p->bu_lo && L <= a.p->bu_hi;
+    const bool bottom_up = rule || forced;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.ctl[MS_TVALID0 + npar] = (sparse && !bottom_up) ? 1 : 0;
         if (bottom_up) atomicAdd(&a.ctl[MS_BULEVELS], 1ull);
         else atomicMax(&a.ctl[MS_MAXE], static_cast<unsigned long long>(E));
+        if (rule) {
+            atomicMin(&a.ctl[MS_BUFIRST], static_cast<unsigned long long>(L));
+            atomicMax(&a.ctl[MS_BULAST], static_cast<unsigned long long>(L));
+        }
     }
     if (bottom_up) {
         // Only sources with a non-empty level-L frontier can reach anything
@@ -1387,6 +1406,10 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     ms.h_params->clear_all = ms.tail_clean ? 0 : 1;
     std::memcpy(ms.h_params->prev_sources, ms.prev_sources, sizeof(ms.prev_sources));
     ms.h_params->prev_k = ms.tail_clean ? ms.prev_k : 0;
+    // The window holds for the direction thresholds it was learned with.
+    const bool window = ms.bu_alpha == g.ms_cfg.alpha && ms.bu_beta == g.ms_cfg.beta;
+    ms.h_params->bu_lo = window ? ms.bu_lo : 1;
+    ms.h_params->bu_hi = window ? ms.bu_hi : 0;
     ms.tail_clean = false;
     auto launch_prologue = [&]() {
         hipLaunchKernelGGL(ms_clear_kernel, dim3(cgrid), dim3(256), 0, stream, a);
@@ -1536,6 +1559,11 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
                            : ms.last_depth >= kMsDeepDepth && fin[MS_BULEVELS] == 0 &&
                                  fin[MS_MAXE] <= static_cast<unsigned long long>(a.sparse_edges);
     ms.last_bu_levels = static_cast<int>(fin[MS_BULEVELS]);
+    const bool any_bu = fin[MS_BUFIRST] != ~0ull && fin[MS_BUFIRST] <= fin[MS_BULAST];
+    ms.bu_lo = any_bu ? static_cast<int>(fin[MS_BUFIRST]) : 1;
+    ms.bu_hi = any_bu ? static_cast<int>(fin[MS_BULAST]) : 0;
+    ms.bu_alpha = g.ms_cfg.alpha;
+    ms.bu_beta = g.ms_cfg.beta;
     // Levels queued for the next batch: this depth + 1 (an empty level ends a
     // batch), raised at once and lowered only after 16 shallower batches in a
     // row, so that batches of slightly different depths replay one capture.

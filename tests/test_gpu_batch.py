@@ -251,6 +251,39 @@ def test_bottom_up_batch_on_mixed_graph_and_directed(gpu):
     assert dd.batch_last_bottom_up() == 0
 
 
+def test_bottom_up_window_carried_to_other_batches(gpu):
+    # A runner remembers the levels its last batch's direction rule took
+    # bottom-up; the next batch (same thresholds) runs those levels
+    # bottom-up after a scanning update whatever its own frontiers say, and
+    # their update builds no queue.  Batches whose own choice differs (other
+    # sources, fewer of them, sources in the thin tail) must still equal the
+    # oracle, and so must a batch after the thresholds change.
+    core = gg.Graph.rmat(14, seed=87)
+    grid = gg.Graph.grid(150, 30, 0.05, seed=88)
+    nc = core.n
+    cs = np.repeat(np.arange(nc), np.diff(core.offsets)).astype(np.int32)
+    gs = (np.repeat(np.arange(grid.n), np.diff(grid.offsets)) + nc).astype(np.int32)
+    hub = int(np.argmax(np.diff(core.offsets)))
+    G = gg.Graph.from_edges(nc + grid.n, np.concatenate([cs, gs, [hub, nc]]),
+                            np.concatenate([core.cols, grid.cols + nc, [nc, hub]]))
+    dg = gg.DeviceGraph(G, gpu)
+    dg.batch_direction_config(True, 4, 0)
+    deg = np.diff(G.offsets[: nc + 1])
+    rng = np.random.default_rng(87)
+    by_degree = np.argsort(-deg, kind="stable")
+    hubs = by_degree[:64].astype(np.int32)
+    low = rng.choice(np.nonzero(deg == 1)[0], 48, replace=False).astype(np.int32)
+    tail = (nc + rng.choice(grid.n, 16, replace=False)).astype(np.int32)
+    for name, sources in (("hubs", hubs), ("low-degree + tail", np.concatenate([low, tail])),
+                          ("three", hubs[:3]), ("tail", tail), ("hubs again", hubs)):
+        log(f"window batch: {name}")
+        check_batch(G, dg, sources)
+    assert dg.batch_last_bottom_up() >= 1
+    dg.batch_direction_config(True, 1, 0)
+    log("window batch: alpha changed")
+    check_batch(G, dg, np.concatenate([low, tail]))
+
+
 def test_level_lines_of_earlier_batches_are_not_read(gpu):
     # The per-vertex level lines are not cleared between batches: the first
     # level that reaches a vertex rewrites its whole line, and vertices no
