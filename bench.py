@@ -35,8 +35,10 @@ Roofline (algorithmic bytes of one BFS from source s, int32 levels):
   4 n level init; per reached vertex 16 B of offsets + 12 B of level write,
   enqueue and dequeue; per scanned CSR entry 4 B column + 4 B status probe.
 `roofline.achieved` (= `alg_gbs`) divides the mean B(s) by the mean device
-time of a BFS (HIP events around its launch sequence on the handle's
-stream).  `roofline.traffic` is the HBM bytes per BFS that rocprofv3's
+time of a BFS: from the start of its first kernel to the store of its final
+control block by its last kernel, both stamped with the GPU's constant-rate
+clock (wall_clock64) on the handle's stream (a start event recorded ahead of
+the launches would keep the GPU idle for its own host time).  `roofline.traffic` is the HBM bytes per BFS that rocprofv3's
 FETCH_SIZE and WRITE_SIZE counters gave for the same configuration
 (profiles/bfs_rmat22_traffic.json), and `hbm_gbs` divides it by the same
 device time: the bandwidth actually moved.  The direction-optimising search
@@ -324,9 +326,10 @@ def single_source(gg, args):
             "hbm_gbs": round(hbm_gbs, 2) if hbm_gbs else None,
             "hbm_frac": round(hbm_gbs / HBM_PEAK_GBS, 4) if hbm_gbs else None,
             "traffic_note": prof_note,
-            "kernel": "one BFS: the replayed launch sequence (bfs_clear_kernel, which seeds the source, then "
-                      "one bfs_expand_kernel per level, top-down or bottom-up; bfs_compact_kernel only on "
-                      "directed graphs), HIP events on its stream",
+            "kernel": "one BFS: its launch sequence on the handle's stream (bfs_clear_kernel, which seeds "
+                      "the source, one bfs_expand_kernel per level, top-down or bottom-up, and bfs_tail_kernel "
+                      "for the small last levels; bfs_compact_kernel only on directed graphs), timed from the "
+                      "clear kernel's start to the final control-block store by the GPU's wall_clock64 stamps",
             "kernel_ms_avg": round(kernel_ms, 4),
             "alg_bytes_per_launch": int(sum(b_alg) / nb),
         },
