@@ -75,7 +75,10 @@ enum : int {
     MS_MAXE = 17,      // largest frontier (CSR entries) of a top-down level
     MS_BUFIRST = 18,   // first / last level the direction rule chose bottom-up (~0 / 0: none)
     MS_BULAST = 19,
-    MS_WORDS = 20
+    MS_FMP0 = 20,      // fmp[L & 1]: which mask copy is current at level L (0: fmask, 1: fmask2)
+    MS_POST0 = 22,     // post[L & 1]: expand(L-1) updated the light vertices itself (update_compact(L)
+                       // takes the heavy ones only)
+    MS_WORDS = 24
 };
 enum : unsigned long long { MS_ERR_TILE = 1, MS_ERR_RANGE = 2 };
 
@@ -105,6 +108,8 @@ struct MsArgs {
     const int32_t* __restrict__ cols;
     SeenNext* sn;
     unsigned long long* fmask;   // per vertex: seen, stored when it changes (the mask bottom-up levels gather)
+    unsigned long long* fmask2;  // the other copy: a bottom-up level that updates its vertices itself
+                                 // writes the new seen of every vertex there and gathers from the first
     unsigned long long* lv8;     // per vertex: its level line (8 words of bytes, or 4 of nibbles when nib)
     unsigned long long* reached; // bit per vertex: its lv8 line has been written in this batch
     MsParams* p;                 // sources, k, output rows (device memory)
@@ -341,8 +346,8 @@ __device__ __forceinline__ ulonglong2 merge_line_pair(const MsArgs& a, ulonglong
 // they read.  Each vertex is in at most one slot of a launch (the touched
 // list holds distinct vertices), so no other thread writes its line or state.
 __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, int64_t span, bool list, uint32_t count,
-                            unsigned long long kmask, bool no_queue, unsigned long long& fulldeg,
-                            unsigned long long& act) {
+                            unsigned long long kmask, bool no_queue, unsigned long long* fm, bool post,
+                            int64_t scan_end, unsigned long long& fulldeg, unsigned long long& act) {
     __shared__ unsigned long long s_wave[kUcBlock / 64];
     __shared__ unsigned long long s_excl;
     __shared__ uint32_t s_ok;
@@ -358,7 +363,7 @@ __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, int64_t span, bo
         int64_t v = -1;
         if (list) {
             if (i < count && i < iend) v = a.touched[i];
-        } else if (i < a.nz) {  // a scan comes after level 0: vertices without edges get nothing
+        } else if (i < scan_end) {  // a scan comes after level 0: vertices without edges get nothing
             v = i;
         }
         vs[r] = v;
@@ -400,7 +405,10 @@ __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, int64_t span, bo
         const unsigned long long nv = newbits[r];
         if (v >= 0) {
             if (st[r].y) *reinterpret_cast<ulonglong2*>(a.sn + v) = make_ulonglong2(st[r].x | nv, 0ull);
-            if (nv) a.fmask[v] = st[r].x | nv;  // seen only changes here
+            // seen only changes here; after a level that updated the light
+            // vertices itself, every heavy vertex's copy is written (the
+            // copy in use changed)
+            if (nv || post) fm[v] = st[r].x | nv;
         }
         act |= nv;
         if (list) {
@@ -490,7 +498,14 @@ __global__ __launch_bounds__(kUcBlock) void ms_update_compact_kernel(MsArgs a, i
         return;
     }
     const int64_t span = list && count <= static_cast<unsigned long long>(kUcChunk) * gridDim.x ? kUcBlock : kUcChunk;
-    const uint32_t nch = list ? static_cast<uint32_t>((count + span - 1) / span) : a.nz_chunks;
+    // After a bottom-up level that updated the light vertices itself, only
+    // the heavy prefix [0, heavy) is left.
+    const bool post = !list && a.ctl[MS_POST0 + par] != 0;
+    unsigned long long* const fm = a.ctl[MS_FMP0 + par] ? a.fmask2 : a.fmask;
+    const int64_t scan_end = post ? a.heavy : a.nz;
+    const uint32_t nch = list   ? static_cast<uint32_t>((count + span - 1) / span)
+                         : post ? static_cast<uint32_t>((a.heavy + kUcChunk - 1) / kUcChunk)
+                                : a.nz_chunks;
     const int k = a.p->k;
     const unsigned long long kmask = k >= 64 ? ~0ull : ((1ull << k) - 1);
     // Levels in the learned bottom-up window go bottom-up after a scan
@@ -498,7 +513,8 @@ __global__ __launch_bounds__(kUcBlock) void ms_update_compact_kernel(MsArgs a, i
     const bool no_queue = a.bottom_up && !list && L >= a.p->bu_lo && L <= a.p->bu_hi;
     unsigned long long fulldeg = 0, act = 0;
     for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
-        ms_uc_chunk(a, L, c, span, list, static_cast<uint32_t>(count), kmask, no_queue, fulldeg, act);
+        ms_uc_chunk(a, L, c, span, list, static_cast<uint32_t>(count), kmask, no_queue, fm, post, scan_end, fulldeg,
+                    act);
         __syncthreads();
     }
     // One atomic per workgroup for the edges of the newly fully seen
@@ -532,11 +548,65 @@ __global__ __launch_bounds__(kUcBlock) void ms_update_compact_kernel(MsArgs a, i
 constexpr int kSubStep = 4;     // bottom-up long rows: entries per lane in a sub-wave's later steps
 constexpr int kHeavyBatch = 4;  // bottom-up heavy rows: tiles a wave scans together
 
-__device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask, const unsigned long long* hub, uint32_t nhub) {
+// Per-thread totals of the light vertices a bottom-up level updates itself.
+struct InlineTotals {
+    unsigned long long cnt = 0;      // (vertices << 32) | edges queued for level L+1
+    unsigned long long fulldeg = 0;  // edges of the vertices every source has now reached
+    unsigned long long act = 0;      // sources that reached new vertices
+    bool anynew = false;
+};
+
+// The update of update_compact(L+1) for light vertex v (lane of the wave that
+// owns the 64 vertices of reached word g; every lane calls it): seen and its
+// new copy in fm_out, level L+1 into the line of the sources in nb, the
+// reached bit, and the totals.  No queue entry: level L+1 goes bottom-up.
+__device__ __forceinline__ void ms_inline_update(const MsArgs& a, unsigned long long* fm_out, int64_t v, uint64_t g,
+                                                 bool mine, unsigned long long sv, unsigned long long nb,
+                                                 uint32_t deg, int L1, unsigned long long kfull, InlineTotals& t) {
+    const bool first = nb && !sv;
+    const ulonglong2* src = line_of(a, mine ? v : 0);
+    ulonglong2 line[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        line[j] = (nb && !first && j < line_pairs(a)) ? src[j] : make_ulonglong2(~0ull, ~0ull);
+    if (mine) {
+        fm_out[v] = sv | nb;
+        if (nb) *reinterpret_cast<ulonglong2*>(a.sn + v) = make_ulonglong2(sv | nb, 0ull);
+    }
+    // this wave alone writes word g in this kernel (the heavy vertices of
+    // the word are updated by the next update_compact)
+    const uint64_t fb = wave_ballot(first);
+    if (fb && (threadIdx.x & 63u) == 0) a.reached[g] |= fb;
+    if (L1 >= line_levels(a)) {
+        // every lane of the wave takes part in the shuffles of write_levels
+        if (__ballot(nb != 0)) write_levels(a.p->levels, a.n, nb ? a.iperm[v] : 0, nb, L1);
+    }
+    if (nb) {
+        const unsigned long long lv = static_cast<unsigned long long>(L1 < line_levels(a) ? L1 : line_levels(a));
+        ulonglong2* dst = const_cast<ulonglong2*>(line_of(a, v));
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (j < line_pairs(a)) dst[j] = merge_line_pair(a, line[j], nb, j, lv);
+        t.cnt += (1ull << 32) | deg;
+        if (((sv | nb) & kfull) == kfull) t.fulldeg += deg;
+        t.act |= nb;
+        t.anynew = true;
+    }
+}
+
+// fm_in: the mask copy current at level L.  fm_out (null: the level only
+// stores next): the light vertices are updated here for level L1 = L + 1
+// (ms_inline_update) and every light vertex's seen goes to fm_out; kfull =
+// all k sources.  `hub` (LDS, at least 64 words) also holds the workgroup's
+// totals at the end.
+__device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask, unsigned long long* hub, uint32_t nhub,
+                             const unsigned long long* fm_in, unsigned long long* fm_out, int L1,
+                             unsigned long long kfull) {
     const uint32_t lane = threadIdx.x & 63u;
     bool bad = false;
+    InlineTotals tot;
     // Vertex u's mask (u in [0, n)): from the LDS copy for the first nhub.
-    auto mask_of = [&](int32_t u) { return static_cast<uint32_t>(u) < nhub ? hub[u] : a.fmask[u]; };
+    auto mask_of = [&](int32_t u) { return static_cast<uint32_t>(u) < nhub ? hub[u] : fm_in[u]; };
     // Heavy rows (a prefix of the degree-ordered ids): tiles of kMsHeavyRow
     // entries spread over every wave, so the longest rows do not hold up a
     // few waves.  Tiles are numbered round by round (tile r of every row
@@ -618,9 +688,16 @@ __device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask, const un
     for (uint64_t g = g_first + static_cast<uint64_t>(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
          g < ngroups; g += wstride) {
         const int64_t v = static_cast<int64_t>(g * 64 + lane);
-        unsigned long long un = 0;
-        if (v >= a.heavy && v < a.nz) un = kmask & ~a.sn[v].seen;
-        if (!wave_ballot(un != 0)) continue;
+        unsigned long long un = 0, sv = 0;
+        const bool mine = v >= a.heavy && v < a.nz;
+        if (mine) {
+            sv = a.sn[v].seen;
+            un = kmask & ~sv;
+        }
+        if (!wave_ballot(un != 0)) {
+            if (fm_out && mine) fm_out[v] = sv;
+            continue;
+        }
         int64_t rb = 0, re = 0;
         if (un) {
             rb = a.offsets[v];
@@ -699,9 +776,41 @@ __device__ void ms_bottom_up(const MsArgs& a, unsigned long long kmask, const un
             if ((taken >> lane) & 1ull) acc = got;
         }
         const unsigned long long nb = acc & un;
-        if (nb) a.sn[v].next = nb;
+        if (fm_out) ms_inline_update(a, fm_out, v, g, mine, sv, nb, deg, L1, kfull, tot);
+        else if (nb) a.sn[v].next = nb;
     }
     if (bad) atomicOr(&a.ctl[MS_ERR], MS_ERR_RANGE);
+    if (!fm_out) return;
+    // One atomic per workgroup for each total of level L1, summed in the
+    // hub words once every wave is past its last read of them.
+    const unsigned long long c = wave_sum(tot.cnt), f = wave_sum(tot.fulldeg);
+    unsigned long long b = tot.act;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) b |= __shfl_xor(b, o, 64);
+    const bool anynew = __ballot(tot.anynew) != 0;
+    const uint32_t w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) {
+        hub[w] = c;
+        hub[16 + w] = f;
+        hub[32 + w] = b;
+        hub[48 + w] = anynew ? 1ull : 0ull;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tc = 0, tf = 0, tb = 0, tn = 0;
+        for (uint32_t i = 0; i < blockDim.x / 64; ++i) {
+            tc += hub[i];
+            tf += hub[16 + i];
+            tb |= hub[32 + i];
+            tn |= hub[48 + i];
+        }
+        if (tc) atomicAdd(&a.ctl[MS_CNT0 + (L1 & 1)], tc);
+        if (tf) atomicAdd(&a.ctl[MS_FULLDEG], tf);
+        if (tb) atomicOr(&a.ctl[MS_ACT0 + (L1 & 1)], tb);
+        // every writer of level L1 stores the same value
+        if (tn) a.ctl[MS_MAXLEV] = static_cast<unsigned long long>(L1);
+    }
 }
 
 // Level L, step 2: push the queue's masks along every edge.
@@ -740,7 +849,19 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(MsArgs a, int L) {
     // The window learned for the next batch is the rule's own choice.
     const bool forced = a.bottom_up && scanned && L >= a.p->bu_lo && L <= a.p->bu_hi;
     const bool bottom_up = rule || forced;
+    // When level L+1 is in the window too (bottom-up after a scan, so no
+    // queue), this level updates its light vertices itself, for level L+1:
+    // update_compact(L+1) is left the heavy prefix.  The new seen of every
+    // vertex then goes to the other mask copy (the neighbours of a vertex
+    // updated early must still read its level-L mask), which is current from
+    // level L+1 on.
+    const bool inline_update = bottom_up && L + 1 >= a.p->bu_lo && L + 1 <= a.p->bu_hi;
+    const unsigned long long fmp = a.ctl[MS_FMP0 + (L & 1)];
+    const unsigned long long* const fm_in = fmp ? a.fmask2 : a.fmask;
+    unsigned long long* const fm_out = fmp ? a.fmask : a.fmask2;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.ctl[MS_FMP0 + npar] = inline_update ? (fmp ^ 1ull) : fmp;
+        a.ctl[MS_POST0 + npar] = inline_update ? 1ull : 0ull;
         a.ctl[MS_TVALID0 + npar] = (sparse && !bottom_up) ? 1 : 0;
         if (bottom_up) atomicAdd(&a.ctl[MS_BULEVELS], 1ull);
         else atomicMax(&a.ctl[MS_MAXE], static_cast<unsigned long long>(E));
@@ -755,11 +876,13 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(MsArgs a, int L) {
         // component) would otherwise keep every row it never reaches open.
         const int k = a.p->k;
         const unsigned long long kmask = k >= 64 ? ~0ull : ((1ull << k) - 1);
-        // fmask does not change during the level (update_compact stores it).
+        // The mask copy does not change during the level (update_compact
+        // stores it, or an inline update stores the other copy).
         const uint32_t nhub = a.hub_lds ? static_cast<uint32_t>(std::min<int64_t>(BLOCK * kHubPerThread, a.n)) : 0u;
-        for (uint32_t i = threadIdx.x; i < nhub; i += BLOCK) lds.hub[i] = a.fmask[i];
+        for (uint32_t i = threadIdx.x; i < nhub; i += BLOCK) lds.hub[i] = fm_in[i];
         __syncthreads();
-        ms_bottom_up(a, kmask & a.ctl[MS_ACT0 + (L & 1)], lds.hub, nhub);
+        ms_bottom_up(a, kmask & a.ctl[MS_ACT0 + (L & 1)], lds.hub, nhub, fm_in, inline_update ? fm_out : nullptr,
+                     L + 1, kmask);
         return;
     }
     const uint32_t ntiles = static_cast<uint32_t>((uint64_t(E) + T - 1) / T);
@@ -1255,7 +1378,7 @@ void build_ms_graph(DeviceGraph& dg, const HostGraph& g) {
 
 void Runner::free_ms_workspace() {
     if (ms.graph_exec) (void)hipGraphExecDestroy(ms.graph_exec);
-    void* ptrs[] = {ms.sn,      ms.fmask,      ms.lv8,     ms.reached, ms.q_rb,   ms.q_eoff, ms.q_v,
+    void* ptrs[] = {ms.sn,      ms.fmask,      ms.fmask2,   ms.lv8,     ms.reached, ms.q_rb,   ms.q_eoff, ms.q_v,
                     ms.q_fm,    ms.tile_start, ms.touched, ms.ctl,     ms.params, ms.levels, ms.partial,
                     ms.touched2, ms.next1};
     for (void* p : ptrs)
@@ -1281,6 +1404,7 @@ void Runner::ensure_ms_workspace() {
     try {
         alloc(&ms.sn, sizeof(SeenNext) * n);
         alloc(&ms.fmask, sizeof(unsigned long long) * n);
+        alloc(&ms.fmask2, sizeof(unsigned long long) * n);
         alloc(&ms.lv8, sizeof(unsigned long long) * 8 * n);
         alloc(&ms.reached, sizeof(unsigned long long) * ((n + 63) / 64));
         alloc(&ms.q_rb, sizeof(int64_t) * n);
@@ -1350,6 +1474,7 @@ void Runner::bfs_batch(const int32_t* h_sources, int k, int32_t* d_levels, gb_ba
     a.nz_chunks = static_cast<uint32_t>((g.ms_nonzero + kUcChunk - 1) / kUcChunk);
     a.sn = reinterpret_cast<SeenNext*>(ms.sn);
     a.fmask = ms.fmask;
+    a.fmask2 = ms.fmask2;
     a.lv8 = ms.lv8;
     a.reached = ms.reached;
     a.p = ms.params;
