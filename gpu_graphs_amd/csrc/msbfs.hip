@@ -347,7 +347,8 @@ __device__ __forceinline__ ulonglong2 merge_line_pair(const MsArgs& a, ulonglong
 // list holds distinct vertices), so no other thread writes its line or state.
 __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, int64_t span, bool list, uint32_t count,
                             unsigned long long kmask, bool no_queue, unsigned long long* fm, bool post,
-                            int64_t scan_end, unsigned long long& fulldeg, unsigned long long& act) {
+                            int64_t scan_end, unsigned long long& fulldeg, unsigned long long& act,
+                            unsigned long long& wg_count) {
     __shared__ unsigned long long s_wave[kUcBlock / 64];
     __shared__ unsigned long long s_excl;
     __shared__ uint32_t s_ok;
@@ -447,7 +448,11 @@ __device__ void ms_uc_chunk(const MsArgs& a, int L, uint32_t c, int64_t span, bo
     const unsigned long long excl_local = block_exclusive_scan<kUcBlock>(mine, s_wave, total);
     if (tid == 0) {
         s_ok = 1;
-        if (total) {
+        if (total && no_queue) {
+            // no slots to reserve: the counts go to ctl once per workgroup
+            // (an atomic per chunk on the one counter serialises)
+            wg_count += total;
+        } else if (total) {
             s_excl = atomicAdd(&a.ctl[MS_CNT0 + (L & 1)], total);
             if (!reservation_fits(s_excl, total, a.n, a.m)) {
                 atomicOr(&a.ctl[MS_ERR], MS_ERR_RANGE);
@@ -511,14 +516,15 @@ __global__ __launch_bounds__(kUcBlock) void ms_update_compact_kernel(MsArgs a, i
     // Levels in the learned bottom-up window go bottom-up after a scan
     // (expand() forces it), so their queue is not built.
     const bool no_queue = a.bottom_up && !list && L >= a.p->bu_lo && L <= a.p->bu_hi;
-    unsigned long long fulldeg = 0, act = 0;
+    unsigned long long fulldeg = 0, act = 0, wg_count = 0;
     for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
         ms_uc_chunk(a, L, c, span, list, static_cast<uint32_t>(count), kmask, no_queue, fm, post, scan_end, fulldeg,
-                    act);
+                    act, wg_count);
         __syncthreads();
     }
     // One atomic per workgroup for the edges of the newly fully seen
-    // vertices, one for the sources that reached new vertices.
+    // vertices, one for the sources that reached new vertices, and (levels
+    // without a queue) one for the level's counts.
     __shared__ unsigned long long s_fd[kUcBlock / 64], s_act[kUcBlock / 64];
     const unsigned long long w = wave_sum(fulldeg);
 #pragma unroll
@@ -536,6 +542,7 @@ __global__ __launch_bounds__(kUcBlock) void ms_update_compact_kernel(MsArgs a, i
         }
         if (t) atomicAdd(&a.ctl[MS_FULLDEG], t);
         if (b) atomicOr(&a.ctl[MS_ACT0 + (L & 1)], b);
+        if (wg_count) atomicAdd(&a.ctl[MS_CNT0 + (L & 1)], wg_count);
     }
 }
 
