@@ -63,6 +63,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <vector>
 
@@ -100,6 +101,10 @@ enum : int {
     CTL_T1 = 31,       // in a stored snapshot only: device clock when it was taken
     CTL_WORDS = 32
 };
+// What the host fills a snapshot slot with before the launch: no control word
+// takes this value (counts and packed counts stay below 2^63, flags and
+// levels are small, "none" is ~0, the clock would need 10^11 s to get there).
+constexpr unsigned long long kSnapUnwritten = 0xA5A5A5A55A5A5A5Aull;
 enum : unsigned long long { ERR_TILE = 1, ERR_RANGE = 2 };
 enum : unsigned long long { REP_QUEUE = 1, REP_BITMAP = 2 };
 enum : unsigned long long { DIR_TD = 1, DIR_BU = 2 };
@@ -1323,26 +1328,21 @@ __device__ void tail_levels(const BfsArgs& a, int T) {
 // sequence holds no copy node (a copy is a separate blit dispatch, and in a
 // replayed graph it also delays the submission of the kernels behind it).
 // Every thread of the workgroup calls this after its last update of ctl.
-// The snapshot's CTL_T1 word is the device clock at the store, stored last
-// (never 0): the host, which cleared it before the launch, polls it to learn
-// that the snapshot is complete, and with CTL_T0 it times the sequence on the
-// device without events.  Every thread of the workgroup calls this after its
-// stores to `out`.
-__device__ __forceinline__ void store_stamp(unsigned long long* out) {
-    __threadfence_system();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned long long t = static_cast<unsigned long long>(wall_clock64());
-        __hip_atomic_store(out + CTL_T1, t ? t : 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
+// The snapshot's CTL_T1 word is the device clock at the store: with CTL_T0
+// it times the sequence on the device without events.  The host fills the
+// snapshot with kSnapUnwritten before the launch and takes it as complete
+// once no word holds that value any more, so the words need no ordering
+// among themselves (no system-scope fence here; an aligned 8-byte store
+// reaches host memory whole).
+__device__ __forceinline__ unsigned long long snap_word(const unsigned long long* ctl, uint32_t i) {
+    return i == CTL_T1 ? static_cast<unsigned long long>(wall_clock64())
+                       : __hip_atomic_load(ctl + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ void snapshot_ctl(unsigned long long* ctl, unsigned long long* out) {
     __threadfence();
     __syncthreads();
-    if (threadIdx.x < CTL_WORDS && threadIdx.x != CTL_T1)
-        out[threadIdx.x] = __hip_atomic_load(ctl + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    store_stamp(out);
+    if (threadIdx.x < CTL_WORDS) out[threadIdx.x] = snap_word(ctl, threadIdx.x);
 }
 
 // `snap` (host-visible, may be null): where the final control block goes.
@@ -1354,8 +1354,7 @@ __global__ __launch_bounds__(kTailBlock) void bfs_tail_kernel(BfsArgs a, int T, 
 // The control block to host-visible memory when the sequence does not end
 // with the tail kernel (earlier kernels' updates are complete at its start).
 __global__ __launch_bounds__(64) void bfs_snapshot_kernel(const unsigned long long* ctl, unsigned long long* out) {
-    if (threadIdx.x < CTL_WORDS && threadIdx.x != CTL_T1) out[threadIdx.x] = ctl[threadIdx.x];
-    store_stamp(out);
+    if (threadIdx.x < CTL_WORDS) out[threadIdx.x] = snap_word(ctl, threadIdx.x);
 }
 
 // Upload of a symmetric graph: each row's neighbours ordered by descending
@@ -1858,21 +1857,29 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
                            static_cast<const unsigned long long*>(ws.ctl), snap);
     };
     bool clock_timed = false;  // timed by the device clock stamps, not by ev_start / ev_stop
-    // The host waits for a batch of levels by polling the end stamp of its
-    // snapshot (cleared here before the batch is enqueued, stored last by the
-    // kernel): an event would add its record and signal latency, a few
-    // microseconds each.  The stream is queried now and then, so that a
-    // failed launch ends the wait with its error.
-    auto arm = [&](int slot) { __atomic_store_n(&h_ctl[slot * kCtlWords + CTL_T1], 0ull, __ATOMIC_RELEASE); };
+    // The host waits for a batch of levels by polling its snapshot: filled
+    // with kSnapUnwritten here before the batch is enqueued, complete once
+    // the kernel has overwritten every word (an event would add its record
+    // and signal latency, a few microseconds each).  The stream is queried
+    // now and then, so that a failed launch ends the wait with its error.
+    auto arm = [&](int slot) {
+        for (int i = 0; i < kCtlWords; ++i) __atomic_store_n(&h_ctl[slot * kCtlWords + i], kSnapUnwritten, __ATOMIC_RELAXED);
+        std::atomic_thread_fence(std::memory_order_release);
+    };
+    auto written = [&](int slot) {
+        const unsigned long long* w = &h_ctl[slot * kCtlWords];
+        if (__atomic_load_n(w + CTL_T1, __ATOMIC_ACQUIRE) == kSnapUnwritten) return false;
+        for (int i = 0; i < kCtlWords; ++i)
+            if (__atomic_load_n(w + i, __ATOMIC_ACQUIRE) == kSnapUnwritten) return false;
+        return true;
+    };
     auto wait_snapshot = [&](int slot) {
-        const unsigned long long* t1 = &h_ctl[slot * kCtlWords + CTL_T1];
-        for (uint32_t it = 1; __atomic_load_n(t1, __ATOMIC_ACQUIRE) == 0; ++it) {
+        for (uint32_t it = 1; !written(slot); ++it) {
             if (it % 65536 != 0) continue;  // every 4,096 polls cost deep searches 0.6%
             const hipError_t r = hipStreamQuery(stream);
             if (r == hipErrorNotReady) continue;
             GG_HIP(r);
-            if (__atomic_load_n(t1, __ATOMIC_ACQUIRE) == 0)
-                fail(GB_EDEVICE, "internal error: the launched levels stored no control block");
+            if (!written(slot)) fail(GB_EDEVICE, "internal error: the launched levels stored no control block");
         }
     };
     auto enqueue = [&](int count, int slot, int tail_at = -1) {
