@@ -1448,8 +1448,11 @@ void Runner::prepare_batch() {
     ensure_ms_workspace();
 }
 
+// What ensure_ms_workspace() allocates: per vertex 14 words (sn 2, fmask and
+// fmask2, lv8 8, q_fm, next1), q_rb, and four 4-byte arrays (q_eoff, q_v,
+// touched, touched2); the reached bitmap; tile indices; the statistics.
 size_t batch_bytes_estimate(int64_t n_vertices, int64_t m_entries, int k, bool host_levels) {
-    size_t b = sizeof(unsigned long long) * n_vertices * 13 + (sizeof(int64_t) + 16) * n_vertices +
+    size_t b = sizeof(unsigned long long) * n_vertices * 14 + (sizeof(int64_t) + 16) * n_vertices +
                sizeof(unsigned long long) * ((n_vertices + 63) / 64) +
                sizeof(uint32_t) * (m_entries / 256 + 2) + 1024 * 64 * 16 + 4096;
     if (host_levels) b += sizeof(int32_t) * n_vertices * k;
