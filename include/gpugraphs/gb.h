@@ -73,8 +73,11 @@ typedef struct gb_server gb_server;  /* continuous-batching BFS query server   *
  * (always 0: 1D Partition/src/gpuBFS_kernels.cu:37, gpuBFS.cu:63); these are
  * exact. */
 typedef struct gb_stats {
-    double  ms;           /* device time, source injection -> last level (hipEvent) */
-    double  kernel_ms;    /* traversal kernel(s) alone, without the level init      */
+    double  ms;           /* device time, level init + source injection -> last
+                           * level: GPU clock stamps in the first and last kernel
+                           * (GB_MODE_ASYNC), hipEvents (the other modes)         */
+    double  kernel_ms;    /* traversal kernel(s) alone, without the level init
+                           * (GB_MODE_LEVEL_LOOP; the other modes report ms)      */
     int32_t num_levels;   /* 1 + max level                                           */
     int32_t reserved;
     int64_t n_reached;
