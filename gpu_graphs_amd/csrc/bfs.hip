@@ -108,6 +108,13 @@ constexpr unsigned long long kSnapUnwritten = 0xA5A5A5A55A5A5A5Aull;
 enum : unsigned long long { ERR_TILE = 1, ERR_RANGE = 2 };
 enum : unsigned long long { REP_QUEUE = 1, REP_BITMAP = 2 };
 enum : unsigned long long { DIR_TD = 1, DIR_BU = 2 };
+// BfsParams.flags of a direct launch.
+enum : int32_t {
+    BFS_PARAM_INLINE_L0 = 1,  // the clear kernel runs level 0 (the host starts at level 1)
+    BFS_PARAM_BU_NEXT = 2,    // ... and level 1 may go bottom-up: its winners also go to fb[1]
+};
+constexpr int kClearBlock = 256;
+constexpr uint32_t kInlineL0 = 128;  // largest source degree whose level 0 the clear kernel runs
 
 constexpr int kCompactBlock = 1024;
 constexpr int kCompactRounds = 16;                          // vertices per thread
@@ -1435,7 +1442,7 @@ __global__ void row_head_kernel(const int64_t* __restrict__ offsets, const int32
 //  * level 0 = the source alone: claimed, level 0, parent = itself, the only
 //    queue entry, and the control block reset.  The thread that clears an
 //    entry of the source also makes the source's store to it.
-__global__ void bfs_clear_kernel(BfsArgs a, BfsParams arg) {
+__global__ __launch_bounds__(kClearBlock) void bfs_clear_kernel(BfsArgs a, BfsParams arg) {
     // The call's parameters: `arg` for a direct launch, else (a replayed
     // sequence, whose arguments are fixed) the host-visible block, one
     // scalar read per wave.  The later kernels read the device copy made
@@ -1483,20 +1490,94 @@ __global__ void bfs_clear_kernel(BfsArgs a, BfsParams arg) {
             }
         }
     }
-    if (blockIdx.x != 0) return;
+    // Level 0 here, for a source of at most kInlineL0 neighbours on a
+    // one-launch (symmetric) search, as expand(0) would run it: its launch
+    // is a chain of dependent steps for one vertex.  Every workgroup reads
+    // the source's row; once its own clearing stores are done, it stores
+    // level 1 (and parent, visited and queued bits) for the neighbours whose
+    // entries and bitmap words it owns; workgroup 0 builds the queue of
+    // level 1 and the control block expand(0) would have left.
+    const bool inline_l0 = (prm.flags & BFS_PARAM_INLINE_L0) != 0;
+    __shared__ int32_t s_nb[kInlineL0];
+    __shared__ unsigned long long s_wave[kClearBlock / 64];
     const int64_t rb = a.offsets[s];
     const uint32_t d = static_cast<uint32_t>(a.offsets[s + 1] - rb);
+    bool win = false;
+    int32_t u = -1;
+    if (inline_l0) {
+        if (threadIdx.x < d && threadIdx.x < kInlineL0) u = a.cols[rb + threadIdx.x];
+        if (threadIdx.x < kInlineL0) s_nb[threadIdx.x] = u;
+        __threadfence_block();  // the clearing stores above come before the stores below
+        __syncthreads();
+        // a winner: a vertex, not the source, the first of its duplicates
+        win = d <= kInlineL0 && u >= 0 && u < a.n && u != s;
+        for (uint32_t j = 0; win && j < threadIdx.x; ++j)
+            if (s_nb[j] == u) win = false;
+        if (win) {
+            const int64_t li = vec ? (u >> 2) : u;  // the item whose thread cleared entry u
+            if ((li % stride) / blockDim.x == blockIdx.x) {
+                levels[u] = 1;
+                if (parents) parents[u] = s;
+            }
+            const int64_t w = u >> 5;
+            if ((w % stride) / blockDim.x == blockIdx.x) {
+                const uint32_t bit = 1u << (u & 31);
+                atomicOr(&a.vis[w], bit);
+                atomicOr(&a.prev[w], bit);
+                if (prm.flags & BFS_PARAM_BU_NEXT) atomicOr(&a.fb[1][w], bit);
+            }
+        }
+    }
+    if (blockIdx.x != 0) return;
+    unsigned long long l1_total = 0;
+    uint32_t l1_zero = 0;
+    if (inline_l0) {
+        int64_t urb = 0;
+        uint32_t ud = 0;
+        if (win) {
+            urb = a.offsets[u];
+            ud = static_cast<uint32_t>(a.offsets[u + 1] - urb);
+        }
+        const bool keep = win && ud > 0;
+        const unsigned long long ex =
+            block_exclusive_scan<kClearBlock>(keep ? ((1ull << 32) | ud) : 0ull, s_wave, l1_total);
+        if (keep) {
+            const uint32_t slot = static_cast<uint32_t>(ex >> 32), eo = static_cast<uint32_t>(ex);
+            a.q_rb[1][slot] = urb;
+            a.q_eoff[1][slot] = eo;
+            a.q_v[1][slot] = u;
+            mark_tiles(a.tile_start[1], a.tile_edges, eo, ud, slot);
+        }
+        l1_zero = static_cast<uint32_t>(__syncthreads_count(win && ud == 0));  // reached, never queued
+    }
     if (threadIdx.x == 0) {
         a.q_rb[0][0] = rb;
         a.q_eoff[0][0] = 0;
         a.q_v[0][0] = s;
+        const unsigned long long err = inline_l0 && d > kInlineL0 ? ERR_RANGE : 0ull;  // never asked for
         for (int i = 0; i < CTL_WORDS; ++i) a.ctl[i] = 0;
+        a.ctl[CTL_ERR] = err;
         a.ctl[CTL_RING0] = d ? ((1ull << 32) | d) : 0ull;
         a.ctl[CTL_NREACH] = d ? 0 : 1;  // expand(0) counts a source with edges
         a.ctl[CTL_REP0] = REP_QUEUE;
         a.ctl[CTL_BUFIRST] = ~0ull;
         a.ctl[CTL_TAILFROM] = ~0ull;
         a.ctl[CTL_T0] = t0;
+        if (inline_l0) {
+            // what expand(0) leaves: level 0 (F = 1, E = d) counted and
+            // top-down, level 1 queued
+            a.ctl[CTL_NREACH] = 1 + l1_zero;
+            a.ctl[CTL_MAXLEV] = l1_zero ? 1 : 0;
+            a.ctl[CTL_MAXE] = d;
+            a.ctl[CTL_DIR0] = DIR_TD;
+            a.ctl[CTL_REP0 + 1] = (prm.flags & BFS_PARAM_BU_NEXT) ? (REP_QUEUE | REP_BITMAP) : REP_QUEUE;
+            a.ctl[CTL_MREACH] = d;
+            a.ctl[CTL_EXP0 + 1] = d;
+            a.ctl[CTL_FCNT0] = a.ctl[CTL_RING0];
+            a.ctl[CTL_RCNT0] = 1;
+            a.ctl[CTL_TAILFROM] = (d <= kTailLearnEdges && 1 <= kTailCap) ? 0ull : ~0ull;
+            a.ctl[CTL_RING0 + 1] = l1_total;
+        }
     }
     for (uint64_t tt = threadIdx.x; tt * a.tile_edges < d; tt += blockDim.x) a.tile_start[0][tt] = 0;
 }
@@ -1544,7 +1625,12 @@ DeviceGraph::DeviceGraph(const HostGraph& g, int dev) : device(dev), n(g.n), m(g
     num_cus = prop.multiProcessorCount;
     cfg.dense_edges = default_dense_edges();
     symmetric = is_symmetric(g);
-    for (int64_t v = 0; v < n; ++v) n_edged += g.offsets[v + 1] > g.offsets[v] ? 1 : 0;
+    deg8.resize(static_cast<size_t>(n));
+    for (int64_t v = 0; v < n; ++v) {
+        const int64_t dv = g.offsets[v + 1] - g.offsets[v];
+        n_edged += dv > 0 ? 1 : 0;
+        deg8[v] = static_cast<uint8_t>(std::min<int64_t>(dv, 255));
+    }
     try {
         GG_HIP(hipMalloc(&offsets, sizeof(int64_t) * (n + 1)));
         GG_HIP(hipMalloc(&cols, sizeof(int32_t) * std::max<int64_t>(m, 1)));
@@ -1827,14 +1913,15 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
     // reaches it (the previous call has ended, so nothing reads the block
     // meanwhile).
     ws.h_params->source = source;
-    ws.h_params->pad = 0;
+    ws.h_params->flags = 0;
     ws.h_params->levels = d_levels;
     ws.h_params->parents = d_parents;
     // A captured launch takes its parameters from ws.h_params when it runs.
-    auto launch_prologue = [&](bool captured) {
+    auto launch_prologue = [&](bool captured, int32_t flags = 0) {
         BfsParams arg = *ws.h_params;
         if (captured) arg.levels = nullptr;
-        hipLaunchKernelGGL(bfs_clear_kernel, dim3(clear_grid), dim3(256), 0, stream, a, arg);
+        arg.flags = flags;
+        hipLaunchKernelGGL(bfs_clear_kernel, dim3(clear_grid), dim3(kClearBlock), 0, stream, a, arg);
     };
 
     int L = 0;
@@ -2009,8 +2096,16 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
             // (about 4 us per search); the device time comes from the clock
             // stamps of the first and the last kernel instead.
             clock_timed = true;
-            launch_prologue(false);
-            enqueue(D, 0, tail_at);
+            // A source of at most kInlineL0 neighbours: the clear kernel runs
+            // level 0 (on one-launch searches, where it is top-down, not
+            // dense, and not the tail's), and the launches start at level 1.
+            const uint32_t ds = g.deg8[static_cast<size_t>(source)];
+            const uint32_t dense_min0 = (1 <= a.grow_until) ? kGrowDenseEdges : a.dense_edges;
+            const bool inline_l0 = a.fused && tail_at != 0 && ds >= 1 && ds <= kInlineL0 &&
+                                   !(ds > dense_min0 && bu_at(1));
+            launch_prologue(false, inline_l0 ? (BFS_PARAM_INLINE_L0 | (bu_at(1) ? BFS_PARAM_BU_NEXT : 0)) : 0);
+            if (inline_l0) L = 1;
+            enqueue(inline_l0 ? D - 1 : D, 0, tail_at);
         }
         timed_traversal = false;
         if (tail_at >= 0 && !done(0)) {

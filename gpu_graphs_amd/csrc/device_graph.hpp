@@ -13,6 +13,7 @@
 #include <cstdint>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "common.hpp"
 #include "host_graph.hpp"
@@ -58,7 +59,7 @@ struct BfsConfig {
 // device memory so that one captured launch sequence serves every call.
 struct BfsParams {
     int32_t source;
-    int32_t pad;
+    int32_t flags;     // BFS_PARAM_* (direct launches only; a replayed graph reads 0)
     int32_t* levels;
     int32_t* parents;  // may be null
 };
@@ -245,6 +246,7 @@ struct DeviceGraph {
     int device = 0;
     int64_t n = 0, m = 0;
     int64_t n_edged = 0;  // vertices with at least one CSR entry
+    std::vector<uint8_t> deg8;  // host: each vertex's degree, clipped at 255
     int64_t* offsets = nullptr;  // n + 1
     int32_t* cols = nullptr;     // m
     uint32_t* iso = nullptr;     // ceil(n/32) words: degree-0 vertices (symmetric graphs only)

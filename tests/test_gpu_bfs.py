@@ -476,3 +476,74 @@ def test_scan_known_answer(gpu):
     # 1D Partition/misc/test_prefix.cu:91-93
     assert gg.exclusive_scan(np.arange(16), gpu).tolist() == [0, 0, 1, 3, 6, 10, 15, 21, 28, 36, 45, 55, 66, 78,
                                                                91, 105]
+
+
+class _View:
+    """A device pointer into another allocation (at an element offset)."""
+
+    def __init__(self, base, offset):
+        import ctypes
+        self.ptr = ctypes.c_void_p(base.ptr.value + offset * base.dtype.itemsize)
+
+
+def test_level_zero_in_the_clear_kernel(gpu):
+    # Direct launches on symmetric graphs run level 0 inside the clear kernel
+    # when the source has at most 128 CSR entries: sources of 127, 128 and
+    # 129 entries, a row with every neighbour twice and a self-loop (rows in
+    # file order), outputs at each 4-byte offset (16-byte stores or entry
+    # stores), with and without parents, with the tail off, at level 1 and
+    # learned; the replayed and level-loop modes for the same sources.
+    rng = np.random.default_rng(91)
+    base = gg.Graph.rmat(12, seed=91)
+    n0 = base.n
+    rows = np.repeat(np.arange(n0), np.diff(base.offsets))
+    one_way = rows < base.cols
+    src = [rows[one_way].astype(np.int64)]
+    dst = [base.cols[one_way].astype(np.int64)]
+    specials = []
+    nxt = n0
+    for d in (127, 128, 129, 3):
+        nb = rng.choice(n0, d, replace=False)
+        src.append(np.full(d, nxt))
+        dst.append(nb)
+        specials.append(nxt)
+        nxt += 1
+    dup = nxt
+    nxt += 1
+    nb = rng.choice(n0, 40, replace=False)
+    src += [np.full(81, dup)]
+    dst += [np.concatenate([nb, nb, [dup]])]
+    specials.append(dup)
+    G = gg.Graph.from_edges(nxt, np.concatenate(src).astype(np.int32), np.concatenate(dst).astype(np.int32),
+                            flags=gg.GB_SYMMETRIZE | gg.GB_KEEP_ORDER)
+    assert G.is_symmetric
+    deg = np.diff(G.offsets)
+    assert [int(deg[v]) for v in specials] == [127, 128, 129, 3, 82]
+    cand = np.nonzero(deg)[0]
+    sources = specials + [int(x) for x in rng.choice(cand, 8, replace=False)]
+    want = {s: O.bfs(G.offsets, G.cols, s)[0] for s in sources}
+    dg = gg.DeviceGraph(G, gpu)
+    lv_buf = gg.DeviceArray(gpu, np.int32, G.n + 8)
+    par_buf = gg.DeviceArray(gpu, np.int32, G.n + 8)
+    for tail in (gg.GB_TAIL_OFF, 1, gg.GB_TAIL_AUTO):
+        dg.tail_config(tail)
+        for parents in (False, True):
+            for off in range(4):
+                log(f"level 0 in the clear: tail {tail} parents {parents} offset {off}")
+                for s in sources:
+                    lv_buf.upload(np.full(lv_buf.count, 0x5A5A5A5A, dtype=np.int32))
+                    dg.bfs_device(s, _View(lv_buf, off), _View(par_buf, off) if parents else None)
+                    got = lv_buf.numpy()
+                    np.testing.assert_array_equal(got[off:off + G.n], want[s], err_msg=f"source {s}")
+                    assert (got[:off] == 0x5A5A5A5A).all() and (got[off + G.n:] == 0x5A5A5A5A).all()
+                    if parents:
+                        par = par_buf.numpy()[off:off + G.n]
+                        assert O.check_parents(G.offsets, G.cols, s, want[s], par) == 0, s
+    dg.tail_config(gg.GB_TAIL_AUTO)
+    for mode in (gg.GB_MODE_GRAPH, gg.GB_MODE_LEVEL_LOOP):
+        cfg = dg.get_config()
+        cfg["mode"] = mode
+        dg.config(**cfg)
+        for s in specials:
+            lv, _, _ = dg.bfs(s)
+            np.testing.assert_array_equal(lv, want[s], err_msg=f"mode {mode} source {s}")
