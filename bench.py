@@ -326,6 +326,9 @@ def single_source(gg, args):
             "hbm_gbs": round(hbm_gbs, 2) if hbm_gbs else None,
             "hbm_frac": round(hbm_gbs / HBM_PEAK_GBS, 4) if hbm_gbs else None,
             "traffic_note": prof_note,
+            "achieved_note": "achieved = alg_gbs: B(s) = 4n + 28 n_R + 8 m_R bytes per search over its device time; "
+                             "bottom-up levels read fewer bytes than B(s) charges, so frac can exceed 1. "
+                             "hbm_gbs / hbm_frac divide the counter-measured traffic by the same time.",
             "kernel": "one BFS: its launch sequence on the handle's stream (bfs_clear_kernel, which seeds "
                       "the source, one bfs_expand_kernel per level, top-down or bottom-up, and bfs_tail_kernel "
                       "for the small last levels; bfs_compact_kernel only on directed graphs), timed from the "
@@ -529,7 +532,7 @@ def served(gg, args, g24=None, dg24=None):
         return {"offered_qps": round(rate, 1) if rate else "saturated", "queries": nq,
                 "qps": round(st["qps"], 1), "p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2),
                 "mean_batch": round(st["mean_batch"], 2), "failed": len(failures),
-                "memory_used": st["memory_used"], "per_graph": per_graph}
+                "memory_used": st["memory_used"], "workspace_drops": st["workspace_drops"], "per_graph": per_graph}
 
     out = {"graphs": [{"name": s[0], "n": g.n, "m_directed": g.m} for s, g in zip(specs, gs)],
            "seed": SEED_SERVE, "memory_budget": args.serve_budget}

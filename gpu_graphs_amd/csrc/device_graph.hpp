@@ -121,7 +121,8 @@ struct MsParams {
 struct MsWorkspace {
     unsigned long long* sn = nullptr;     // 2n: per vertex, sources that reached it and sources reaching it now
     unsigned long long* fmask = nullptr;  // n: copy of seen, stored when it changes (gathered bottom-up)
-    unsigned long long* fmask2 = nullptr; // n: the other copy (bottom-up levels that update inline write it)
+    unsigned long long* fmask2 = nullptr; // n: the other copy (bottom-up levels that update inline write it);
+                                          // next1's memory, not freed on its own
     unsigned long long* lv8 = nullptr;    // 8n: one level byte per (vertex, source)
     unsigned long long* reached = nullptr;  // ceil(n/64): vertices whose lv8 line the batch has written
     int64_t* q_rb = nullptr;              // queue of active vertices

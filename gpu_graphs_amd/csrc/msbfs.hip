@@ -1385,7 +1385,7 @@ void build_ms_graph(DeviceGraph& dg, const HostGraph& g) {
 
 void Runner::free_ms_workspace() {
     if (ms.graph_exec) (void)hipGraphExecDestroy(ms.graph_exec);
-    void* ptrs[] = {ms.sn,      ms.fmask,      ms.fmask2,   ms.lv8,     ms.reached, ms.q_rb,   ms.q_eoff, ms.q_v,
+    void* ptrs[] = {ms.sn,      ms.fmask,      ms.lv8,     ms.reached, ms.q_rb,   ms.q_eoff, ms.q_v,
                     ms.q_fm,    ms.tile_start, ms.touched, ms.ctl,     ms.params, ms.levels, ms.partial,
                     ms.touched2, ms.next1};
     for (void* p : ptrs)
@@ -1411,7 +1411,6 @@ void Runner::ensure_ms_workspace() {
     try {
         alloc(&ms.sn, sizeof(SeenNext) * n);
         alloc(&ms.fmask, sizeof(unsigned long long) * n);
-        alloc(&ms.fmask2, sizeof(unsigned long long) * n);
         alloc(&ms.lv8, sizeof(unsigned long long) * 8 * n);
         alloc(&ms.reached, sizeof(unsigned long long) * ((n + 63) / 64));
         alloc(&ms.q_rb, sizeof(int64_t) * n);
@@ -1422,6 +1421,10 @@ void Runner::ensure_ms_workspace() {
         alloc(&ms.touched, sizeof(int32_t) * n);
         alloc(&ms.touched2, sizeof(int32_t) * n);
         alloc(&ms.next1, sizeof(unsigned long long) * n);
+        // The second mask copy shares next1's memory: only fused deep batches
+        // use next1 and only the others fmask2; the clear zeroes next1 at the
+        // start of every batch, and fmask2 is written whole before it is read.
+        ms.fmask2 = ms.next1;
         alloc(&ms.ctl, sizeof(unsigned long long) * kCtlWords);
         alloc(&ms.params, sizeof(MsParams));
         alloc(&ms.partial, sizeof(unsigned long long) * kStatsBlocks * 64 * 2);
@@ -1448,11 +1451,11 @@ void Runner::prepare_batch() {
     ensure_ms_workspace();
 }
 
-// What ensure_ms_workspace() allocates: per vertex 14 words (sn 2, fmask and
-// fmask2, lv8 8, q_fm, next1), q_rb, and four 4-byte arrays (q_eoff, q_v,
+// What ensure_ms_workspace() allocates: per vertex 13 words (sn 2, fmask,
+// lv8 8, q_fm, next1 = fmask2), q_rb, and four 4-byte arrays (q_eoff, q_v,
 // touched, touched2); the reached bitmap; tile indices; the statistics.
 size_t batch_bytes_estimate(int64_t n_vertices, int64_t m_entries, int k, bool host_levels) {
-    size_t b = sizeof(unsigned long long) * n_vertices * 14 + (sizeof(int64_t) + 16) * n_vertices +
+    size_t b = sizeof(unsigned long long) * n_vertices * 13 + (sizeof(int64_t) + 16) * n_vertices +
                sizeof(unsigned long long) * ((n_vertices + 63) / 64) +
                sizeof(uint32_t) * (m_entries / 256 + 2) + 1024 * 64 * 16 + 4096;
     if (host_levels) b += sizeof(int32_t) * n_vertices * k;
