@@ -1456,6 +1456,19 @@ __global__ __launch_bounds__(kClearBlock) void bfs_clear_kernel(BfsArgs a, BfsPa
     const int64_t sw = static_cast<uint32_t>(s) >> 5;
     const uint32_t sbit = 1u << (static_cast<uint32_t>(s) & 31u);
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    // Level 0 in this kernel (below): the source's row, and in workgroup 0
+    // the row bounds of its neighbours, are loaded before the clearing
+    // stores, so that their latency overlaps them.
+    const bool inline_l0 = (prm.flags & BFS_PARAM_INLINE_L0) != 0;
+    const int64_t rb = a.offsets[s];
+    const uint32_t d = static_cast<uint32_t>(a.offsets[s + 1] - rb);
+    int32_t u = -1;
+    if (inline_l0 && threadIdx.x < d && threadIdx.x < kInlineL0) u = a.cols[rb + threadIdx.x];
+    int64_t urb = 0, ure = 0;
+    if (inline_l0 && blockIdx.x == 0 && u >= 0 && u < a.n) {
+        urb = a.offsets[u];
+        ure = a.offsets[u + 1];
+    }
     // Thread i clears entries 4i..4i+3 with 16-byte stores when the output
     // rows allow it (entry i otherwise), and bitmap word i.
     const bool vec = ((reinterpret_cast<uintptr_t>(levels) | reinterpret_cast<uintptr_t>(parents)) & 15u) == 0;
@@ -1497,15 +1510,10 @@ __global__ __launch_bounds__(kClearBlock) void bfs_clear_kernel(BfsArgs a, BfsPa
     // level 1 (and parent, visited and queued bits) for the neighbours whose
     // entries and bitmap words it owns; workgroup 0 builds the queue of
     // level 1 and the control block expand(0) would have left.
-    const bool inline_l0 = (prm.flags & BFS_PARAM_INLINE_L0) != 0;
     __shared__ int32_t s_nb[kInlineL0];
     __shared__ unsigned long long s_wave[kClearBlock / 64];
-    const int64_t rb = a.offsets[s];
-    const uint32_t d = static_cast<uint32_t>(a.offsets[s + 1] - rb);
     bool win = false;
-    int32_t u = -1;
     if (inline_l0) {
-        if (threadIdx.x < d && threadIdx.x < kInlineL0) u = a.cols[rb + threadIdx.x];
         if (threadIdx.x < kInlineL0) s_nb[threadIdx.x] = u;
         __threadfence_block();  // the clearing stores above come before the stores below
         __syncthreads();
@@ -1532,12 +1540,7 @@ __global__ __launch_bounds__(kClearBlock) void bfs_clear_kernel(BfsArgs a, BfsPa
     unsigned long long l1_total = 0;
     uint32_t l1_zero = 0;
     if (inline_l0) {
-        int64_t urb = 0;
-        uint32_t ud = 0;
-        if (win) {
-            urb = a.offsets[u];
-            ud = static_cast<uint32_t>(a.offsets[u + 1] - urb);
-        }
+        const uint32_t ud = win ? static_cast<uint32_t>(ure - urb) : 0u;
         const bool keep = win && ud > 0;
         const unsigned long long ex =
             block_exclusive_scan<kClearBlock>(keep ? ((1ull << 32) | ud) : 0ull, s_wave, l1_total);
