@@ -112,6 +112,7 @@ enum : unsigned long long { DIR_TD = 1, DIR_BU = 2 };
 enum : int32_t {
     BFS_PARAM_INLINE_L0 = 1,  // the clear kernel runs level 0 (the host starts at level 1)
     BFS_PARAM_BU_NEXT = 2,    // ... and level 1 may go bottom-up: its winners also go to fb[1]
+    BFS_PARAM_ROWS_UNIQUE = 4,  // ... and no row repeats a neighbour (no duplicate check)
 };
 constexpr int kClearBlock = 256;
 constexpr uint32_t kInlineL0 = 128;  // largest source degree whose level 0 the clear kernel runs
@@ -1519,8 +1520,9 @@ __global__ __launch_bounds__(kClearBlock) void bfs_clear_kernel(BfsArgs a, BfsPa
         __syncthreads();
         // a winner: a vertex, not the source, the first of its duplicates
         win = d <= kInlineL0 && u >= 0 && u < a.n && u != s;
-        for (uint32_t j = 0; win && j < threadIdx.x; ++j)
-            if (s_nb[j] == u) win = false;
+        if (!(prm.flags & BFS_PARAM_ROWS_UNIQUE))
+            for (uint32_t j = 0; win && j < threadIdx.x; ++j)
+                if (s_nb[j] == u) win = false;
         if (win) {
             const int64_t li = vec ? (u >> 2) : u;  // the item whose thread cleared entry u
             if ((li % stride) / blockDim.x == blockIdx.x) {
@@ -1628,6 +1630,7 @@ DeviceGraph::DeviceGraph(const HostGraph& g, int dev) : device(dev), n(g.n), m(g
     num_cus = prop.multiProcessorCount;
     cfg.dense_edges = default_dense_edges();
     symmetric = is_symmetric(g);
+    rows_unique = !(g.flags & GB_KEEP_ORDER);
     deg8.resize(static_cast<size_t>(n));
     for (int64_t v = 0; v < n; ++v) {
         const int64_t dv = g.offsets[v + 1] - g.offsets[v];
@@ -2106,7 +2109,9 @@ void Runner::bfs(int32_t source, int32_t* d_levels, int32_t* d_parents, gb_stats
             const uint32_t dense_min0 = (1 <= a.grow_until) ? kGrowDenseEdges : a.dense_edges;
             const bool inline_l0 = a.fused && tail_at != 0 && ds >= 1 && ds <= kInlineL0 &&
                                    !(ds > dense_min0 && bu_at(1));
-            launch_prologue(false, inline_l0 ? (BFS_PARAM_INLINE_L0 | (bu_at(1) ? BFS_PARAM_BU_NEXT : 0)) : 0);
+            launch_prologue(false, inline_l0 ? (BFS_PARAM_INLINE_L0 | (bu_at(1) ? BFS_PARAM_BU_NEXT : 0) |
+                                                (g.rows_unique ? BFS_PARAM_ROWS_UNIQUE : 0))
+                                             : 0);
             if (inline_l0) L = 1;
             enqueue(inline_l0 ? D - 1 : D, 0, tail_at);
         }

@@ -248,6 +248,7 @@ struct DeviceGraph {
     int64_t n = 0, m = 0;
     int64_t n_edged = 0;  // vertices with at least one CSR entry
     std::vector<uint8_t> deg8;  // host: each vertex's degree, clipped at 255
+    bool rows_unique = true;    // no row repeats a neighbour (built without GB_KEEP_ORDER)
     int64_t* offsets = nullptr;  // n + 1
     int32_t* cols = nullptr;     // m
     uint32_t* iso = nullptr;     // ceil(n/32) words: degree-0 vertices (symmetric graphs only)
