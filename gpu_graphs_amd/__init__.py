@@ -305,12 +305,14 @@ class DeviceGraph:
         check(lib().gb_bfs(self._h, int(source), _ptr(levels), _ptr(par), C.byref(st)))
         return levels, par, st.as_dict()
 
-    def bfs_device(self, source, d_levels, d_parents=None):
-        """Device-pointer BFS: results stay in HBM (DeviceArray outputs)."""
+    def bfs_device(self, source, d_levels, d_parents=None, raw=False):
+        """Device-pointer BFS: results stay in HBM (DeviceArray outputs).
+        Returns the statistics as a dict, or (raw=True) as the gb_stats
+        structure itself (no conversion inside a timed loop)."""
         st = Stats()
         check(lib().gb_bfs_device(self._h, int(source), d_levels.ptr, d_parents.ptr if d_parents else None,
                                   C.byref(st)))
-        return st.as_dict()
+        return st if raw else st.as_dict()
 
     def batch_config(self, block=256, edges_per_thread=2, use_graph=True, sparse_edges=0):
         check(lib().gb_batch_config(self._h, block, edges_per_thread, int(use_graph), int(sparse_edges)))
