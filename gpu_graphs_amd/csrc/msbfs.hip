@@ -1067,9 +1067,30 @@ __global__ __launch_bounds__(kMsDeepBlock) void ms_deep_kernel(MsArgs a, int L) 
         }
         return static_cast<uint32_t>(base) + incl - c;
     };
-    for (uint64_t i = gtid;; i += gstride) {
+    // The same for the whole workgroup (every thread calls it): one block
+    // scan and one atomicAdd.  A per-wave reservation on the one counter
+    // serialises when a level touches many vertices (64-source batches).
+    __shared__ uint32_t s_cnt[kMsDeepBlock / 64];
+    __shared__ unsigned long long s_base;
+    auto reserve_block = [&](uint32_t c) -> uint32_t {
+        uint32_t tot;
+        const uint32_t excl = block_exclusive_scan<kMsDeepBlock>(c, s_cnt, tot);
+        if (threadIdx.x == 0) s_base = tot ? atomicAdd(cnt_next, static_cast<unsigned long long>(tot)) : 0ull;
+        __syncthreads();
+        const unsigned long long base = s_base;
+        __syncthreads();  // s_base is written again by the next call
+        if (tot == 0) return 0u;
+        if (base + tot > n) {
+            bad = true;
+            return ~0u;
+        }
+        return static_cast<uint32_t>(base) + excl;
+    };
+    // Workgroup-uniform steps: the workgroup's threads hold consecutive
+    // entries [base, base + kMsDeepBlock).
+    for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * kMsDeepBlock; base < F; base += gstride) {
+        const uint64_t i = base + threadIdx.x;
         const bool has = i < F;
-        if (!wave_ballot(has)) break;  // the lanes of a wave hold consecutive entries
         int64_t v = -1;
         if (has) {
             const int32_t t = i == gtid ? v_first : tin[i];
@@ -1144,7 +1165,7 @@ __global__ __launch_bounds__(kMsDeepBlock) void ms_deep_kernel(MsArgs a, int L) 
         for (int j = 0; j < kMsDeepRow; ++j)
             if (mk[j] && __hip_atomic_fetch_or(next_out(u[j]), mk[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
                 claim |= 1u << j;
-        const uint32_t slot0 = reserve(static_cast<uint32_t>(__popc(claim)));
+        const uint32_t slot0 = reserve_block(static_cast<uint32_t>(__popc(claim)));
         if (claim && slot0 != ~0u) {
             uint32_t r = 0;
 #pragma unroll
