@@ -893,7 +893,6 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(MsArgs a, int L) {
         return;
     }
     const uint32_t ntiles = static_cast<uint32_t>((uint64_t(E) + T - 1) / T);
-    const uint32_t lane = threadIdx.x & 63u;
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint32_t tid = threadIdx.x;
         const uint32_t base = t * T;
@@ -981,19 +980,27 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(MsArgs a, int L) {
                         first[k] = __hip_atomic_fetch_or(&a.sn[v[k]].next, mk, __ATOMIC_RELAXED,
                                                          __HIP_MEMORY_SCOPE_AGENT) == 0;
                 }
-                // Wave-aggregated appends: one counter atomic per wave and k.
+                // Workgroup-aggregated appends: one counter atomic per tile
+                // (one per wave and k serialised on the counter when a
+                // sparse level touches millions of vertices).  The scan's
+                // words reuse the tile's LDS once every wave is past it.
+                uint32_t mine = 0;
+#pragma unroll
+                for (int k = 0; k < K; ++k) mine += first[k] ? 1u : 0u;
+                __syncthreads();
+                uint32_t* const s_scan = reinterpret_cast<uint32_t*>(lds.hub);
+                unsigned long long* const s_pos = lds.hub + BLOCK / 64;
+                uint32_t tot;
+                const uint32_t excl = block_exclusive_scan<BLOCK>(mine, s_scan, tot);
+                if (tid == 0) *s_pos = tot ? atomicAdd(&a.ctl[MS_TCNT0 + npar], static_cast<unsigned long long>(tot)) : 0ull;
+                __syncthreads();
+                unsigned long long slot = *s_pos + excl;
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    const uint64_t fb = wave_ballot(first[k]);
-                    if (!fb) continue;
-                    unsigned long long pos = 0;
-                    if (lane == __ffsll(static_cast<long long>(fb)) - 1)
-                        pos = atomicAdd(&a.ctl[MS_TCNT0 + npar], static_cast<unsigned long long>(__popcll(fb)));
-                    pos = __shfl(pos, __ffsll(static_cast<long long>(fb)) - 1, 64);
                     if (first[k]) {
-                        const unsigned long long slot = pos + rank_below(fb);
                         if (slot < static_cast<unsigned long long>(a.n)) a.touched[slot] = v[k];
                         else atomicOr(&a.ctl[MS_ERR], MS_ERR_RANGE);
+                        ++slot;
                     }
                 }
             }
