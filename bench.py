@@ -604,7 +604,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--batch-reps", type=int, default=5)
-    ap.add_argument("--serve-budget", type=int, default=64 << 30,
+    ap.add_argument("--serve-budget", type=int, default=128 << 30,
                     help="device bytes for the batched server's workspaces and result buffers")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="arrival window of each swept rate")
     ap.add_argument("--serve-plain-queries", type=int, default=96)
