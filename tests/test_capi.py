@@ -1,11 +1,13 @@
 """The C-ABI library loads and exports every entry point include/*.h declares
 (no GPU needed), errors come back as status codes with messages, and the
-CLI surface `graphBFS <input_file> <source_num>` parses its arguments."""
+CLI surface `graphBFS <input_file> <source_num>` parses its arguments; bench.py
+refuses a run on more than one GPU."""
 import ctypes as C
 import glob
 import os
 import re
 import subprocess
+import sys
 
 import pytest
 
@@ -69,3 +71,14 @@ def test_facade_program_usage():
     assert os.path.exists(exe), "facade_main not built"
     r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 1 and "Usage" in r.stderr
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # The bench measures one GPU; asked for more (or started as one rank of
+    # several) it exits with status 2 and says why, before touching a device.
+    bench = os.path.join(ROOT, "bench.py")
+    r = subprocess.run([sys.executable, bench, "--gpus", "2"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 2 and "one GPU only" in r.stderr
+    env = dict(os.environ, WORLD_SIZE="2")
+    r = subprocess.run([sys.executable, bench], capture_output=True, text=True, timeout=300, env=env)
+    assert r.returncode == 2 and "one GPU only" in r.stderr
